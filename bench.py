@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the D2Q9 hot path on one MI355X.
+
+Default workload (the headline configuration): lid-driven cavity with
+4097 x 4097 fluid nodes in the reference mesh convention, Re = 1000 on L = N-1
+(nu = 0.4096, tau = 1.7288), MRT with rates s = (0, 1.4, 1.4, 0, 1.2, 0, 1.2,
+1/tau, 1/tau) and textbook equilibria, FP32 storage of f - w_k rho0, rest start,
+lid u = 0.1.  A step is one fused pass over the lattice (pull-stream,
+boundaries, macroscopic update, collision) — the reference loop body.
+
+Prints ONE JSON line: MLUPS = fluid nodes x steps / wall seconds / 1e6, the
+HBM-roofline fraction of the dominant (bulk) kernel from HIP events recorded
+around each of its launches on the solver's stream, and the CPU oracle timed on
+a bounded sample of the same workload (1 core).
+
+  python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+                  [--N 4097] [--collision mrt|bgk|trt] [--precision f32|f64|f16]
+                  [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS (% HBM roofline) at 4097² MRT; time-to-1e-6 residual, learned vs rest"
+HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "bulk_traffic_pmc.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(N, collision, visc, rates, steps=3):
+    """The oracle (C restatement of the reference serial solver: 528-byte AoS
+    cells, six sweeps per step, FP64) on 1 core, same workload, `steps` timed
+    iterations after one untimed one."""
+    import numpy as np  # noqa: F401
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import meshgen
+    import oracle as O
+    nodes, conn = meshgen.cavity_arrays_fast(N)
+    kw = dict(collision=collision, init_rest=1, uavg=0.1, rho_ini=1.0, viscosity=visc,
+              resid_kind=3)
+    if collision == 3:
+        kw.update(mrt_textbook=1, mrt_s=rates)
+    if collision == 2:
+        kw.update(trt_lambda=0.25, trt_fp64=1)
+    p = O.params(**kw)
+    o = O.Oracle(nodes, conn, p)
+    del nodes, conn
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    nf = N * N
+    return {"value": round(nf * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"{N}x{N} cavity, same collision/rates, FP64, {steps} timed steps after 1 "
+                      f"untimed (oracle/lbm_oracle.c, 1 thread, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--N", type=int, default=4097)
+    ap.add_argument("--collision", default="mrt", choices=["bgk", "trt", "mrt"])
+    ap.add_argument("--precision", default="f32", choices=["f64", "f32", "f16"])
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU slab decomposition is a design note, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import parallellbmcranfield_amd as lbm
+
+    N = args.N
+    coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
+    prec = {"f64": lbm.PREC_F64, "f32": lbm.PREC_F32, "f16": lbm.PREC_F16}[args.precision]
+    visc = 0.1 * (N - 1) / 1000.0  # Re = 1000 on L = N - 1, lid 0.1
+    tau = 3 * float(__import__("numpy").float32(visc)) + 0.5
+    rates = [0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, 1 / tau, 1 / tau]
+    kw = dict(collision=coll, uavg=0.1, rho_ini=1.0, viscosity=visc, init=lbm.INIT_REST,
+              resid_kind=lbm.RESID_NONE, use_stop_rule=0, block_size=args.block)
+    if coll == lbm.MRT:
+        kw.update(mrt_textbook_meq=1, mrt_rates=rates)
+    if coll == lbm.TRT:
+        kw.update(trt_lambda=0.25, trt_fp64_parts=1)
+    log(f"bench: building case {N}x{N} ...")
+    t = time.perf_counter()
+    case = lbm.Case.cavity(N)
+    opts = lbm.make_options(**kw)
+    opts.precision = prec
+    s = lbm.Solver(case, opts)
+    log(f"bench: set-up {time.perf_counter() - t:.1f} s, device memory "
+        f"{s.device_bytes() / 2**30:.2f} GiB, fluid nodes {case.info.num_fluid}, "
+        f"boundary-list nodes {case.info.num_boundary}")
+    s.step(args.warmup)
+    s.sync()
+    s.sync()
+    t0 = time.perf_counter()
+    total_ms, bulk_ms = s.step_timed(args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    nf = case.info.num_fluid
+    n_bulk = nf - case.info.num_boundary
+    mlups = nf * args.steps / wall / 1e6
+    esize = {lbm.PREC_F64: 8, lbm.PREC_F32: 4, lbm.PREC_F16: 2}[prec]
+    bytes_per_launch = 2 * 9 * esize * n_bulk
+    avg_bulk_s = bulk_ms / 1e3 / args.steps
+    achieved = bytes_per_launch / avg_bulk_s / 1e9
+    traffic = None
+    if os.path.exists(PMC_SUMMARY):
+        try:
+            pm = json.load(open(PMC_SUMMARY))
+            key = f"{args.collision}_{args.precision}_{N}"
+            if key in pm.get("kernels", {}):
+                traffic = pm["kernels"][key]["hbm_bytes_per_launch"]
+        except (OSError, ValueError, KeyError):
+            traffic = None
+    log(f"bench: {args.steps} steps in {wall * 1e3:.1f} ms (stream {total_ms:.1f} ms), "
+        f"bulk kernel avg {avg_bulk_s * 1e6:.1f} us -> {achieved:.0f} GB/s, {mlups:.0f} MLUPS")
+    s.close()
+    del case
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU oracle on a bounded sample (1 core) ...")
+        cpu = cpu_baseline(N, coll, visc, rates, steps=args.cpu_steps)
+        log(f"bench: cpu baseline {cpu['value']} MLUPS")
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32"}[args.precision],
+        "data": "synthetic",
+        "config": {
+            "workload": f"lid-driven cavity {N}x{N} fluid nodes (reference mesh convention), "
+                        f"D2Q9 {args.collision.upper()}, Re=1000, lid u=0.1, rest start, "
+                        f"{args.precision} storage" + (" of f - w*rho0" if prec else ""),
+            "fluid_nodes": nf,
+            "bulk_nodes": n_bulk,
+            "collision": args.collision,
+            "precision": args.precision,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_bulk",
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": round(avg_bulk_s * 1e6, 2),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,216 @@
+/*
+ * lbm.h — C ABI of the MI355X D2Q9 lattice-Boltzmann engine.
+ *
+ * Drop-in boundary for the serial solver path of mate-szoke/ParallelLbmCranfield.
+ * The reference has no library interface: its path sits behind the process
+ * contract of serial/main.c (SetUpData.ini, Mesh/D2node.dat,
+ * Mesh/BCconnectors.dat in; Results/ out) and the function-level interface
+ * of serial/include/Iterate.h:8-27 (Iteration plus CollisionStep,
+ * StreamingStep, HandleBoundariesStep, UpdateMacroscopicStep) and
+ * serial/include/ComputeResiduals.h:6-7.  Each entry point below names what it
+ * replaces.
+ *
+ * Conventions: plain C types, caller-owned host buffers, library-owned device
+ * memory.  Host arrays of per-cell values use the reference's storage order,
+ * idx = j*n + i (serial/Iterate.c:499, CellFunctions.c:128), n cells in x and
+ * m in y; per-direction values are cell-major, [idx*9 + k], with the
+ * reference's direction numbering (serial/CellFunctions.c:281-289).
+ * Every call returns an lbm_status; nothing calls exit().  A solver handle is
+ * not thread-safe; different handles are independent (each owns a HIP stream).
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+typedef enum lbm_status {
+  LBM_OK = 0,
+  LBM_EIO = 1,       /* file cannot be opened / written (FileIO.c:48-52 printed and returned NULL) */
+  LBM_EPARSE = 2,    /* malformed ini or mesh */
+  LBM_DIVERGED = 3,  /* residual became NaN (ComputeResiduals.c:47-51 printed DIVERGENCE! and exit(1)) */
+  LBM_EHIP = 4,      /* HIP runtime error (no device, launch failure, ...) */
+  LBM_EINVAL = 5,    /* invalid argument or option combination */
+  LBM_ENOMEM = 6
+} lbm_status;
+
+/* Collision models, boundary switches: the reference's ini codes
+ * (serial/Iterate.c:127-156, cuda/cuda_step3/include/Arguments.h:12-48). */
+enum { LBM_BGKW = 1, LBM_TRT = 2, LBM_MRT = 3 };
+enum { LBM_INLET_PARABOLIC = 1, LBM_INLET_UNIFORM = 2, LBM_INLET_PULSATILE = 3 };
+enum { LBM_CURVED_ON = 1, LBM_CURVED_OFF = 2 };
+enum { LBM_OUTLET_VELOCITY = 1, LBM_OUTLET_OPEN2 = 2, LBM_OUTLET_OPEN1 = 3 };
+enum { LBM_FORMAT_CSV = 1, LBM_FORMAT_TECPLOT = 2 };
+
+/* Storage / arithmetic of the distributions. */
+enum {
+  LBM_PREC_F64 = 0, /* FP64 f, FP64 arithmetic in the reference's evaluation order */
+  LBM_PREC_F32 = 1, /* FP32 storage of f - w_k*rho0, FP32 arithmetic */
+  LBM_PREC_F16 = 2  /* FP16 storage of (f - w_k*rho0)*scale, FP32 arithmetic */
+};
+/* Initial state. */
+enum {
+  LBM_INIT_REFERENCE = 0, /* F = 0, U = Uo, V = 0, Rho = rho_ini (CellFunctions.c:148-150,267) */
+  LBM_INIT_REST = 1,      /* f = w*rho_ini, u = 0 */
+  LBM_INIT_FIELD = 2      /* f = feq(rho, u, v) of a caller field (lbm_solver_create_field) */
+};
+/* Residual definitions. */
+enum {
+  LBM_RESID_REFERENCE = 0, /* ComputeResiduals.c:6-51 as written (first m cells, n times each), every step */
+  LBM_RESID_FULL = 1,      /* L2(F - METAF) over every cell (cuda_step3 ComputeResiduals.cu:12-47) */
+  LBM_RESID_VELOCITY = 2,  /* ||u^t - u^{t-K}|| / (K ||u^t||) and the same for rho, every K steps */
+  LBM_RESID_NONE = 3
+};
+
+/* The 16 positional values of SetUpData.ini (serial/FileIO.c:263-289). */
+typedef struct lbm_ini {
+  float uavg, vavg, rho_ini, viscosity;
+  int   inlet_profile, collision_model, curved_boundaries, outlet_profile;
+  int   iterations, autosave_every, autosave_after, postproc_prog, calculate_drag_lift;
+  float convergence_crit_veloc, convergence_crit_rho;
+  int   read_former_data;
+} lbm_ini;
+
+/* Solver options.  lbm_options_from_ini() fills the reference behaviour. */
+typedef struct lbm_options {
+  int    collision;        /* LBM_BGKW / LBM_TRT / LBM_MRT */
+  int    curved;           /* LBM_CURVED_ON / LBM_CURVED_OFF */
+  int    inlet_profile;    /* LBM_INLET_* */
+  int    outlet_profile;   /* LBM_OUTLET_* */
+  int    drag_lift;        /* boundary id whose cells carry drag/lift, 0 = none */
+  float  uavg, vavg, rho_ini, viscosity;
+  int    precision;        /* LBM_PREC_* */
+  int    init;             /* LBM_INIT_* */
+  int    trt_fp64_parts;   /* 0: TRT parts rounded to binary32 as CellFunctions.c:383-393 (F64 only) */
+  double trt_lambda;       /* > 0: omega- from this magic parameter; 0: 8(2-w)/(8-w) (Iterate.c:180) */
+  int    mrt_textbook_meq; /* 1: m_eq = rho(-2+3u^2), rho(1-3u^2); 0: the rho^2 forms of CellFunctions.c:412-413 */
+  int    mrt_custom_rates; /* 1: mrt_rates[] replace (1,1.4,1.4,1,1.2,1,1.2,w,w) (CellFunctions.c:60-68) */
+  double mrt_rates[9];
+  int    resid_kind;       /* LBM_RESID_* */
+  int    resid_every;      /* K for LBM_RESID_FULL / LBM_RESID_VELOCITY (>= 1) */
+  int    use_stop_rule;    /* 1: stop when !(r0 > crit_v && r1 > crit_rho) (Iterate.c:263) */
+  float  crit_v, crit_rho;
+  int    max_iterations;   /* stop rule's iteration cap (Iterate.c:263 "iter < Iterations") */
+  float  f16_scale;        /* LBM_PREC_F16 fixed scale; 0 = 4096 */
+  int    block_size;       /* bulk-kernel workgroup size (64..1024, multiple of 64); 0 = tuned default */
+} lbm_options;
+
+typedef struct lbm_case lbm_case;
+typedef struct lbm_solver lbm_solver;
+
+typedef struct lbm_case_info {
+  int    n, m;             /* CompDataNode (FileIO.c:193-194) */
+  int    num_nodes, num_conn;
+  int    num_fluid, num_boundary;
+  int    num_inlet;        /* NumInletNodes (FileIO.c:235-250) */
+  double delta;            /* grid spacing (FileIO.c:196-208) */
+  double min_inlet_y, max_inlet_y;
+} lbm_case_info;
+
+typedef struct lbm_timing {
+  double bulk_ms, boundary_ms, outlet_ms, residual_ms, observe_ms, init_ms;
+  long   steps;
+} lbm_timing;
+
+/* ---------------------------------------------------------------- status */
+const char* lbm_last_error(void);
+int lbm_abi_version(void);
+
+/* ---------------------------------------------------------------- input  */
+/* Replaces ReadIniData (serial/FileIO.c:263-289). */
+int lbm_read_ini(const char* path, lbm_ini* out);
+/* Reference behaviour for an ini: every switch at its reference value. */
+void lbm_options_from_ini(const lbm_ini* ini, lbm_options* out);
+
+/* Replaces ReadNodes + ReadBCconn + CompDataNode + CompDataConn
+ * (serial/FileIO.c:14-254, called at serial/Iterate.c:98-110). */
+int lbm_case_load(const char* node_file, const char* bc_file, lbm_case** out);
+/* Same from arrays already in memory: nodes[num_nodes*5] = (i j x y fluid),
+ * conn[num_conn*7] = (i j k type x y boundary_id), as ReadNodes / ReadBCconn
+ * store them (all float). */
+int lbm_case_from_arrays(const float* nodes, int num_nodes, const float* conn, int num_conn,
+                         lbm_case** out);
+/* Mesh generator: the shipped cases' lid-driven cavity with N fluid nodes per
+ * side (Delta = 1/N, n = N+2, m = N+3; as cases/coarse/Mesh). lid_type 2
+ * (inlet, the shipped lid) or 1 (plain wall). */
+int lbm_case_cavity(int N, int lid_type, lbm_case** out);
+/* Mesh generator: channel NX x NY fluid nodes, west inlet (type 2),
+ * east outlet (type 3), walls top/bottom; with an optional circular obstacle
+ * (radius r > 0 at (cx, cy) in node units) whose links carry the exact cut
+ * distance q and boundary id 5. */
+int lbm_case_channel(int NX, int NY, double cx, double cy, double r, lbm_case** out);
+/* Writes the case as Mesh/D2node.dat + Mesh/BCconnectors.dat text. */
+int lbm_case_write(const lbm_case* c, const char* node_file, const char* bc_file);
+int lbm_case_info_get(const lbm_case* c, lbm_case_info* out);
+/* Per-cell set-up exactly as CellIni (serial/CellFunctions.c:99-270) leaves it.
+ * Any pointer may be NULL.  stream/bc: n*m*9 ints; q: n*m*9 doubles. */
+int lbm_case_cells(const lbm_case* c, int* fluid, int* corner, int* stream, int* id,
+                   int* boundary, int* boundary_id, int* bc, double* q, double* x, double* y);
+void lbm_case_free(lbm_case* c);
+
+/* ---------------------------------------------------------------- solver */
+/* Replaces Iteration()'s set-up half (serial/Iterate.c:168-228): device state,
+ * node classes, boundary-link list, first collision. */
+int lbm_solver_create(const lbm_case* c, const lbm_options* o, lbm_solver** out);
+/* As lbm_solver_create with LBM_INIT_FIELD: rho/u/v are n*m host arrays. */
+int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const double* rho,
+                            const double* u, const double* v, lbm_solver** out);
+void lbm_solver_destroy(lbm_solver* s);
+
+/* Replaces one or more passes of the main loop body (serial/Iterate.c:263-332:
+ * CollisionStep, UpdateF, StreamingStep, HandleBoundariesStep,
+ * UpdateMacroscopicStep, ComputeResiduals).  Runs up to nsteps iterations;
+ * with use_stop_rule it stops early when the loop condition fails.
+ * *done (may be NULL) receives the number of iterations run. */
+int lbm_solver_step(lbm_solver* s, int nsteps, int* done);
+/* Same, timing every bulk-kernel launch with HIP events on the solver's
+ * stream.  total_ms: wall time of the nsteps on the stream; bulk_ms: sum of
+ * the bulk-kernel durations. */
+int lbm_solver_step_timed(lbm_solver* s, int nsteps, double* total_ms, double* bulk_ms);
+int lbm_solver_sync(lbm_solver* s);
+int lbm_solver_iter(const lbm_solver* s);
+/* Residuals[0..3] after the last iteration (Iterate.c:300, :259-260 before any). */
+int lbm_solver_residuals(lbm_solver* s, double out[4]);
+/* Residual rows recorded since the last call (for residuals.dat): iters[k],
+ * vals[4k..4k+3].  *count receives how many were written (<= cap). */
+int lbm_solver_residual_history(lbm_solver* s, int cap, int* iters, double* vals, int* count);
+/* U, V, Rho of every cell after the last iteration (n*m each; may be NULL). */
+int lbm_solver_get_macro(lbm_solver* s, double* rho, double* u, double* v);
+/* F (post-boundary distributions) and METAF (post-collision) of every cell
+ * after the last iteration, absolute values, n*m*9 each (may be NULL). */
+int lbm_solver_get_f(lbm_solver* s, double* f, double* metaf);
+/* Replaces WriteResults / WriteFinalResults (serial/FileIO.c:298-427).
+ * final_layout adds f0..f8; row0_bug reproduces FileIO.c:363-386. */
+int lbm_solver_write_results(lbm_solver* s, const char* path, int format, int final_layout,
+                             int row0_bug);
+int lbm_solver_timing(const lbm_solver* s, lbm_timing* out);
+/* Bytes of device memory held by the solver. */
+size_t lbm_solver_device_bytes(const lbm_solver* s);
+
+/* ---------------------------------------------------------------- driver */
+/* The whole reference program, serial/main.c:20-93 + Iteration
+ * (serial/Iterate.c:17-410): reads ini_file (if NULL, "SetUpData.ini") and the
+ * two mesh files, writes Results/ under out_dir with the reference's names
+ * and formats.  opt_overrides may be NULL (reference behaviour).  Returns
+ * LBM_DIVERGED where the reference printed DIVERGENCE! and exited 1. */
+int lbm_run(const char* ini_file, const char* node_file, const char* bc_file,
+            const char* out_dir, const lbm_options* opt_overrides, int verbose);
+
+/* The reference's Iteration() signature (serial/include/Iterate.h:21-27),
+ * writing Results/ in the current directory. */
+int lbm_iteration(const char* NodeDataFile, const char* BCconnectorDataFile, float Uavg,
+                  float Vavg, float rho_ini, float Viscosity, int InletProfile,
+                  int CollisionModel, int CurvedBoundaries, int OutletProfile, int Iterations,
+                  int AutosaveAfter, int AutosaveEvery, int postproc_prog,
+                  int CalculateDragLift, float ConvergenceCritVeloc, float ConvergenceCritRho,
+                  int ReadFormerData);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_H */

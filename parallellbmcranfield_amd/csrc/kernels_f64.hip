@@ -1,0 +1,99 @@
+// kernels_f64.hip — FP64 (reference-order) instantiations and the FP64
+// reductions.  Built with -ffp-contract=off: no fused multiply-add, so every
+// FP64 result follows the reference's C expressions operation by operation.
+#include "lbm_kernels.inl"
+#include "lbm_launch.hpp"
+#include "lbm_pass_impl.inl"
+
+namespace lbmdev {
+
+hipError_t launch_pass_f64(int coll, int mode, int pass, const Launch& L) {
+  switch (coll) {
+    case 1: return pass_impl<StF64, 1>(mode, pass, L);
+    case 2: return pass_impl<StF64, 2>(mode, pass, L);
+    case 3: return pass_impl<StF64, 3>(mode, pass, L);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_init_f64(int coll, int f0_kind, const Launch& L) {
+  switch (coll) {
+    case 1: return init_impl<StF64, 1>(f0_kind, L);
+    case 2: return init_impl<StF64, 2>(f0_kind, L);
+    case 3: return init_impl<StF64, 3>(f0_kind, L);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fill_solid_f64(const Launch& L, void* buf) { return fill_impl<StF64>(L, buf); }
+
+// ---------------------------------------------------------------- reductions
+// Sum of nparts x 4 partials in a fixed order: thread t adds parts t, t+256,
+// ... in sequence, then a fixed-shape LDS tree.  Same result on every run.
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ part,
+                                                         int nparts, double* __restrict__ out) {
+  __shared__ double red[4][256];
+  const int t = threadIdx.x;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = t; i < nparts; i += 256) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) s[c] = s[c] + part[(size_t)i * 4 + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) red[c][t] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) red[c][t] = red[c][t] + red[c][t + w];
+    }
+    __syncthreads();
+  }
+  if (t < 4) out[t] = red[t][0];
+}
+
+// serial/ComputeResiduals.c:19-30: for i < m, for j < n, for k < 9,
+// L2 += (F-METAF)^2 of cell i; drag/lift summed n times per matching cell.
+// Cells never written by the boundary pass are solid (all terms zero), and
+// adding an exact zero leaves the running sum unchanged, so they are skipped.
+__global__ void k_ref_residual(const double* __restrict__ cres, int m, int n,
+                               double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double l2 = 0.0, dr = 0.0, li = 0.0;
+  for (int i = 0; i < m; i++) {
+    const double* c = cres + (size_t)i * 12;
+    if (c[11] == 0.0) continue;
+    double d[9];
+    bool nz = false;
+    for (int k = 0; k < 9; k++) {
+      d[k] = c[k];
+      nz = nz || (d[k] != 0.0);
+    }
+    const bool dz = (c[9] != 0.0) || (c[10] != 0.0);
+    if (!nz && !dz) continue;
+    for (int j = 0; j < n; j++) {
+      if (nz)
+        for (int k = 0; k < 9; k++) l2 = l2 + d[k] * d[k];
+      if (dz) {
+        dr += c[9];
+        li += c[10];
+      }
+    }
+  }
+  out[0] = l2;
+  out[1] = dr;
+  out[2] = li;
+  out[3] = 0.0;
+}
+
+hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st) {
+  k_reduce_partials<<<1, 256, 0, st>>>(part, nparts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_residual(const double* cres, int m, int n, double* out, hipStream_t st) {
+  k_ref_residual<<<1, 64, 0, st>>>(cres, m, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace lbmdev

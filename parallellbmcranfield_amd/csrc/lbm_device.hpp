@@ -1,0 +1,299 @@
+// lbm_device.hpp — device-side building blocks of the D2Q9 step.
+//
+// One step of the reference loop (serial/Iterate.c:263-332) is
+//   collide -> UpdateF -> pull-stream -> boundaries -> macroscopic update.
+// Stored state here is the post-collision field (the reference's METAF), so
+// a fused pass per node does: pull METAF_t from the upstream neighbours
+// (stream), apply the node's boundary rules, form rho/u/v (the reference's
+// macroscopic update of step t), then collide with them (the collision that
+// opens step t+1) and store METAF_{t+1} into the other buffer.  Macros and
+// the post-boundary F of step t therefore exist in registers only; "observe"
+// passes recompute them from the previous buffer when the host asks.
+//
+// Storage kinds:
+//   F64 : absolute f in FP64; arithmetic in FP64 with the reference's operand
+//         order (compiled with -ffp-contract=off) -> bit-identical to the C.
+//   F32 : g = f - w_k*rho0 in FP32; FP32 arithmetic on deviations.
+//   F16 : g*scale in FP16 (fixed power-of-two scale); FP32 arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_kparams.hpp"
+
+namespace lbmdev {
+
+// D2Q9 in the reference's numbering (serial/CellFunctions.c:281-340).
+__device__ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+__device__ constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+__device__ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+__device__ constexpr double W64[9] = {4. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 9.,
+                                      1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+__device__ constexpr float W32[9] = {4.f / 9.f, 1.f / 9.f, 1.f / 9.f, 1.f / 9.f, 1.f / 9.f,
+                                     1.f / 36.f, 1.f / 36.f, 1.f / 36.f, 1.f / 36.f};
+
+// Non-zero pattern of M^-1 (serial/CellFunctions.c:24-35): the reference's
+// dense sums over these rows add exact zeros elsewhere, so skipping them
+// leaves every FP64 result unchanged.
+__device__ constexpr bool MINV_NZ[9][9] = {
+    {1, 1, 1, 0, 0, 0, 0, 0, 0}, {1, 1, 1, 1, 1, 0, 0, 1, 0}, {1, 1, 1, 0, 0, 1, 1, 1, 0},
+    {1, 1, 1, 1, 1, 0, 0, 1, 0}, {1, 1, 1, 0, 0, 1, 1, 1, 0}, {1, 1, 1, 1, 1, 1, 1, 0, 1},
+    {1, 1, 1, 1, 1, 1, 1, 0, 1}, {1, 1, 1, 1, 1, 1, 1, 0, 1}, {1, 1, 1, 1, 1, 1, 1, 0, 1}};
+// Moment matrix M (serial/CellFunctions.c:37-47).
+__device__ constexpr int MM[9][9] = {
+    {1, 1, 1, 1, 1, 1, 1, 1, 1},     {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+    {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
+    {0, -2, 0, 2, 0, 1, -1, -1, 1},  {0, 0, 1, 0, -1, 1, 1, -1, -1},
+    {0, 0, -2, 0, 2, 1, 1, -1, -1},  {0, 1, -1, 1, -1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+
+// ---------------------------------------------------------------- storage
+struct StF64 {
+  using T = double;
+  using R = double;  // arithmetic type
+  static constexpr bool DEV = false;
+  __device__ static R load(const T* p, const KParams&) { return *p; }
+  __device__ static void store(T* p, R v, const KParams&) { __builtin_nontemporal_store(v, p); }
+};
+struct StF32 {
+  using T = float;
+  using R = float;
+  static constexpr bool DEV = true;
+  __device__ static R load(const T* p, const KParams&) { return *p; }
+  __device__ static void store(T* p, R v, const KParams&) { __builtin_nontemporal_store(v, p); }
+};
+struct StF16 {
+  using T = _Float16;
+  using R = float;
+  static constexpr bool DEV = true;
+  __device__ static R load(const T* p, const KParams& kp) { return (float)(*p) * kp.inv_scale_f; }
+  __device__ static void store(T* p, R v, const KParams& kp) {
+    __builtin_nontemporal_store((_Float16)(v * kp.scale_f), p);
+  }
+};
+
+// ---------------------------------------------------------------- collision
+// Reference-order FP64 kernels (absolute f).
+// BGKW: serial/CellFunctions.c:352-365
+__device__ inline void collide_bgk_f64(const double F[9], double R, double U, double V,
+                                       const KParams& p, double out[9]) {
+  const double om = p.omega;
+  const double T1 = U * U + V * V;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double T2 = U * (double)CX[k] + V * (double)CY[k];
+    const double feq = R * W64[k] * (1.0 + 3.0 * T2 + 4.5 * T2 * T2 - 1.5 * T1);
+    out[k] = om * feq + (1.0 - om) * F[k];
+  }
+}
+
+// TRT: serial/CellFunctions.c:372-395 (binary32 parts unless trt_fp64)
+__device__ inline void collide_trt_f64(const double F[9], double R, double U, double V,
+                                       const KParams& p, double out[9]) {
+  const double om = p.omega, oma = p.omega_a;
+  const double T1 = U * U + V * V;
+  double Feq[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double T2 = U * (double)CX[k] + V * (double)CY[k];
+    Feq[k] = R * W64[k] * (1.0 + 3.0 * T2 + 4.5 * T2 * T2 - 1.5 * T1);
+  }
+  if (!(p.flags & KF_TRT_FP64)) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int o = OPP[k];
+      const float fp = (float)(0.5 * (F[k] + F[o]));
+      const float fep = (float)(0.5 * (Feq[k] + Feq[o]));
+      const float fm = (float)(0.5 * (F[k] - F[o]));
+      const float fem = (float)(0.5 * (Feq[k] - Feq[o]));
+      const float dp = fp - fep;
+      const float dm = fm - fem;
+      out[k] = F[k] - (double)dp * om - (double)dm * oma;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int o = OPP[k];
+      const double fp = 0.5 * (F[k] + F[o]);
+      const double fep = 0.5 * (Feq[k] + Feq[o]);
+      const double fm = 0.5 * (F[k] - F[o]);
+      const double fem = 0.5 * (Feq[k] - Feq[o]);
+      out[k] = F[k] - (fp - fep) * om - (fm - fem) * oma;
+    }
+  }
+}
+
+// MRT: serial/CellFunctions.c:401-438, M*F with M's zeros skipped (exact),
+// stmiv = M^-1 S prepared on the host exactly as MRTInitializer does.
+__device__ inline void collide_mrt_f64(const double F[9], double Rh, double U, double V,
+                                       const KParams& p, double out[9]) {
+  double meq[9], mom[9];
+  meq[0] = Rh;
+  if (!(p.flags & KF_MRT_TEXTBOOK)) {
+    meq[1] = Rh * (-2.0 + 3.0 * Rh * (U * U + V * V));
+    meq[2] = Rh * (1.0 - 3.0 * Rh * (U * U + V * V));
+  } else {
+    meq[1] = Rh * (-2.0 + 3.0 * (U * U + V * V));
+    meq[2] = Rh * (1.0 - 3.0 * (U * U + V * V));
+  }
+  meq[3] = Rh * U;
+  meq[4] = -Rh * U;
+  meq[5] = Rh * V;
+  meq[6] = -Rh * V;
+  meq[7] = Rh * (U * U - V * V);
+  meq[8] = Rh * U * V;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double a = 0.0;
+#pragma unroll
+    for (int l = 0; l < 9; l++)
+      if (MM[k][l] != 0) a = a + (double)MM[k][l] * F[l];
+    mom[k] = a;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double b = 0.0;
+#pragma unroll
+    for (int l = 0; l < 9; l++)
+      if (MINV_NZ[k][l]) b = b + p.stmiv[k][l] * (mom[l] - meq[l]);
+    out[k] = F[k] - b;
+  }
+}
+
+template <int COLL>
+__device__ inline void collide_f64(const double F[9], double R, double U, double V,
+                                   const KParams& p, double out[9]) {
+  if constexpr (COLL == 1) collide_bgk_f64(F, R, U, V, p, out);
+  if constexpr (COLL == 2) collide_trt_f64(F, R, U, V, p, out);
+  if constexpr (COLL == 3) collide_mrt_f64(F, R, U, V, p, out);
+}
+
+// Deviation-form FP32 kernels: g_k = f_k - w_k rho0; drho = sum g; rho = rho0 + drho.
+// feq_k - w_k rho0 = w_k (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
+__device__ inline void collide_bgk_dev(const float g[9], float drho, float rho, float u, float v,
+                                       const KParams& p, float out[9]) {
+  const float om = p.omega_f;
+  const float usq15 = 1.5f * (u * u + v * v);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const float cu = (float)CX[k] * u + (float)CY[k] * v;
+    const float geq = W32[k] * (drho + rho * (3.f * cu + 4.5f * cu * cu - usq15));
+    out[k] = g[k] + om * (geq - g[k]);
+  }
+}
+
+__device__ inline void collide_trt_dev(const float g[9], float drho, float rho, float u, float v,
+                                       const KParams& p, float out[9]) {
+  const float op = p.omega_f, om = p.omega_a_f;
+  const float usq15 = 1.5f * (u * u + v * v);
+  // rest population: symmetric only
+  {
+    const float geq0 = W32[0] * (drho - rho * usq15);
+    out[0] = g[0] - op * (g[0] - geq0);
+  }
+  // direction pairs (1,3) (2,4) (5,7) (6,8)
+  constexpr int PA[4] = {1, 2, 5, 6};
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int k = PA[t], o = OPP[k];
+    const float cu = (float)CX[k] * u + (float)CY[k] * v;
+    const float gsym = 0.5f * (g[k] + g[o]);
+    const float gasy = 0.5f * (g[k] - g[o]);
+    const float esym = W32[k] * (drho + rho * (4.5f * cu * cu - usq15));
+    const float easy = W32[k] * rho * 3.f * cu;
+    const float a = op * (gsym - esym);
+    const float b = om * (gasy - easy);
+    out[k] = g[k] - a - b;
+    out[o] = g[o] - a + b;
+  }
+}
+
+__device__ inline void collide_mrt_dev(const float g[9], float drho, float rho, float u, float v,
+                                       const KParams& p, float out[9]) {
+  // moments of g (rows of M)
+  const float s14 = g[1] + g[2] + g[3] + g[4];
+  const float s58 = g[5] + g[6] + g[7] + g[8];
+  const float jx = (g[1] - g[3]) + (g[5] - g[7]) + (g[8] - g[6]);
+  const float jy = (g[2] - g[4]) + (g[5] - g[7]) + (g[6] - g[8]);
+  const float m1 = -4.f * g[0] - s14 + 2.f * s58;
+  const float m2 = 4.f * g[0] - 2.f * s14 + s58;
+  const float m4 = 2.f * (g[3] - g[1]) + (g[5] - g[7]) + (g[8] - g[6]);
+  const float m6 = 2.f * (g[4] - g[2]) + (g[5] - g[7]) + (g[6] - g[8]);
+  const float m7 = (g[1] + g[3]) - (g[2] + g[4]);
+  const float m8 = (g[5] + g[7]) - (g[6] + g[8]);
+  const float usq = u * u + v * v;
+  const float r3 = (p.flags & KF_MRT_TEXTBOOK) ? 3.f * rho * usq : 3.f * rho * rho * usq;
+  // d_l = m_l(f) - m_l^eq; M w = (1,-2,1,0,...) absorbs rho0
+  float d[9];
+  d[0] = 0.f;
+  d[1] = m1 + 2.f * drho - r3;
+  d[2] = m2 - drho + r3;
+  d[3] = jx - rho * u;
+  d[4] = m4 + rho * u;
+  d[5] = jy - rho * v;
+  d[6] = m6 + rho * v;
+  d[7] = m7 - rho * (u * u - v * v);
+  d[8] = m8 - rho * u * v;
+  // e_l = s_l d_l / |M_l|^2 ; out = g - M^T e
+  float e[9];
+#pragma unroll
+  for (int l = 0; l < 9; l++) e[l] = p.mrt_e[l] * d[l];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    float acc = 0.f;
+#pragma unroll
+    for (int l = 1; l < 9; l++) {
+      if (MM[l][k] == 1) acc += e[l];
+      else if (MM[l][k] == -1) acc -= e[l];
+      else if (MM[l][k] != 0) acc += (float)MM[l][k] * e[l];
+    }
+    out[k] = g[k] - acc;
+  }
+}
+
+template <int COLL>
+__device__ inline void collide_dev(const float g[9], float drho, float rho, float u, float v,
+                                   const KParams& p, float out[9]) {
+  if constexpr (COLL == 1) collide_bgk_dev(g, drho, rho, u, v, p, out);
+  if constexpr (COLL == 2) collide_trt_dev(g, drho, rho, u, v, p, out);
+  if constexpr (COLL == 3) collide_mrt_dev(g, drho, rho, u, v, p, out);
+}
+
+// ---------------------------------------------------------------- macros
+// serial/CellFunctions.c:462-481, summation order k = 0..8.
+__device__ inline void macro_f64(const double F[9], double& R, double& U, double& V) {
+  double S = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) S = S + F[k];
+  R = S;
+  double us = 0.0, vs = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    us = us + F[k] * (double)CX[k];
+    vs = vs + F[k] * (double)CY[k];
+  }
+  U = us / R;
+  V = vs / R;
+}
+
+__device__ inline void macro_dev(const float g[9], float rho0, float& drho, float& rho, float& u,
+                                 float& v) {
+  drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+  rho = rho0 + drho;
+  const float jx = (g[1] - g[3]) + (g[5] - g[7]) + (g[8] - g[6]);
+  const float jy = (g[2] - g[4]) + (g[5] - g[7]) + (g[6] - g[8]);
+  const float ir = 1.f / rho;
+  u = jx * ir;
+  v = jy * ir;
+}
+
+// ---------------------------------------------------------------- XCD-aware block order
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2);
+// remapping gives each XCD a contiguous run of tiles so that horizontally
+// adjacent tiles (which share cache lines through the +-1 pulls) meet in the
+// same L2.  Needs gridDim.x % 8 == 0 (the host pads the grid).
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+  return (b & 7u) * (nb >> 3) + (b >> 3);
+}
+
+}  // namespace lbmdev

@@ -1,0 +1,104 @@
+// lbm_host.hpp — host-side structures behind the C ABI (include/lbm.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "lbm_kparams.hpp"
+
+namespace lbmhost {
+
+void set_error(const std::string& msg);
+
+// Per-cell record for cells that at least one connector line matched
+// (the only cells whose CellIni result differs from the defaults
+// BC_ID = 0, Q = 0.5, Boundary = BoundaryID = Corner = 0, stream all 1).
+struct BCell {
+  int bc[9];
+  int corner, boundary, boundary_id;
+  int stream;  // bit k = StreamLattice[k]
+  double q[9];
+};
+
+}  // namespace lbmhost
+
+struct lbm_case {
+  int num_nodes = 0, num_conn = 0;
+  std::vector<float> nodes;  // num_nodes x 5 (i j x y fluid)
+  std::vector<float> conn;   // num_conn x 7 (i j k type x y bid)
+  int n = 0, m = 0;
+  double delta = 0.0;
+  int num_inlet = 0;
+  double min_inlet_y = 0.0, max_inlet_y = 0.0;
+  // CellIni results (options-independent part), reference order j*n+i
+  std::vector<int32_t> id;      // node line of each cell (0 if none, as calloc leaves it)
+  std::vector<int32_t> fluid;   // (int)Nod[ID][4]
+  std::vector<int32_t> bcell;   // index into bcells or -1
+  std::vector<lbmhost::BCell> bcells;
+  int num_fluid = 0, num_boundary = 0;
+
+  double x(long lin) const { return (double)nodes[(size_t)id[lin] * 5 + 2]; }
+  double y(long lin) const { return (double)nodes[(size_t)id[lin] * 5 + 3]; }
+};
+
+namespace lbmhost {
+// case_io.cpp
+int finish_case(lbm_case* c);  // CompDataNode, CompDataConn, CellIni
+double inlet_uo(const lbm_case* c, const lbm_options* o, long lin);
+}  // namespace lbmhost
+
+struct lbm_solver {
+  const lbm_case* cs = nullptr;  // not owned
+  lbm_options opt{};
+  int prec = 0, coll = 1;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // layout
+  int pitch = 0, rows = 0, plane = 0;
+  size_t elem = 8;
+  // device buffers
+  void* buf[2] = {nullptr, nullptr};
+  int cur = 0;  // buf[cur] holds METAF_{t+1}
+  uint8_t* d_cls = nullptr;
+  lbmdev::BNode* d_bn = nullptr;
+  int nbn = 0;
+  lbmdev::BNode* d_on = nullptr;
+  int* d_orow = nullptr;
+  int norows = 0, non = 0;
+  void* d_side = nullptr;
+  int nside = 0;
+  double *d_rho = nullptr, *d_u = nullptr, *d_v = nullptr;  // padded macro outputs
+  double* d_f = nullptr;      // padded [9][plane] (allocated on demand)
+  double* d_metaf = nullptr;  // padded [9][plane] (allocated on demand)
+  double* d_snap = nullptr;   // velocity residual snapshot [3][plane]
+  double* d_cres = nullptr;   // [m][12]
+  double* d_part = nullptr;   // [nparts][4]
+  double* d_hist = nullptr;   // [HCAP][4]
+  int nparts = 0;
+  int nb_bulk = 0, nb_bnd = 0, nb_out = 0;
+  lbmdev::KParams kp{};
+  // host state
+  int iter = 0;
+  int observed = -1;          // iteration whose macros are in d_rho/u/v
+  int observed_f = -1;        // iteration whose F/METAF are in d_f/d_metaf
+  double resid[4] = {100.0, 100.0, 0.0, 0.0};
+  // residual history not yet handed to the caller
+  std::vector<int> hist_iter;
+  std::vector<double> hist_val;
+  int pending = 0;            // device hist entries not yet copied
+  std::vector<int> pending_iter;
+  bool diverged = false;
+  int diverged_iter = -1;
+  std::vector<double> rho0, u0, v0;  // initial macros (reference order), for t = 0
+  int f0_kind = 0;
+  lbm_timing timing{};
+  std::vector<hipEvent_t> ev;  // timing pool
+  size_t dev_bytes = 0;
+};
+
+namespace lbmhost {
+int check_hip(hipError_t e, const char* what);
+}

@@ -1,0 +1,363 @@
+// lbm_kernels.inl — kernel templates, included by one translation unit per
+// floating-point contract (kernels_f64.hip: -ffp-contract=off for the
+// bit-exact FP64 path; kernels_lp.hip: FP32/FP16 storage).
+//
+// Passes of one step (all read buffer `src` = METAF_t, write `dst` = METAF_{t+1}):
+//   k_bulk      interior fluid nodes (class 1): pull 9, macros, collide, store.
+//   k_boundary  boundary-list nodes (class 2): pull with stream mask, the
+//               reference's boundary rules, macros, collide, store.
+//   k_outlet    open-outlet nodes (class 3), one thread per row, in increasing
+//               i, after the boundary pass has left the post-boundary F of
+//               their upstream neighbours in the side buffer.
+// MODE 0: step.  MODE 1: step + residual partials.  MODE 2: observe (no
+// state write; macros / F / METAF of step t to the KOut arrays).
+#pragma once
+#include "lbm_device.hpp"
+
+namespace lbmdev {
+
+// ---------------------------------------------------------------- helpers
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Deterministic per-workgroup sum of 4 values -> part[slot*4 + c].
+template <int BS>
+__device__ inline void block_sum4(double a0, double a1, double a2, double a3, double* part,
+                                  int slot) {
+  __shared__ double red[BS / 64][4];
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  a2 = wave_sum(a2);
+  a3 = wave_sum(a3);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wv][0] = a0;
+    red[wv][1] = a1;
+    red[wv][2] = a2;
+    red[wv][3] = a3;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; w++) s += red[w][threadIdx.x];
+    part[(size_t)slot * 4 + threadIdx.x] = s;
+  }
+}
+
+template <class S>
+__device__ inline double absval(typename S::R g, int k, const KParams& p) {
+  if constexpr (S::DEV)
+    return (double)g + W64[k] * p.rho0;
+  else
+    return (double)g;
+}
+
+// Reference boundary rules on one node's post-streaming F (in place), in the
+// order of serial/Iterate.c:499-546: InletBC, then curved or straight walls,
+// then (profile 1) the velocity outlet.  Open outlets (profiles 2/3) are
+// applied by the caller afterwards.  R is double (absolute values) or float
+// (deviations, where every population-weighted density sum gains rho0).
+template <class S>
+__device__ inline void apply_bc(typename S::R F[9], const BNode& nd, const typename S::T* src,
+                                const KParams& p, int idx) {
+  using R = typename S::R;
+  const R uo = (R)nd.uo;
+  const R roff = S::DEV ? (R)p.rho0 : (R)0;
+  // InletBC: serial/BoundaryConditions.c:14-58
+  if (nd.boundary == 2) {
+    if (nd.bc[2] == 2 && nd.corner != 1) {
+      R rhoN = F[0] + F[1] + F[3] + 2 * (F[2] + F[6] + F[5]);
+      if constexpr (S::DEV) rhoN = rhoN + roff;
+      F[4] = F[2];
+      F[8] = F[6] + rhoN * uo / (R)6.0;
+      F[7] = F[5] - rhoN * uo / (R)6.0;
+    }
+    if (nd.bc[3] == 2 && nd.corner != 1) {
+      R num = F[0] + F[2] + F[4] + (R)2.0 * (F[3] + F[6] + F[7]);
+      if constexpr (S::DEV) num = num + roff;
+      const R rhoW = num / ((R)1.0 - uo);
+      F[1] = F[3] + 2 * rhoW * uo / 3;
+      F[5] = F[7] + rhoW * uo / 6;
+      F[8] = F[6] + rhoW * uo / 6;
+    }
+  }
+  if (p.curved == 1) {
+    // CurvedWallBoundaries: serial/BoundaryConditions.c:127-146 with
+    // Fneighbours[k] = METAF[k] of the upstream node (serial/Iterate.c:509-510)
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      if (nd.bc[k] == 1) {
+        const int o = OPP[k];
+        const R q = (R)nd.q[k];
+        const R mk = S::load(src + k * p.plane + idx, p);
+        if (nd.q[k] < 0.5) {
+          const R nk = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+          F[o] = 2 * q * mk + (1 - 2 * q) * nk;
+        } else {
+          const R mo = S::load(src + o * p.plane + idx, p);
+          F[o] = mk / 2 / q + (2 * q - 1) / (2 * q) * mo;
+        }
+      }
+    }
+  } else if (p.curved == 2) {
+    // WallBC: serial/BoundaryConditions.c:104-118 (post-streaming F[k])
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if (nd.bc[k] == 1) F[OPP[k]] = F[k];
+  }
+  if (p.outlet_profile == 1 && nd.bc[1] == 3) {
+    // OutletBoundaries: serial/BoundaryConditions.c:67-81
+    R num = F[0] + F[2] + F[4] + (R)2.0 * (F[1] + F[5] + F[8]);
+    if constexpr (S::DEV) num = num + roff;
+    const R rhoE = num / (1 - uo);
+    F[3] = F[1] - 2 * rhoE * uo / (R)3.0;
+    F[7] = F[5] - rhoE * uo / (R)6.0;
+    F[6] = F[8] - rhoE * uo / (R)6.0;
+  }
+}
+
+// Everything after the boundary rules for one node: macros (with the
+// outlet's V = 0 and the drag/lift placeholder), residual terms, observe
+// outputs, collision and store.  acc[] receives residual terms (MODE 1).
+template <class S, int COLL, int MODE, bool BND>
+__device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
+                                   typename S::T* dst, const KParams& p, const KOut& o, int idx,
+                                   const BNode* nd, double acc[4]) {
+  using R = typename S::R;
+  R Rh, U, V, dr = 0;
+  if constexpr (S::DEV) {
+    macro_dev(F, p.rho0_f, dr, Rh, U, V);
+  } else {
+    macro_f64(F, Rh, U, V);
+  }
+  double drag = 0.0, lift = 0.0;
+  if constexpr (BND) {
+    // serial/CellFunctions.c:483-493
+    if (nd->bc[1] == 3) V = (R)0.0;
+    if (nd->flags & BN_DRAG) {
+      const double rr = (double)Rh;
+      drag = rr / 3 * (20 - nd->x) / 5;
+      lift = rr / 3 * (20 - nd->y) / 5;
+    }
+  }
+  // residual terms of step t
+  if constexpr (MODE == 1 || BND) {
+    const bool need_full = (MODE == 1 && p.res_kind == RK_FULL);
+    const bool need_cref = BND && (nd->flags & BN_COMPAT_RES) && o.cres != nullptr;
+    if (need_full || need_cref) {
+      double d2 = 0.0;
+      double* cr = need_cref ? o.cres + (size_t)nd->lin * 12 : nullptr;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const R mk = S::load(src + k * p.plane + idx, p);
+        double d;
+        if constexpr (S::DEV)
+          d = (double)(F[k] - mk);
+        else
+          d = F[k] - mk;
+        d2 = d2 + d * d;
+        if (cr) cr[k] = d;
+      }
+      if (cr) {
+        cr[9] = drag;
+        cr[10] = lift;
+        cr[11] = 1.0;
+      }
+      if (need_full) {
+        acc[0] += d2;
+        acc[2] += drag;
+        acc[3] += lift;
+      }
+    }
+    if (MODE == 1 && p.res_kind == RK_VEL) {
+      const double rr = (double)Rh, uu = (double)U, vv = (double)V;
+      const double r0 = o.snap[idx], u0 = o.snap[p.plane + idx], v0 = o.snap[2 * p.plane + idx];
+      acc[0] += (uu - u0) * (uu - u0) + (vv - v0) * (vv - v0);
+      acc[1] += uu * uu + vv * vv;
+      acc[2] += (rr - r0) * (rr - r0);
+      acc[3] += rr * rr;
+      o.snap[idx] = rr;
+      o.snap[p.plane + idx] = uu;
+      o.snap[2 * p.plane + idx] = vv;
+    }
+  }
+  if constexpr (MODE == 2) {
+    o.rho[idx] = (double)Rh;
+    o.u[idx] = (double)U;
+    o.v[idx] = (double)V;
+    if (o.f) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) o.f[(size_t)k * p.plane + idx] = absval<S>(F[k], k, p);
+    }
+    if (o.metaf) {
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+        o.metaf[(size_t)k * p.plane + idx] = absval<S>(S::load(src + k * p.plane + idx, p), k, p);
+    }
+  } else {
+    R out[9];
+    if constexpr (S::DEV)
+      collide_dev<COLL>(F, dr, Rh, U, V, p, out);
+    else
+      collide_f64<COLL>(F, Rh, U, V, p, out);
+#pragma unroll
+    for (int k = 0; k < 9; k++) S::store(dst + k * p.plane + idx, out[k], p);
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+template <class S, int COLL, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ src,
+                                             typename S::T* __restrict__ dst,
+                                             const uint8_t* __restrict__ cls, KParams p, KOut o) {
+  using R = typename S::R;
+  unsigned b = blockIdx.x;
+  if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x);
+  const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
+  const bool active = idx < p.bulk_end && cls[idx] == 1;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (MODE != 1) {
+    if (!active) return;
+  }
+  if (active) {
+    R F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+    finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc);
+  }
+  if constexpr (MODE == 1) block_sum4<BS>(acc[0], acc[1], acc[2], acc[3], o.part, blockIdx.x);
+}
+
+template <class S>
+__device__ inline void pull_masked(typename S::R F[9], const BNode& nd,
+                                   const typename S::T* src, const KParams& p) {
+  // StreamingStep: serial/Iterate.c:470-488.  Directions that do not stream
+  // keep the node's own METAF (UpdateF, Iterate.c:275-276).
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int off = ((nd.stream >> k) & 1) ? -CX[k] - CY[k] * p.pitch : 0;
+    F[k] = S::load(src + k * p.plane + nd.idx + off, p);
+  }
+}
+
+template <class S, int COLL, int MODE>
+__global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restrict__ src,
+                                                  typename S::T* __restrict__ dst,
+                                                  const BNode* __restrict__ bn, int nbn, KParams p,
+                                                  KOut o) {
+  using R = typename S::R;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
+    const BNode nd = bn[i];
+    R F[9];
+    pull_masked<S>(F, nd, src, p);
+    apply_bc<S>(F, nd, src, p, nd.idx);
+    if (nd.flags & BN_FEEDER) {
+      R* side = (R*)o.side + (size_t)nd.side * 9;
+#pragma unroll
+      for (int k = 0; k < 9; k++) side[k] = F[k];
+    }
+    finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
+  }
+  if constexpr (MODE == 1) block_sum4<256>(acc[0], acc[1], acc[2], acc[3], o.part, p.nb_bulk + blockIdx.x);
+}
+
+// Open outlet (Iterate.c:533-541, 2nd order; cuda_step3 GpuBoundaries.cu:200-206,
+// 1st order): one thread walks one row's outlet nodes in increasing i, reading
+// the post-boundary F of its (i-1), (i-2) neighbours from the side buffer,
+// which holds nodes of earlier columns already finished (as the reference's
+// row-major sweep sees them).
+template <class S, int COLL, int MODE>
+__global__ __launch_bounds__(64) void k_outlet(const typename S::T* __restrict__ src,
+                                               typename S::T* __restrict__ dst,
+                                               const BNode* __restrict__ on,
+                                               const int* __restrict__ row_start, int nrows,
+                                               KParams p, KOut o) {
+  using R = typename S::R;
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (r < nrows) {
+    R* side = (R*)o.side;
+    for (int t = row_start[r]; t < row_start[r + 1]; t++) {
+      const BNode nd = on[t];
+      R F[9];
+      pull_masked<S>(F, nd, src, p);
+      apply_bc<S>(F, nd, src, p, nd.idx);
+      const R* a = side + (size_t)nd.feed1 * 9;
+      const R* b = side + (size_t)nd.feed2 * 9;
+      if (p.outlet_profile == 2) {
+        F[1] = 2 * a[1] - b[1];
+        F[5] = 2 * a[5] - b[5];
+        F[8] = 2 * a[8] - b[8];
+      } else {
+        F[1] = a[1];
+        F[5] = a[5];
+        F[8] = a[8];
+      }
+      R* mine = side + (size_t)nd.side * 9;
+#pragma unroll
+      for (int k = 0; k < 9; k++) mine[k] = F[k];
+      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
+    }
+  }
+  if constexpr (MODE == 1)
+    block_sum4<64>(acc[0], acc[1], acc[2], acc[3], o.part, p.nb_bulk + p.nb_bnd + blockIdx.x);
+}
+
+// First collision (serial/Iterate.c:268 on the state CellIni leaves): from
+// F0 (0 / rest / feq of the given macros) and the macros in o.rho/u/v.
+template <class S, int COLL>
+__global__ __launch_bounds__(256) void k_init(typename S::T* __restrict__ dst,
+                                              const uint8_t* __restrict__ cls, int f0_kind,
+                                              KParams p, KOut o) {
+  using R = typename S::R;
+  const int idx = p.bulk_begin + (int)(blockIdx.x * 256 + threadIdx.x);
+  if (idx >= p.bulk_end) return;
+  if (cls[idx] == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) S::store(dst + k * p.plane + idx, S::DEV ? (R)(-W64[k] * p.rho0) : (R)0, p);
+    return;
+  }
+  const double rho = o.rho[idx], u = o.u[idx], v = o.v[idx];
+  double F0[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    if (f0_kind == 0) {
+      F0[k] = 0.0;
+    } else if (f0_kind == 1) {
+      F0[k] = W64[k] * rho;
+    } else {
+      const double T1 = u * u + v * v;
+      const double T2 = u * (double)CX[k] + v * (double)CY[k];
+      F0[k] = rho * W64[k] * (1.0 + 3.0 * T2 + 4.5 * T2 * T2 - 1.5 * T1);
+    }
+  }
+  R out[9];
+  if constexpr (S::DEV) {
+    float g[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = (float)(F0[k] - W64[k] * p.rho0);
+    collide_dev<COLL>(g, (float)(rho - p.rho0), (float)rho, (float)u, (float)v, p, out);
+  } else {
+    collide_f64<COLL>(F0, rho, u, v, p, out);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) S::store(dst + k * p.plane + idx, out[k], p);
+}
+
+// Fill a storage buffer with the solid value everywhere (halo included).
+template <class S>
+__global__ void k_fill_solid(typename S::T* __restrict__ buf, KParams p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.plane) return;
+  using R = typename S::R;
+#pragma unroll
+  for (int k = 0; k < 9; k++) S::store(buf + k * p.plane + i, S::DEV ? (R)(-W64[k] * p.rho0) : (R)0, p);
+}
+
+}  // namespace lbmdev

@@ -1,0 +1,80 @@
+// lbm_kparams.hpp — plain structs shared by host code and kernels.
+#pragma once
+#include <stdint.h>
+
+namespace lbmdev {
+
+enum KFlags : int {
+  KF_TRT_FP64 = 1,       // TRT parts in FP64 (reference rounds them to binary32)
+  KF_MRT_TEXTBOOK = 2,   // textbook MRT e/eps equilibria
+  KF_XCD_REMAP = 4,      // grid is a multiple of 8 blocks; use XCD-contiguous order
+};
+
+// Residual accumulation requested from a pass.
+enum ResKind : int { RK_NONE = 0, RK_FULL = 1, RK_VEL = 2 };
+
+// Kernel parameters (passed by value; lives in the kernarg segment).
+struct KParams {
+  // layout of the padded storage grid: rows 0 and m+1, columns 0 and >= n+1 are halo
+  int n, m;
+  int pitch;          // elements per storage row (multiple of 64)
+  int plane;          // elements per direction plane
+  int bulk_begin;     // first flat element of storage row 1
+  int bulk_end;       // one past the last element of storage row m
+  int flags;
+  int res_kind;       // ResKind for this pass
+  int curved;         // 1 interpolated (Bouzidi) walls, else half-way bounce-back
+  int outlet_profile; // 1 velocity, 2 open 2nd order, 3 open 1st order
+  int drag_lift;      // boundary id for the drag/lift placeholder (0 = off)
+  int nb_bulk;        // number of bulk workgroups (partials offset of the boundary pass)
+  int nb_bnd;         // number of boundary workgroups
+  // physics
+  double omega, omega_a, rho0;
+  float omega_f, omega_a_f, rho0_f;
+  float scale_f, inv_scale_f;
+  double stmiv[9][9];  // FP64 MRT: M^-1 S exactly as MRTInitializer forms it
+  float mrt_e[9];      // FP32 MRT: s_l / |M_l|^2
+};
+
+// Boundary-node record (host builds, kernel reads).  One per fluid node that
+// is not a plain interior node.
+struct BNode {
+  int32_t idx;      // padded storage index
+  int32_t lin;      // reference cell index j*n + i
+  int32_t flags;    // BN_* bits
+  int32_t side;     // side-buffer slot (outlet feeders / outlet nodes) or -1
+  int8_t bc[9];     // BC_ID[k] (CellIni)
+  int8_t corner;    // Corner
+  int8_t boundary;  // Boundary
+  int8_t pad0;
+  uint16_t stream;  // bit k = StreamLattice[k]
+  uint16_t pad1;
+  int32_t boundary_id;
+  int32_t feed1, feed2;  // side slots of the (i-1) and (i-2) feeders (outlet nodes), -1 = solid
+  int32_t pad2;
+  double q[9];      // Q[k]
+  double uo;        // Uo
+  double x, y;      // CoordX, CoordY (drag/lift placeholder)
+};
+
+enum BNFlags : int {
+  BN_COMPAT_RES = 1,  // lin < m: the reference residual sweep reads this cell
+  BN_FEEDER = 2,      // store post-boundary F in side[side]
+  BN_OUTLET = 4,      // open-outlet node: handled by the outlet pass
+  BN_DRAG = 8,        // boundary_id == drag_lift (placeholder drag/lift)
+};
+
+// Outputs of an observe pass / residual pass.
+struct KOut {
+  double* rho;      // padded [plane] macro outputs (observe)
+  double* u;
+  double* v;
+  double* f;        // padded [9][plane] post-boundary F (observe, optional)
+  double* metaf;    // padded [9][plane] own METAF (observe, optional)
+  double* part;     // per-workgroup residual partials [nb][4]
+  double* snap;     // velocity residual snapshot, padded [3][plane] (rho, u, v)
+  double* cres;     // reference-residual cells: [m][12] (9 diffs, drag, lift, written flag)
+  void* side;       // outlet side buffer [slots][9] (compute type)
+};
+
+}  // namespace lbmdev

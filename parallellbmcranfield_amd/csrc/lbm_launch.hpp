@@ -1,0 +1,47 @@
+// lbm_launch.hpp — host-side launchers of the kernels (defined in the .hip units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lbm_kparams.hpp"
+
+namespace lbmdev {
+
+struct Launch {
+  const void* src = nullptr;   // METAF_t
+  void* dst = nullptr;         // METAF_{t+1}
+  const uint8_t* cls = nullptr;
+  const BNode* bn = nullptr;   // boundary-list nodes
+  int nbn = 0;
+  const BNode* on = nullptr;   // open-outlet nodes, grouped by row
+  const int* orow = nullptr;   // row groups: [orow[r], orow[r+1])
+  int norows = 0;
+  int nb_bulk = 0;             // bulk grid (multiple of 8)
+  int block = 256;             // bulk workgroup size
+  KParams kp{};
+  KOut out{};
+  hipStream_t stream = nullptr;
+};
+
+enum Pass { PASS_BULK = 0, PASS_BOUNDARY = 1, PASS_OUTLET = 2 };
+enum Mode { MODE_STEP = 0, MODE_RESID = 1, MODE_OBSERVE = 2 };
+
+// prec: 0 F64, 1 F32, 2 F16.  coll: 1 BGKW, 2 TRT, 3 MRT.
+hipError_t launch_pass(int prec, int coll, int mode, int pass, const Launch& L);
+hipError_t launch_init(int prec, int coll, int f0_kind, const Launch& L);
+hipError_t launch_fill_solid(int prec, const Launch& L, void* buf);
+
+// FP64 launchers live in the -ffp-contract=off unit.
+hipError_t launch_pass_f64(int coll, int mode, int pass, const Launch& L);
+hipError_t launch_init_f64(int coll, int f0_kind, const Launch& L);
+hipError_t launch_fill_solid_f64(const Launch& L, void* buf);
+hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& L);
+hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L);
+hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf);
+
+// Deterministic FP64 sum of nparts x 4 partials -> out[4].
+hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st);
+// The reference residual's sweep (serial/ComputeResiduals.c:19-30) over the
+// cres rows of the first m cells -> out[4] = (sum, drag, lift, 0).
+hipError_t launch_ref_residual(const double* cres, int m, int n, double* out, hipStream_t st);
+
+}  // namespace lbmdev

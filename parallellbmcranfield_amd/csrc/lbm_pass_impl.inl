@@ -1,0 +1,76 @@
+// lbm_pass_impl.inl — grid shapes of the passes (one copy per storage unit).
+#pragma once
+#include "lbm_kernels.inl"
+#include "lbm_launch.hpp"
+
+namespace lbmdev {
+
+template <class S, int COLL, int MODE>
+static hipError_t bulk_launch(const Launch& L) {
+  using T = typename S::T;
+  const T* src = (const T*)L.src;
+  T* dst = (T*)L.dst;
+  switch (L.block) {
+    case 512:
+      k_bulk<S, COLL, MODE, 512><<<L.nb_bulk, 512, 0, L.stream>>>(src, dst, L.cls, L.kp, L.out);
+      break;
+    case 1024:
+      k_bulk<S, COLL, MODE, 1024><<<L.nb_bulk, 1024, 0, L.stream>>>(src, dst, L.cls, L.kp, L.out);
+      break;
+    default:
+      k_bulk<S, COLL, MODE, 256><<<L.nb_bulk, 256, 0, L.stream>>>(src, dst, L.cls, L.kp, L.out);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <class S, int COLL, int MODE>
+static hipError_t pass_mode(int pass, const Launch& L) {
+  using T = typename S::T;
+  const T* src = (const T*)L.src;
+  T* dst = (T*)L.dst;
+  switch (pass) {
+    case PASS_BULK:
+      if (L.nb_bulk <= 0) return hipSuccess;
+      return bulk_launch<S, COLL, MODE>(L);
+    case PASS_BOUNDARY:
+      if (L.nbn <= 0) return hipSuccess;
+      k_boundary<S, COLL, MODE><<<(L.nbn + 255) / 256, 256, 0, L.stream>>>(src, dst, L.bn, L.nbn,
+                                                                           L.kp, L.out);
+      return hipGetLastError();
+    case PASS_OUTLET:
+      if (L.norows <= 0) return hipSuccess;
+      k_outlet<S, COLL, MODE><<<(L.norows + 63) / 64, 64, 0, L.stream>>>(
+          src, dst, L.on, L.orow, L.norows, L.kp, L.out);
+      return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class S, int COLL>
+static hipError_t pass_impl(int mode, int pass, const Launch& L) {
+  switch (mode) {
+    case MODE_STEP: return pass_mode<S, COLL, 0>(pass, L);
+    case MODE_RESID: return pass_mode<S, COLL, 1>(pass, L);
+    case MODE_OBSERVE: return pass_mode<S, COLL, 2>(pass, L);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class S, int COLL>
+static hipError_t init_impl(int f0_kind, const Launch& L) {
+  using T = typename S::T;
+  const int count = L.kp.bulk_end - L.kp.bulk_begin;
+  if (count <= 0) return hipSuccess;
+  k_init<S, COLL><<<(count + 255) / 256, 256, 0, L.stream>>>((T*)L.dst, L.cls, f0_kind, L.kp, L.out);
+  return hipGetLastError();
+}
+
+template <class S>
+static hipError_t fill_impl(const Launch& L, void* buf) {
+  using T = typename S::T;
+  k_fill_solid<S><<<(L.kp.plane + 255) / 256, 256, 0, L.stream>>>((T*)buf, L.kp);
+  return hipGetLastError();
+}
+
+}  // namespace lbmdev

@@ -1,0 +1,687 @@
+// solver.cpp — device state, node classification and the step loop behind
+// the lbm_solver_* entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_host.hpp"
+#include "lbm_launch.hpp"
+
+using namespace lbmdev;
+
+namespace lbmhost {
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return LBM_OK;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return LBM_EHIP;
+}
+
+}  // namespace lbmhost
+
+using namespace lbmhost;
+
+#define HC(x)                                  \
+  do {                                         \
+    int _rc = check_hip((x), #x);              \
+    if (_rc != LBM_OK) return _rc;             \
+  } while (0)
+
+namespace {
+
+constexpr int HCAP = 1024;  // device residual-history slots
+
+int dev_alloc(lbm_solver* s, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? LBM_ENOMEM : LBM_EHIP;
+  }
+  s->dev_bytes += bytes;
+  return LBM_OK;
+}
+
+// MRTInitializer (serial/CellFunctions.c:11-92): stmiv = M^-1 S with M^-1's
+// entries formed as k * (1/36) in FP64, the values the reference's
+// initialiser list produces.
+void mrt_matrices(const double sm[9], double stmiv[9][9]) {
+  static const int Minv36[9][9] = {
+      {4, -4, 4, 0, 0, 0, 0, 0, 0},      {4, -1, -2, 6, -6, 0, 0, 9, 0},
+      {4, -1, -2, 0, 0, 6, -6, -9, 0},   {4, -1, -2, -6, 6, 0, 0, 9, 0},
+      {4, -1, -2, 0, 0, -6, 6, -9, 0},   {4, 2, 1, 6, 3, 6, 3, 0, 9},
+      {4, 2, 1, -6, -3, 6, 3, 0, -9},    {4, 2, 1, -6, -3, -6, -3, 0, 9},
+      {4, 2, 1, 6, 3, -6, -3, 0, -9}};
+  const double a1 = 1. / 36.;
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 9; j++) {
+      const int k = Minv36[i][j];
+      const double minv = (k == 0) ? 0.0 : (double)k * a1;
+      stmiv[i][j] = minv * sm[j];
+    }
+}
+
+Launch make_launch(lbm_solver* s, int src_buf) {
+  Launch L;
+  L.src = s->buf[src_buf];
+  L.dst = s->buf[src_buf ^ 1];
+  L.cls = s->d_cls;
+  L.bn = s->d_bn;
+  L.nbn = s->nbn;
+  L.on = s->d_on;
+  L.orow = s->d_orow;
+  L.norows = s->norows;
+  L.nb_bulk = s->nb_bulk;
+  L.block = s->opt.block_size > 0 ? s->opt.block_size : 256;
+  L.kp = s->kp;
+  L.out.rho = s->d_rho;
+  L.out.u = s->d_u;
+  L.out.v = s->d_v;
+  L.out.part = s->d_part;
+  L.out.snap = s->d_snap;
+  L.out.side = s->d_side;
+  L.stream = s->stream;
+  return L;
+}
+
+hipError_t pass(lbm_solver* s, int mode, int which, const Launch& L) {
+  if (s->prec == 0) return launch_pass_f64(s->coll, mode, which, L);
+  return launch_pass_lp(s->prec, s->coll, mode, which, L);
+}
+
+// Finalise one raw residual row (device sums) into Residuals[0..3].
+bool finalize_resid(lbm_solver* s, const double raw[4], double r[4]) {
+  const double nm = (double)(s->cs->m * s->cs->n);
+  switch (s->opt.resid_kind) {
+    case LBM_RESID_REFERENCE:  // serial/ComputeResiduals.c:33-41
+      r[1] = std::sqrt(raw[0] / (s->cs->m * s->cs->n));
+      r[0] = std::sqrt(raw[0]);
+      r[2] = raw[1];
+      r[3] = raw[2];
+      break;
+    case LBM_RESID_FULL:
+      r[1] = std::sqrt(raw[0] / nm);
+      r[0] = std::sqrt(raw[0]);
+      r[2] = raw[2];
+      r[3] = raw[3];
+      break;
+    case LBM_RESID_VELOCITY: {
+      const double K = (double)s->opt.resid_every;
+      r[0] = raw[1] > 0 ? std::sqrt(raw[0]) / (K * std::sqrt(raw[1])) : std::sqrt(raw[0]);
+      r[1] = raw[3] > 0 ? std::sqrt(raw[2]) / (K * std::sqrt(raw[3])) : std::sqrt(raw[2]);
+      r[2] = 0.0;
+      r[3] = 0.0;
+    } break;
+    default: return true;
+  }
+  return r[0] == r[0];  // NaN -> diverged (ComputeResiduals.c:47-51)
+}
+
+int flush_history(lbm_solver* s) {
+  if (s->pending == 0) return LBM_OK;
+  std::vector<double> raw((size_t)s->pending * 4);
+  HC(hipMemcpyAsync(raw.data(), s->d_hist, raw.size() * sizeof(double), hipMemcpyDeviceToHost,
+                    s->stream));
+  HC(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < s->pending; i++) {
+    double r[4];
+    const bool ok = finalize_resid(s, &raw[(size_t)i * 4], r);
+    if (s->diverged) continue;  // rows after the divergence are not reported
+    std::memcpy(s->resid, r, sizeof r);
+    if (!ok) {
+      // the reference exits inside ComputeResiduals, before the row is printed
+      s->diverged = true;
+      s->diverged_iter = s->pending_iter[i];
+      continue;
+    }
+    s->hist_iter.push_back(s->pending_iter[i]);
+    s->hist_val.insert(s->hist_val.end(), r, r + 4);
+  }
+  s->pending = 0;
+  s->pending_iter.clear();
+  return s->diverged ? LBM_DIVERGED : LBM_OK;
+}
+
+int observe(lbm_solver* s, bool with_f) {
+  if (s->iter == 0) return LBM_OK;
+  if (s->observed == s->iter && (!with_f || s->observed_f == s->iter)) return LBM_OK;
+  if (with_f && !s->d_f) {
+    int rc = dev_alloc(s, (void**)&s->d_f, sizeof(double) * 9 * (size_t)s->plane);
+    if (rc) return rc;
+    rc = dev_alloc(s, (void**)&s->d_metaf, sizeof(double) * 9 * (size_t)s->plane);
+    if (rc) return rc;
+  }
+  Launch L = make_launch(s, s->cur ^ 1);  // previous buffer holds METAF_t
+  L.dst = nullptr;
+  if (with_f) {
+    L.out.f = s->d_f;
+    L.out.metaf = s->d_metaf;
+  }
+  for (int p = 0; p < 3; p++) HC(pass(s, MODE_OBSERVE, p, L));
+  s->observed = s->iter;
+  if (with_f) s->observed_f = s->iter;
+  return LBM_OK;
+}
+
+template <class T>
+void depad(const lbm_solver* s, const T* padded, T* out, long plane_off) {
+  const int n = s->cs->n, m = s->cs->m;
+  for (int j = 0; j < m; j++)
+    std::memcpy(out + (size_t)j * n, padded + plane_off + (size_t)(j + 1) * s->pitch + 1,
+                sizeof(T) * n);
+}
+
+}  // namespace
+
+static int solver_create(const lbm_case* c, const lbm_options* o, const double* frho,
+                         const double* fu, const double* fv, lbm_solver** out) {
+  if (!c || !o || !out) return LBM_EINVAL;
+  *out = nullptr;
+  if (o->collision < 1 || o->collision > 3) {
+    set_error("collision model must be 1 (BGKW), 2 (TRT) or 3 (MRT)");
+    return LBM_EINVAL;
+  }
+  if (o->precision < 0 || o->precision > 2) {
+    set_error("precision must be LBM_PREC_F64/F32/F16");
+    return LBM_EINVAL;
+  }
+  if (o->resid_kind < 0 || o->resid_kind > 3) {
+    set_error("unknown residual kind");
+    return LBM_EINVAL;
+  }
+  if (o->init == LBM_INIT_FIELD && (!frho || !fu || !fv)) {
+    set_error("LBM_INIT_FIELD needs rho/u/v arrays (lbm_solver_create_field)");
+    return LBM_EINVAL;
+  }
+  if (o->block_size != 0 && o->block_size != 256 && o->block_size != 512 &&
+      o->block_size != 1024) {
+    set_error("block_size must be 0, 256, 512 or 1024");
+    return LBM_EINVAL;
+  }
+  const int n = c->n, m = c->m;
+  lbm_solver* s = new lbm_solver();
+  s->cs = c;
+  s->opt = *o;
+  if (s->opt.resid_every < 1) s->opt.resid_every = 1;
+  s->prec = o->precision;
+  s->coll = o->collision;
+  s->elem = s->prec == 0 ? 8 : (s->prec == 1 ? 4 : 2);
+  s->pitch = ((n + 2) + 63) / 64 * 64;
+  s->rows = m + 2;
+  const long long plane = ((long long)s->pitch * s->rows + 255) / 256 * 256;
+  if (plane * 9 >= (1LL << 31)) {
+    set_error("lattice too large for 32-bit plane indexing");
+    delete s;
+    return LBM_EINVAL;
+  }
+  s->plane = (int)plane;
+
+  int rc = check_hip(hipGetDevice(&s->device), "hipGetDevice");
+  if (rc == LBM_OK) rc = check_hip(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking),
+                                   "hipStreamCreate");
+  if (rc != LBM_OK) {
+    delete s;
+    return rc;
+  }
+
+  // ---- physics constants (serial/Iterate.c:179-188)
+  KParams& kp = s->kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.n = n;
+  kp.m = m;
+  kp.pitch = s->pitch;
+  kp.plane = s->plane;
+  kp.bulk_begin = s->pitch;
+  kp.bulk_end = (m + 1) * s->pitch;
+  kp.curved = o->curved;
+  kp.outlet_profile = o->outlet_profile;
+  kp.drag_lift = o->drag_lift;
+  const double omega = 1.0 / (3. * (double)o->viscosity + 0.5);
+  double omega_a = 8 * (2 - omega) / (8 - omega);
+  if (o->trt_lambda > 0) omega_a = 1.0 / (o->trt_lambda / (1.0 / omega - 0.5) + 0.5);
+  kp.omega = omega;
+  kp.omega_a = omega_a;
+  kp.omega_f = (float)omega;
+  kp.omega_a_f = (float)omega_a;
+  kp.rho0 = (double)o->rho_ini;
+  kp.rho0_f = o->rho_ini;
+  const float scale = o->f16_scale > 0 ? o->f16_scale : 4096.0f;
+  kp.scale_f = scale;
+  kp.inv_scale_f = 1.0f / scale;
+  double sm[9] = {1.0, 1.4, 1.4, 1.0, 1.2, 1.0, 1.2, omega, omega};
+  if (o->mrt_custom_rates) std::memcpy(sm, o->mrt_rates, sizeof sm);
+  mrt_matrices(sm, kp.stmiv);
+  static const double norm[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
+  for (int l = 0; l < 9; l++) kp.mrt_e[l] = (float)(sm[l] / norm[l]);
+  kp.flags = 0;
+  if (o->trt_fp64_parts) kp.flags |= KF_TRT_FP64;
+  if (o->mrt_textbook_meq) kp.flags |= KF_MRT_TEXTBOOK;
+
+  // ---- node classes, boundary list, open-outlet rows
+  std::vector<uint8_t> cls((size_t)s->plane, 0);
+  const long nc = (long)n * m;
+  const bool open_outlet = (o->outlet_profile == 2 || o->outlet_profile == 3);
+  std::vector<int> side_of((size_t)nc, -1);
+  std::vector<uint8_t> is_outlet((size_t)nc, 0);
+  int nside = 1;  // slot 0: solid cell (F = 0)
+  if (open_outlet) {
+    for (long lin = 0; lin < nc; lin++) {
+      if (c->fluid[lin] != 1) continue;
+      const int b = c->bcell[lin];
+      if (b >= 0 && c->bcells[b].bc[1] == 3) is_outlet[lin] = 1;
+    }
+    for (long lin = 0; lin < nc; lin++) {
+      if (!is_outlet[lin]) continue;
+      if (side_of[lin] < 0) side_of[lin] = nside++;
+      for (int d = 1; d <= (o->outlet_profile == 2 ? 2 : 1); d++) {
+        const long f = lin - d;  // Cells + n*j + i - d (serial/Iterate.c:538)
+        if (f >= 0 && c->fluid[f] == 1 && side_of[f] < 0) side_of[f] = nside++;
+      }
+    }
+  }
+  std::vector<BNode> bn, on;
+  std::vector<int> orow;
+  int last_row = -1;
+  for (long lin = 0; lin < nc; lin++) {
+    if (c->fluid[lin] != 1) continue;
+    const int i = (int)(lin % n), j = (int)(lin / n);
+    const int idx = (j + 1) * s->pitch + (i + 1);
+    const int b = c->bcell[lin];
+    const BCell* B = b >= 0 ? &c->bcells[b] : nullptr;
+    bool special = lin < m || side_of[lin] >= 0;
+    if (B) {
+      for (int k = 0; k < 9; k++) special = special || B->bc[k] != 0;
+      special = special || B->boundary_id != 0;
+    }
+    if (!special) {
+      cls[idx] = 1;
+      continue;
+    }
+    BNode r;
+    std::memset(&r, 0, sizeof r);
+    r.idx = idx;
+    r.lin = (int)lin;
+    r.side = side_of[lin];
+    r.feed1 = r.feed2 = 0;
+    for (int k = 0; k < 9; k++) {
+      const int v = B ? B->bc[k] : 0;
+      if (v < -128 || v > 127) {
+        set_error("boundary type out of range");
+        lbm_solver_destroy(s);
+        return LBM_EPARSE;
+      }
+      r.bc[k] = (int8_t)v;
+      r.q[k] = B ? B->q[k] : 0.5;
+    }
+    r.corner = (int8_t)(B ? B->corner : 0);
+    r.boundary = (int8_t)(B ? B->boundary : 0);
+    r.stream = (uint16_t)(B ? B->stream : 0x1ff);
+    r.boundary_id = B ? B->boundary_id : 0;
+    r.uo = inlet_uo(c, o, lin);
+    r.x = c->x(lin);
+    r.y = c->y(lin);
+    if (lin < m) r.flags |= BN_COMPAT_RES;
+    if (o->drag_lift != 0 && r.boundary_id == o->drag_lift) r.flags |= BN_DRAG;
+    if (is_outlet[lin]) {
+      r.flags |= BN_OUTLET;
+      const long f1 = lin - 1, f2 = lin - 2;
+      r.feed1 = (f1 >= 0 && c->fluid[f1] == 1) ? side_of[f1] : 0;
+      r.feed2 = (f2 >= 0 && c->fluid[f2] == 1) ? side_of[f2] : 0;
+      cls[idx] = 3;
+      if (j != last_row) {
+        orow.push_back((int)on.size());
+        last_row = j;
+      }
+      on.push_back(r);
+    } else {
+      if (side_of[lin] >= 0) r.flags |= BN_FEEDER;
+      cls[idx] = 2;
+      bn.push_back(r);
+    }
+  }
+  orow.push_back((int)on.size());
+  s->nbn = (int)bn.size();
+  s->non = (int)on.size();
+  s->norows = (int)orow.size() - 1;
+  s->nside = nside;
+
+  // ---- grids
+  const int BS = o->block_size > 0 ? o->block_size : 256;
+  const int count = kp.bulk_end - kp.bulk_begin;
+  s->nb_bulk = ((count + BS - 1) / BS + 7) / 8 * 8;
+  kp.flags |= KF_XCD_REMAP;
+  s->nb_bnd = (s->nbn + 255) / 256;
+  s->nb_out = (s->norows + 63) / 64;
+  kp.nb_bulk = s->nb_bulk;
+  kp.nb_bnd = s->nb_bnd;
+  s->nparts = s->nb_bulk + s->nb_bnd + s->nb_out;
+
+  // ---- device memory
+  const size_t sbytes = (size_t)9 * s->plane * s->elem;
+  const size_t rsz = s->prec == 0 ? sizeof(double) : sizeof(float);
+  rc = LBM_OK;
+  if (!rc) rc = dev_alloc(s, &s->buf[0], sbytes);
+  if (!rc) rc = dev_alloc(s, &s->buf[1], sbytes);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_cls, (size_t)s->plane);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_bn, sizeof(BNode) * std::max(1, s->nbn));
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_on, sizeof(BNode) * std::max(1, s->non));
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_orow, sizeof(int) * orow.size());
+  if (!rc) rc = dev_alloc(s, &s->d_side, rsz * 9 * (size_t)nside);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_rho, sizeof(double) * (size_t)s->plane);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_u, sizeof(double) * (size_t)s->plane);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_v, sizeof(double) * (size_t)s->plane);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_part, sizeof(double) * 4 * (size_t)s->nparts);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_hist, sizeof(double) * 4 * HCAP);
+  if (!rc && o->resid_kind == LBM_RESID_REFERENCE)
+    rc = dev_alloc(s, (void**)&s->d_cres, sizeof(double) * 12 * (size_t)m);
+  if (!rc && o->resid_kind == LBM_RESID_VELOCITY)
+    rc = dev_alloc(s, (void**)&s->d_snap, sizeof(double) * 3 * (size_t)s->plane);
+  if (rc) {
+    lbm_solver_destroy(s);
+    return rc;
+  }
+
+  // ---- initial macros (serial/CellFunctions.c:148-150,267): Rho = rho_ini,
+  // U = Uo, V = 0 everywhere; rest / field starts change the fluid cells.
+  s->rho0.assign((size_t)nc, (double)o->rho_ini);
+  s->u0.assign((size_t)nc, 0.0);
+  s->v0.assign((size_t)nc, 0.0);
+  for (long lin = 0; lin < nc; lin++) {
+    s->u0[lin] = inlet_uo(c, o, lin);
+    if (c->fluid[lin] != 1) continue;
+    if (o->init == LBM_INIT_REST) {
+      s->u0[lin] = 0.0;
+    } else if (o->init == LBM_INIT_FIELD) {
+      s->rho0[lin] = frho[lin];
+      s->u0[lin] = fu[lin];
+      s->v0[lin] = fv[lin];
+    }
+  }
+  s->f0_kind = o->init == LBM_INIT_REST ? 1 : (o->init == LBM_INIT_FIELD ? 2 : 0);
+  {
+    std::vector<double> pr((size_t)s->plane, 0.0), pu((size_t)s->plane, 0.0),
+        pv((size_t)s->plane, 0.0);
+    for (int j = 0; j < m; j++)
+      for (int i = 0; i < n; i++) {
+        const size_t idx = (size_t)(j + 1) * s->pitch + (i + 1);
+        const size_t lin = (size_t)j * n + i;
+        pr[idx] = s->rho0[lin];
+        pu[idx] = s->u0[lin];
+        pv[idx] = s->v0[lin];
+      }
+    rc = check_hip(hipMemcpy(s->d_rho, pr.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice), "upload rho");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_u, pu.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice), "upload u");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_v, pv.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice), "upload v");
+    if (!rc && s->d_snap) {
+      rc = check_hip(hipMemcpy(s->d_snap, pr.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice), "snap");
+      if (!rc) rc = check_hip(hipMemcpy(s->d_snap + s->plane, pu.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice), "snap");
+      if (!rc) rc = check_hip(hipMemcpy(s->d_snap + 2 * (size_t)s->plane, pv.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice), "snap");
+    }
+  }
+  if (!rc) rc = check_hip(hipMemcpy(s->d_cls, cls.data(), cls.size(), hipMemcpyHostToDevice), "upload classes");
+  if (!rc && s->nbn) rc = check_hip(hipMemcpy(s->d_bn, bn.data(), sizeof(BNode) * bn.size(), hipMemcpyHostToDevice), "upload boundary list");
+  if (!rc && s->non) rc = check_hip(hipMemcpy(s->d_on, on.data(), sizeof(BNode) * on.size(), hipMemcpyHostToDevice), "upload outlet list");
+  if (!rc) rc = check_hip(hipMemcpy(s->d_orow, orow.data(), sizeof(int) * orow.size(), hipMemcpyHostToDevice), "upload outlet rows");
+  if (!rc && s->d_cres) rc = check_hip(hipMemset(s->d_cres, 0, sizeof(double) * 12 * (size_t)m), "clear cres");
+  if (!rc) {
+    // side slot 0: a solid cell (F = 0)
+    std::vector<char> side0(rsz * 9 * (size_t)nside, 0);
+    if (s->prec != 0) {
+      static const double W[9] = {4. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 9.,
+                                  1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+      float* f = (float*)side0.data();
+      for (int k = 0; k < 9; k++) f[k] = (float)(-W[k] * (double)o->rho_ini);
+    }
+    rc = check_hip(hipMemcpy(s->d_side, side0.data(), side0.size(), hipMemcpyHostToDevice), "upload side");
+  }
+  if (rc) {
+    lbm_solver_destroy(s);
+    return rc;
+  }
+  s->kp = kp;
+  // ---- fill both buffers with the solid value, then the first collision
+  Launch L = make_launch(s, 1);  // dst = buf[0]
+  hipError_t e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[0]) : launch_fill_solid_lp(s->prec, L, s->buf[0]);
+  if (e == hipSuccess) e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[1]) : launch_fill_solid_lp(s->prec, L, s->buf[1]);
+  if (e == hipSuccess) e = s->prec == 0 ? launch_init_f64(s->coll, s->f0_kind, L) : launch_init_lp(s->prec, s->coll, s->f0_kind, L);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  rc = check_hip(e, "initial collision");
+  if (rc) {
+    lbm_solver_destroy(s);
+    return rc;
+  }
+  s->cur = 0;
+  s->iter = 0;
+  s->resid[0] = 100.0;  // serial/Iterate.c:259-260
+  s->resid[1] = 100.0;
+  *out = s;
+  return LBM_OK;
+}
+
+extern "C" {
+
+int lbm_solver_create(const lbm_case* c, const lbm_options* o, lbm_solver** out) {
+  if (o && o->init == LBM_INIT_FIELD) {
+    set_error("LBM_INIT_FIELD needs lbm_solver_create_field");
+    return LBM_EINVAL;
+  }
+  return solver_create(c, o, nullptr, nullptr, nullptr, out);
+}
+
+int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const double* rho,
+                            const double* u, const double* v, lbm_solver** out) {
+  if (!o) return LBM_EINVAL;
+  lbm_options oo = *o;
+  oo.init = LBM_INIT_FIELD;
+  return solver_create(c, &oo, rho, u, v, out);
+}
+
+void lbm_solver_destroy(lbm_solver* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_bn, s->d_on, s->d_orow, s->d_side,
+                  s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
+                  s->d_part, s->d_hist};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_t>* evs) {
+  if (done) *done = 0;
+  if (s->diverged) {
+    set_error("solver diverged");
+    return LBM_DIVERGED;
+  }
+  const lbm_options& o = s->opt;
+  const bool stop_can_trigger = o.use_stop_rule && (o.crit_v >= 0 || o.crit_rho >= 0);
+  const int K = o.resid_every;
+  int ran = 0;
+  for (int t = 0; t < nsteps; t++) {
+    if (o.use_stop_rule) {
+      if (stop_can_trigger) {
+        int rc = flush_history(s);
+        if (rc) return rc;
+      }
+      // serial/Iterate.c:263
+      if (!(s->resid[0] > o.crit_v && s->resid[1] > o.crit_rho && s->iter < o.max_iterations))
+        break;
+    }
+    const int rk = o.resid_kind;
+    const bool eval = rk == LBM_RESID_REFERENCE ||
+                      ((rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY) && (s->iter + 1) % K == 0);
+    const int mode = (eval && (rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY)) ? MODE_RESID : MODE_STEP;
+    Launch L = make_launch(s, s->cur);
+    L.kp.res_kind = mode == MODE_RESID ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
+    if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
+    if (evs) HC(hipEventRecord((*evs)[2 * t], s->stream));
+    HC(pass(s, mode, PASS_BULK, L));
+    if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
+    HC(pass(s, mode, PASS_BOUNDARY, L));
+    HC(pass(s, mode, PASS_OUTLET, L));
+    if (eval) {
+      double* slot = s->d_hist + (size_t)s->pending * 4;
+      if (rk == LBM_RESID_REFERENCE)
+        HC(launch_ref_residual(s->d_cres, s->cs->m, s->cs->n, slot, s->stream));
+      else
+        HC(launch_reduce_partials(s->d_part, s->nparts, slot, s->stream));
+      s->pending_iter.push_back(s->iter);
+      s->pending++;
+    }
+    s->cur ^= 1;
+    s->iter++;
+    ran++;
+    if (s->pending == HCAP || (eval && stop_can_trigger)) {
+      int rc = flush_history(s);
+      if (rc) {
+        if (done) *done = ran;
+        return rc;
+      }
+    }
+  }
+  int rc = flush_history(s);
+  if (done) *done = ran;
+  s->timing.steps += ran;
+  return rc;
+}
+
+int lbm_solver_step(lbm_solver* s, int nsteps, int* done) {
+  if (!s || nsteps < 0) return LBM_EINVAL;
+  return step_impl(s, nsteps, done, nullptr);
+}
+
+int lbm_solver_step_timed(lbm_solver* s, int nsteps, double* total_ms, double* bulk_ms) {
+  if (!s || nsteps < 0) return LBM_EINVAL;
+  while ((int)s->ev.size() < 2 * nsteps + 2) {
+    hipEvent_t e;
+    HC(hipEventCreate(&e));
+    s->ev.push_back(e);
+  }
+  std::vector<hipEvent_t> evs(s->ev.begin(), s->ev.begin() + 2 * nsteps);
+  hipEvent_t e0 = s->ev[2 * nsteps], e1 = s->ev[2 * nsteps + 1];
+  HC(hipEventRecord(e0, s->stream));
+  int done = 0;
+  int rc = step_impl(s, nsteps, &done, &evs);
+  HC(hipEventRecord(e1, s->stream));
+  HC(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HC(hipEventElapsedTime(&ms, e0, e1));
+  double bsum = 0.0;
+  for (int t = 0; t < done; t++) {
+    float b = 0.f;
+    HC(hipEventElapsedTime(&b, evs[2 * t], evs[2 * t + 1]));
+    bsum += b;
+  }
+  if (total_ms) *total_ms = ms;
+  if (bulk_ms) *bulk_ms = bsum;
+  s->timing.bulk_ms += bsum;
+  return rc;
+}
+
+int lbm_solver_sync(lbm_solver* s) {
+  if (!s) return LBM_EINVAL;
+  return check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+}
+
+int lbm_solver_iter(const lbm_solver* s) { return s ? s->iter : -1; }
+
+int lbm_solver_residuals(lbm_solver* s, double out[4]) {
+  if (!s || !out) return LBM_EINVAL;
+  int rc = flush_history(s);
+  std::memcpy(out, s->resid, sizeof(double) * 4);
+  return rc == LBM_DIVERGED ? LBM_OK : rc;
+}
+
+int lbm_solver_residual_history(lbm_solver* s, int cap, int* iters, double* vals, int* count) {
+  if (!s || cap < 0 || !count) return LBM_EINVAL;
+  (void)flush_history(s);
+  const int have = (int)s->hist_iter.size();
+  const int k = std::min(cap, have);
+  for (int i = 0; i < k; i++) {
+    if (iters) iters[i] = s->hist_iter[i];
+    if (vals) std::memcpy(vals + 4 * i, &s->hist_val[(size_t)4 * i], sizeof(double) * 4);
+  }
+  s->hist_iter.erase(s->hist_iter.begin(), s->hist_iter.begin() + k);
+  s->hist_val.erase(s->hist_val.begin(), s->hist_val.begin() + 4 * (size_t)k);
+  *count = k;
+  return LBM_OK;
+}
+
+int lbm_solver_get_macro(lbm_solver* s, double* rho, double* u, double* v) {
+  if (!s) return LBM_EINVAL;
+  const long nc = (long)s->cs->n * s->cs->m;
+  if (s->iter == 0) {
+    if (rho) std::memcpy(rho, s->rho0.data(), sizeof(double) * nc);
+    if (u) std::memcpy(u, s->u0.data(), sizeof(double) * nc);
+    if (v) std::memcpy(v, s->v0.data(), sizeof(double) * nc);
+    return LBM_OK;
+  }
+  int rc = observe(s, false);
+  if (rc) return rc;
+  std::vector<double> tmp((size_t)s->plane);
+  double* outs[3] = {rho, u, v};
+  double* dev[3] = {s->d_rho, s->d_u, s->d_v};
+  for (int a = 0; a < 3; a++) {
+    if (!outs[a]) continue;
+    HC(hipMemcpyAsync(tmp.data(), dev[a], sizeof(double) * s->plane, hipMemcpyDeviceToHost, s->stream));
+    HC(hipStreamSynchronize(s->stream));
+    depad(s, tmp.data(), outs[a], 0);
+  }
+  return LBM_OK;
+}
+
+int lbm_solver_get_f(lbm_solver* s, double* f, double* metaf) {
+  if (!s) return LBM_EINVAL;
+  const lbm_case* c = s->cs;
+  const long nc = (long)c->n * c->m;
+  static const double W[9] = {4. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 9.,
+                              1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+  if (s->iter == 0) {  // the state CellIni leaves: F0, METAF = 0
+    for (long i = 0; i < nc; i++)
+      for (int k = 0; k < 9; k++) {
+        double v = 0.0;
+        if (c->fluid[i] == 1 && s->f0_kind == 1) v = W[k] * s->rho0[i];
+        if (c->fluid[i] == 1 && s->f0_kind == 2) {
+          static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+          static const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+          const double U = s->u0[i], V = s->v0[i];
+          const double T1 = U * U + V * V, T2 = U * CX[k] + V * CY[k];
+          v = s->rho0[i] * W[k] * (1.0 + 3.0 * T2 + 4.5 * T2 * T2 - 1.5 * T1);
+        }
+        if (f) f[i * 9 + k] = v;
+        if (metaf) metaf[i * 9 + k] = 0.0;
+      }
+    return LBM_OK;
+  }
+  int rc = observe(s, true);
+  if (rc) return rc;
+  std::vector<double> tmp((size_t)9 * s->plane), plane_out((size_t)nc);
+  double* outs[2] = {f, metaf};
+  double* dev[2] = {s->d_f, s->d_metaf};
+  for (int a = 0; a < 2; a++) {
+    if (!outs[a]) continue;
+    HC(hipMemcpyAsync(tmp.data(), dev[a], sizeof(double) * 9 * (size_t)s->plane, hipMemcpyDeviceToHost, s->stream));
+    HC(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < 9; k++) {
+      depad(s, tmp.data(), plane_out.data(), (long)k * s->plane);
+      for (long i = 0; i < nc; i++) outs[a][i * 9 + k] = c->fluid[i] == 1 ? plane_out[i] : 0.0;
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_solver_timing(const lbm_solver* s, lbm_timing* out) {
+  if (!s || !out) return LBM_EINVAL;
+  *out = s->timing;
+  return LBM_OK;
+}
+
+size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 0; }
+
+}  // extern "C"

@@ -174,3 +174,35 @@ def tall_cavity(NXf: int, NYf: int) -> Mesh:
 
     _box_links(mesh, classify)
     return mesh
+
+
+def cavity_arrays_fast(N: int, lid_type: int = 2):
+    """Vectorised cavity (same links, classes and order as ``cavity``), with
+    coordinates at full float precision as the native generator
+    lbm_case_cavity stores them in memory (no %g rounding).  For the large
+    lattices the oracle is timed on."""
+    n, m, d = N + 2, N + 3, 1.0 / N
+    jj, ii = np.mgrid[0:m, 0:n]
+    fl = ((ii >= 1) & (ii <= N) & (jj >= 1) & (jj <= N)).astype(np.float32)
+    nodes = np.stack([ii.ravel().astype(np.float32), jj.ravel().astype(np.float32),
+                      ((ii.ravel() - 0.5) * d).astype(np.float32),
+                      ((jj.ravel() - 0.5) * d).astype(np.float32), fl.ravel()], axis=1)
+    # boundary nodes in row-major order, each with its links in increasing k
+    rows = []
+    fluid = fl.astype(bool)
+    bj, bi = np.nonzero(fluid & ((ii == 1) | (ii == N) | (jj == 1) | (jj == N)))
+    for k in range(1, 9):
+        ti, tj = bi + CX[k], bj + CY[k]
+        solid = ~fluid[tj, ti]
+        i, j, ti, tj = bi[solid], bj[solid], ti[solid], tj[solid]
+        t = np.where(tj == 0, 1, np.where((ti == N + 1) & (tj == N + 1), 1,
+                                          np.where(tj == N + 1, lid_type, 1)))
+        b = np.where(tj == 0, 4, np.where((ti == N + 1) & (tj == N + 1), 3,
+                                          np.where(tj == N + 1, 2, np.where(ti == 0, 1, 3))))
+        xi = (i - 0.5) * d + 0.5 * CX[k] * d
+        yi = (j - 0.5) * d + 0.5 * CY[k] * d
+        rows.append(np.stack([i, j, np.full_like(i, k), t, xi, yi, b, j * n + i], axis=1))
+    allr = np.concatenate(rows)
+    order = np.lexsort((allr[:, 2], allr[:, 7]))
+    conn = allr[order][:, :7].astype(np.float32)
+    return nodes.astype(np.float32), conn

@@ -1,0 +1,144 @@
+"""CPU-side checks of the native library (no kernel launches).
+
+- the C ABI library loads and exports every function include/lbm.h declares;
+- the host set-up (ReadNodes/ReadBCconn/CompData*/CellIni restated in O(N))
+  is bit-exact against the oracle on every golden mesh and the shipped-case
+  equivalents;
+- the mesh generators reproduce the reference-format meshes.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import meshgen
+import oracle as O
+import parallellbmcranfield_amd as lbm
+from golden_util import ini_dict, load, small_cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "lbm.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(lbm_[a-z_0-9]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lbm.lib()
+    declared = header_functions()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(lbm.EXPORTS) == declared
+    assert L.lbm_abi_version() == 1
+
+
+def test_struct_layouts_match_header():
+    # sizes the C side sees (via offsets of the last field)
+    assert C.sizeof(lbm.Ini) == 64
+    assert lbm.Options.block_size.offset + 4 == C.sizeof(lbm.Options) or \
+        C.sizeof(lbm.Options) % 8 == 0
+
+
+def _cells_match(case: lbm.Case, o: O.Oracle):
+    a = case.cells()
+    b = o.cells()
+    for f in ("fluid", "corner", "stream", "id", "boundary", "boundary_id", "bc", "q", "x", "y"):
+        assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("name", small_cases())
+def test_setup_matches_oracle(name):
+    g = load(name)
+    case = lbm.Case.from_arrays(g["nodes"], g["conn"])
+    o = O.Oracle(g["nodes"], g["conn"], O.params(**ini_dict(g["ini"])))
+    assert (case.n, case.m) == (o.n, o.m)
+    assert case.info.delta == o.delta and case.info.num_inlet == o.num_inlet
+    assert case.info.min_inlet_y == o.min_inlet_y and case.info.max_inlet_y == o.max_inlet_y
+    _cells_match(case, o)
+    # and against the reference program's own CellIni output (golden)
+    a = case.cells()
+    for f in ("fluid", "corner", "stream", "id", "boundary", "boundary_id", "bc", "q", "x", "y"):
+        assert np.array_equal(a[f], g[f]), f
+
+
+def test_file_parser_matches_fscanf(tmp_path):
+    mesh = meshgen.cavity(13)
+    mesh.write(str(tmp_path))
+    nf = str(tmp_path / "Mesh" / "D2node.dat")
+    bf = str(tmp_path / "Mesh" / "BCconnectors.dat")
+    case = lbm.Case.load(nf, bf)
+    o = O.Oracle.from_files(nf, bf, O.params())
+    assert (case.n, case.m, case.info.num_nodes, case.info.num_conn) == (o.n, o.m, 15 * 16, len(mesh.links))
+    _cells_match(case, o)
+
+
+def test_shipped_coarse_case_parses_identically():
+    path = "/root/reference/cases/coarse/Mesh"
+    if not os.path.isdir(path):
+        pytest.skip("reference checkout not present")
+    case = lbm.Case.load(path + "/D2node.dat", path + "/BCconnectors.dat")
+    o = O.Oracle.from_files(path + "/D2node.dat", path + "/BCconnectors.dat", O.params())
+    _cells_match(case, o)
+
+
+def test_missing_file_is_an_error():
+    with pytest.raises(lbm.LbmError) as e:
+        lbm.Case.load("/nonexistent/D2node.dat", "/nonexistent/BC.dat")
+    assert e.value.code == lbm.EIO
+
+
+def test_cavity_generator_matches_reference_format(tmp_path):
+    """lbm_case_cavity == the shipped convention: same per-cell set-up as the
+    test-side generator (which reproduces cases/coarse), and the written files
+    parse back to the same cells (coordinates at %g precision)."""
+    for N in (5, 16, 125):
+        case = lbm.Case.cavity(N)
+        nodes, conn = meshgen.cavity(N).arrays()
+        ref = lbm.Case.from_arrays(nodes, conn)
+        a, b = case.cells(), ref.cells()
+        for f in ("fluid", "corner", "stream", "boundary", "boundary_id", "bc"):
+            assert np.array_equal(a[f], b[f]), (N, f)
+        assert np.allclose(a["q"], b["q"], atol=1e-5)
+        assert np.allclose(a["x"], b["x"], atol=1e-6)
+    case = lbm.Case.cavity(125)
+    case.write(str(tmp_path / "n.dat"), str(tmp_path / "c.dat"))
+    nodes, conn = meshgen.cavity(125).arrays()
+    assert open(tmp_path / "n.dat").read().splitlines() == list(meshgen.cavity(125).node_lines())
+    back = lbm.Case.load(str(tmp_path / "n.dat"), str(tmp_path / "c.dat")).cells()
+    ref = lbm.Case.from_arrays(nodes, conn).cells()
+    for f in ref:
+        assert np.array_equal(back[f], ref[f]), f
+
+
+def test_channel_generator_matches_test_generator():
+    case = lbm.Case.channel(24, 10)
+    nodes, conn = meshgen.channel(24, 10).arrays()
+    ref = lbm.Case.from_arrays(nodes, conn)
+    a, b = case.cells(), ref.cells()
+    for f in ("fluid", "corner", "stream", "boundary", "boundary_id", "bc"):
+        assert np.array_equal(a[f], b[f]), f
+    # obstacle: every fluid->obstacle link is a wall with 0 < q <= 1
+    c = lbm.Case.channel(60, 30, 20.3, 15.6, 5.2).cells()
+    wall5 = (c["boundary_id"] == 5)
+    assert wall5.sum() > 10
+    q = c["q"][wall5][c["bc"][wall5] == 1]
+    assert np.all(q > 0) and np.all(q <= 1.0 + 1e-6) and q.std() > 0.05
+
+
+def test_ini_reader(tmp_path):
+    txt = "0.08\n0.0\n5\n0.01\n2\n3\n2\n2\n5000\n100\n10000\n1\n0\n-1.0\n-1.0\n0\n\n:01: Mean U\n"
+    (tmp_path / "SetUpData.ini").write_text(txt)
+    ini = lbm.read_ini(str(tmp_path / "SetUpData.ini"))
+    assert ini.collision_model == 3 and ini.iterations == 5000 and ini.rho_ini == 5.0
+    o = lbm.options_from_ini(ini)
+    assert o.collision == 3 and o.precision == lbm.PREC_F64 and o.use_stop_rule == 1
+    assert o.max_iterations == 5000 and o.resid_kind == lbm.RESID_REFERENCE
+    (tmp_path / "bad.ini").write_text("0.1 0.2\n")
+    with pytest.raises(lbm.LbmError):
+        lbm.read_ini(str(tmp_path / "bad.ini"))

@@ -1,0 +1,280 @@
+"""GPU parity of the HIP path, called through the C ABI (include/lbm.h).
+
+Bar: the FP64 path is bit-identical to the reference program (golden
+fixtures from oracle/_ref) and to the CPU oracle; FP32 / FP16 storage are
+held to the relative-L2 tolerances written in each test against the FP64
+oracle.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import meshgen
+import oracle as O
+import parallellbmcranfield_amd as lbm
+from golden_util import ini_dict, load, residual_lines, shipped_cases, small_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def opts_from_golden(g, **kw):
+    d = ini_dict(g["ini"])
+    return lbm.make_options(collision=d["collision"], uavg=d["uavg"], vavg=d["vavg"],
+                            rho_ini=d["rho_ini"], viscosity=d["viscosity"],
+                            inlet_profile=d["inlet_profile"], curved=d["curved"],
+                            outlet_profile=d["outlet_profile"], drag_lift=d["drag_lift"],
+                            crit_v=d["crit_v"], crit_rho=d["crit_rho"],
+                            max_iterations=d["iterations"], **kw)
+
+
+def run_gpu(nodes, conn, opts, steps):
+    case = lbm.Case.from_arrays(nodes, conn)
+    s = lbm.Solver(case, opts)
+    s.step(steps)
+    return case, s
+
+
+@pytest.mark.parametrize("name", small_cases())
+def test_fp64_bit_exact_vs_reference(name):
+    print("case", name, flush=True)
+    g = load(name)
+    iters = int(g["ini"][8])
+    case, s = run_gpu(g["nodes"], g["conn"], opts_from_golden(g), iters)
+    rho, u, v = s.macro()
+    f, metaf = s.distributions()
+    fl = g["fluid"] == 1
+    assert np.array_equal(rho, g["rho"]), "rho"
+    assert np.array_equal(u, g["u"]), "u"
+    assert np.array_equal(v, g["v"]), "v"
+    # solid cells carry F = METAF = 0 in the reference as here
+    assert np.array_equal(f, g["f"]), "F"
+    assert np.array_equal(metaf[fl], g["metaf"][fl]), "METAF"
+    its, vals = s.residual_history()
+    lines = ["Iter L2_norm L2_norm_weighted Drag Lift"] + [
+        "%d %5.4e %5.4e %f %f" % (i, *r) for i, r in zip(its, vals)]
+    assert lines == residual_lines(g)
+
+
+@pytest.mark.parametrize("name", shipped_cases())
+def test_fp64_bit_exact_on_coarse(name):
+    print("case", name, flush=True)
+    g = load(name)
+    nodes, conn = meshgen.cavity(int(g["N"])).arrays()
+    case, s = run_gpu(nodes, conn, opts_from_golden(g), int(g["ini"][8]))
+    rho, u, v = s.macro()
+    assert np.array_equal(u, g["u"]) and np.array_equal(v, g["v"]) and np.array_equal(rho, g["rho"])
+    f, _ = s.distributions()
+    assert hashlib.sha256(np.ascontiguousarray(f).tobytes()).hexdigest() == str(g["f_sha256"])
+
+
+# ------------------------------------------------------- switches vs the oracle
+SWITCH_CASES = {
+    # name: (mesh factory, oracle params, steps)
+    "rest_bgk": (lambda: meshgen.cavity(20), dict(collision=1, init_rest=1), 60),
+    "rest_mrt_textbook_config_rates": (
+        lambda: meshgen.cavity(20),
+        dict(collision=3, init_rest=1, mrt_textbook=1, mrt_s=None), 60),
+    "trt_lambda_quarter_fp64": (lambda: meshgen.cavity(20),
+                                dict(collision=2, init_rest=1, trt_fp64=1, trt_lambda=0.25), 60),
+    "trt_lambda_quarter_f32parts": (lambda: meshgen.cavity(20),
+                                    dict(collision=2, trt_lambda=0.25), 60),
+    "full_residual": (lambda: meshgen.cavity(20), dict(collision=3, resid_kind=1), 25),
+    "velocity_residual": (lambda: meshgen.cavity(20),
+                          dict(collision=1, resid_kind=2, resid_every=5), 40),
+    "outlet_first_order": (lambda: meshgen.channel(30, 12),
+                           dict(collision=3, outlet_profile=3, uavg=0.05), 80),
+    "parabolic_inlet_open_outlet": (lambda: meshgen.channel(30, 12),
+                                    dict(collision=1, inlet_profile=1, uavg=0.05), 80),
+    "parabolic_inlet_velocity_outlet": (lambda: meshgen.channel(30, 12),
+                                        dict(collision=2, inlet_profile=1, outlet_profile=1,
+                                             uavg=0.05), 80),
+    "obstacle_curved_mrt_rest": (lambda: meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4),
+                                 dict(collision=3, curved=1, init_rest=1), 80),
+    "drag_lift_full": (lambda: meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4),
+                       dict(collision=1, curved=1, drag_lift=5, resid_kind=1), 30),
+}
+
+
+def _lbm_opts_from_oracle(p: O.Params):
+    kw = dict(collision=p.collision, uavg=p.uavg, vavg=p.vavg, rho_ini=p.rho_ini,
+              viscosity=p.viscosity, inlet_profile=p.inlet_profile, curved=p.curved,
+              outlet_profile=p.outlet_profile, drag_lift=p.drag_lift)
+    extra = dict(init=lbm.INIT_REST if p.init_rest else lbm.INIT_REFERENCE,
+                 trt_fp64_parts=p.trt_fp64, trt_lambda=p.trt_lambda,
+                 mrt_textbook_meq=p.mrt_textbook, resid_kind=p.resid_kind,
+                 resid_every=max(p.resid_every, 1), max_iterations=p.iterations)
+    if p.mrt_custom_s:
+        extra["mrt_rates"] = list(p.mrt_s)
+    return lbm.make_options(**kw, **extra)
+
+
+@pytest.mark.parametrize("name", sorted(SWITCH_CASES))
+def test_fp64_switches_match_oracle(name):
+    print("case", name, flush=True)
+    factory, kw, steps = SWITCH_CASES[name]
+    kw = dict(kw)
+    base = dict(uavg=0.1, rho_ini=1.0, viscosity=0.04)
+    base.update({k: v for k, v in kw.items() if k in ("uavg",)})
+    kw.pop("uavg", None)
+    if "mrt_s" in kw:
+        tau = 3 * np.float64(np.float32(base["viscosity"])) + 0.5
+        kw["mrt_s"] = [0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau]
+    p = O.params(**base, **kw)
+    nodes, conn = factory().arrays()
+    o = O.Oracle(nodes, conn, p)
+    case, s = run_gpu(nodes, conn, _lbm_opts_from_oracle(p), 0)
+    for chunk in (1, steps - 1):
+        o.step(chunk)
+        s.step(chunk)
+        c = o.cells()
+        rho, u, v = s.macro()
+        f, mf = s.distributions()
+        assert np.array_equal(rho, c["rho"]) and np.array_equal(u, c["u"]) and np.array_equal(v, c["v"])
+        assert np.array_equal(f, c["f"])
+        fl = c["fluid"] == 1
+        assert np.array_equal(mf[fl], c["metaf"][fl])
+        r_gpu, r_cpu = s.residuals(), o.residuals()
+        # device reductions sum in a different order than the C loop
+        assert np.allclose(r_gpu, r_cpu, rtol=1e-12, atol=1e-300), (r_gpu, r_cpu)
+
+
+# ------------------------------------------------------- reduced precision
+@pytest.mark.parametrize("coll", [1, 2, 3])
+@pytest.mark.parametrize("prec,tol", [(lbm.PREC_F32, 2e-5), (lbm.PREC_F16, 2e-2)])
+def test_reduced_precision_velocity_error(coll, prec, tol):
+    """FP32 (f - w rho0) and FP16 (scaled) storage against the FP64 oracle:
+    relative L2 error of the velocity field after 500 steps on a 64^2 cavity
+    (physics mode: rest start, TRT Lambda = 1/4, MRT with the config rates)."""
+    print("prec", prec, "coll", coll, flush=True)
+    visc = 0.0256
+    tau = 3 * np.float64(np.float32(visc)) + 0.5
+    kw = dict(collision=coll, init_rest=1, uavg=0.1, rho_ini=1.0, viscosity=visc)
+    if coll == 2:
+        kw.update(trt_lambda=0.25, trt_fp64=1)
+    if coll == 3:
+        kw.update(mrt_textbook=1, mrt_s=[0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau])
+    p = O.params(**kw)
+    nodes, conn = meshgen.cavity(64).arrays()
+    o = O.Oracle(nodes, conn, p)
+    o.step(500)
+    c = o.cells()
+    opts = _lbm_opts_from_oracle(p)
+    opts.precision = prec
+    case, s = run_gpu(nodes, conn, opts, 500)
+    rho, u, v = s.macro()
+    fl = c["fluid"] == 1
+    err = np.sqrt(((u - c["u"])[fl] ** 2 + (v - c["v"])[fl] ** 2).sum() /
+                  ((c["u"][fl] ** 2 + c["v"][fl] ** 2).sum()))
+    print("relative L2 velocity error", err, flush=True)
+    assert err < tol
+    assert np.abs(rho - c["rho"])[fl].max() < (1e-5 if prec == lbm.PREC_F32 else 5e-3)
+
+
+# ------------------------------------------------------- writers / drop-in
+def _text_case(name):
+    import golden_util
+    d = os.path.join(golden_util.GOLDEN, "text", name)
+    values = [float(x) for x in open(os.path.join(d, "ini.txt")).read().split()]
+    return d, values
+
+
+def _compare_results(ref_dir, out_dir):
+    timing = ("took", "tOverall", "tIteration", "tInit", "tColl", "tUpd", "tStream", "tBound",
+              "tResid", "tWrit", "MaxInletCoordY", "MinInletCoordY")
+    for fn in sorted(os.listdir(ref_dir)):
+        if fn == "ini.txt":
+            continue
+        a = open(os.path.join(ref_dir, fn)).read().splitlines()
+        b = open(os.path.join(out_dir, fn)).read().splitlines()
+        if fn in ("logFile.log", "SerialTimeMeasuerment.dat"):
+            a = [x for x in a if not any(t in x for t in timing)]
+            b = [x for x in b if not any(t in x for t in timing)]
+        assert a == b, fn
+
+
+@pytest.mark.parametrize("name", ["cav8_text_csv", "cav8_text_tec"])
+def test_run_writes_reference_results(name, tmp_path):
+    ref_dir, values = _text_case(name)
+    meshgen.cavity(8).write(str(tmp_path))
+    (tmp_path / "SetUpData.ini").write_text("\n".join(str(v) for v in values) + "\n")
+    rc = lbm.run(str(tmp_path / "SetUpData.ini"), str(tmp_path / "Mesh" / "D2node.dat"),
+                 str(tmp_path / "Mesh" / "BCconnectors.dat"), str(tmp_path))
+    assert rc == lbm.OK
+    _compare_results(ref_dir, str(tmp_path / "Results"))
+
+
+def test_cli_is_drop_in(tmp_path):
+    import subprocess
+    ref_dir, values = _text_case("cav8_text_csv")
+    meshgen.cavity(8).write(str(tmp_path))
+    (tmp_path / "SetUpData.ini").write_text("\n".join(str(v) for v in values) + "\n")
+    r = subprocess.run([lbm.CLI_PATH], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    _compare_results(ref_dir, str(tmp_path / "Results"))
+    assert "Final results were written to Results/FinalData.csv" in r.stdout
+
+
+def test_divergence_reports_like_reference(tmp_path):
+    """A tau < 1/2 case blows up; the reference-residual sweep only sees solid
+    cells, so (as in the reference) nothing is detected, while the full-field
+    residual turns NaN and the run returns DIVERGED."""
+    nodes, conn = meshgen.cavity(16).arrays()
+    case = lbm.Case.from_arrays(nodes, conn)
+    bad = dict(collision=1, uavg=0.3, rho_ini=1.0, viscosity=-0.16)
+    s = lbm.Solver(case, lbm.make_options(**bad, resid_kind=lbm.RESID_FULL))
+    with pytest.raises(lbm.LbmError) as e:
+        s.step(2000)
+    assert e.value.code == lbm.DIVERGED
+    its, vals = s.residual_history()
+    assert np.all(np.isfinite(vals[:, 0]))
+    s2 = lbm.Solver(case, lbm.make_options(**bad))
+    s2.step(300)
+    assert s2.residuals()[0] == 0.0
+
+
+# ------------------------------------------------------- full-size properties
+def test_large_cavity_fp64_matches_oracle_first_steps():
+    """1025^2 (the bandwidth configuration's smaller size): FP64 MRT, 3 steps,
+    bit-identical to the oracle on every cell."""
+    N = 1025
+    print("cavity", N, flush=True)
+    tau = 3 * np.float64(np.float32(0.1024)) + 0.5
+    p = O.params(collision=3, init_rest=1, uavg=0.1, rho_ini=1.0, viscosity=0.1024,
+                 mrt_textbook=1, mrt_s=[0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau])
+    case = lbm.Case.cavity(N)
+    # the oracle reads the same in-memory mesh through the same node arrays
+    nodes, conn = meshgen.cavity(N).arrays()
+    o = O.Oracle(nodes, conn, p)
+    o.step(3)
+    case = lbm.Case.from_arrays(nodes, conn)
+    s = lbm.Solver(case, _lbm_opts_from_oracle(p))
+    s.step(3)
+    rho, u, v = s.macro()
+    c = o.cells()
+    assert np.array_equal(u, c["u"]) and np.array_equal(v, c["v"]) and np.array_equal(rho, c["rho"])
+
+
+def test_4097_fp32_deterministic_and_close_to_fp64():
+    """4097^2 MRT (the headline configuration): two FP32 runs agree bit for bit,
+    and after 200 steps FP32 stays within 1e-5 relative L2 (velocity) of FP64."""
+    N = 4097
+    print("cavity", N, flush=True)
+    case = lbm.Case.cavity(N)
+    tau = 3 * np.float64(np.float32(0.4096)) + 0.5
+    rates = [0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau]
+    base = dict(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.4096, init=lbm.INIT_REST,
+                mrt_textbook_meq=1, mrt_rates=rates, resid_kind=lbm.RESID_NONE)
+    out = {}
+    for tag, prec in (("a", lbm.PREC_F32), ("b", lbm.PREC_F32), ("d", lbm.PREC_F64)):
+        s = lbm.Solver(case, lbm.make_options(**base, precision=prec))
+        s.step(200)
+        out[tag] = s.macro()
+        s.close()
+    assert all(np.array_equal(x, y) for x, y in zip(out["a"], out["b"]))
+    ua, va = out["a"][1], out["a"][2]
+    ud, vd = out["d"][1], out["d"][2]
+    err = np.sqrt(((ua - ud) ** 2 + (va - vd) ** 2).sum() / (ud ** 2 + vd ** 2).sum())
+    print("4097^2 FP32 vs FP64 relative L2 velocity error", err, flush=True)
+    assert err < 1e-5
