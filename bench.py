@@ -131,6 +131,8 @@ def main():
         f"bulk kernel avg {avg_bulk_s * 1e6:.1f} us -> {achieved:.0f} GB/s, {mlups:.0f} MLUPS")
     s.close()
     del case
+    copy_gbs = lbm.device_copy_gbs(1 << 30, 10)
+    log(f"bench: plain device copy {copy_gbs:.0f} GB/s")
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample (1 core) ...")
@@ -169,6 +171,7 @@ def main():
             "kernel": "k_bulk",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": round(avg_bulk_s * 1e6, 2),
+            "copy_kernel_gbs": round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,
     }

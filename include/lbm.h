@@ -192,6 +192,11 @@ int lbm_solver_timing(const lbm_solver* s, lbm_timing* out);
 /* Bytes of device memory held by the solver. */
 size_t lbm_solver_device_bytes(const lbm_solver* s);
 
+/* Bandwidth of a plain device-to-device copy kernel over a buffer of `bytes`
+ * (read + write counted), best of `reps` launches: the practical HBM ceiling
+ * the step kernels are compared with.  No counterpart in the reference. */
+int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs);
+
 /* ---------------------------------------------------------------- driver */
 /* The whole reference program, serial/main.c:20-93 + Iteration
  * (serial/Iterate.c:17-410): reads ini_file (if NULL, "SetUpData.ini") and the

@@ -35,8 +35,8 @@ EXPORTS = [
     "lbm_solver_create", "lbm_solver_create_field", "lbm_solver_destroy", "lbm_solver_step",
     "lbm_solver_step_timed", "lbm_solver_sync", "lbm_solver_iter", "lbm_solver_residuals",
     "lbm_solver_residual_history", "lbm_solver_get_macro", "lbm_solver_get_f",
-    "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes", "lbm_run",
-    "lbm_iteration",
+    "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
+    "lbm_device_copy_gbs", "lbm_run", "lbm_iteration",
 ]
 
 
@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         "lbm_solver_write_results": (I, [P, C.c_char_p, I, I, I]),
         "lbm_solver_timing": (I, [P, C.POINTER(Timing)]),
         "lbm_solver_device_bytes": (C.c_size_t, [P]),
+        "lbm_device_copy_gbs": (I, [C.c_size_t, I, C.POINTER(D)]),
         "lbm_run": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Options), I]),
         "lbm_iteration": (I, [C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
                               C.c_float, I, I, I, I, I, I, I, I, I, C.c_float, C.c_float, I]),
@@ -340,6 +341,13 @@ def Iteration(NodeDataFile, BCconnectorDataFile, Uavg, Vavg, rho_ini, Viscosity,
                                OutletProfile, Iterations, AutosaveAfter, AutosaveEvery,
                                postproc_prog, CalculateDragLift, ConvergenceCritVeloc,
                                ConvergenceCritRho, ReadFormerData)
+
+
+def device_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Best-of-reps bandwidth (GB/s, read + write) of a plain device copy."""
+    g = C.c_double()
+    _check(lib().lbm_device_copy_gbs(nbytes, reps, C.byref(g)))
+    return g.value
 
 
 def gpu_available() -> bool:

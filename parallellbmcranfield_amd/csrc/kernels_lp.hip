@@ -47,3 +47,28 @@ hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf) {
 }
 
 }  // namespace lbmdev
+
+// ---------------------------------------------------------------- copy probe
+// A plain streaming copy (16-byte vectors, non-temporal stores): the
+// practical HBM ceiling the step kernels are compared with, and the known-size
+// kernel that calibrates hardware byte counters.
+namespace lbmdev {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy(const v4f* __restrict__ src,
+                                              v4f* __restrict__ dst, long n4) {
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const long stride = (long)gridDim.x * 256;
+  for (; i < n4; i += stride) __builtin_nontemporal_store(src[i], dst + i);
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  const long n4 = (long)(bytes / 16);
+  long blocks = (n4 + 255) / 256;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  k_copy<<<(unsigned)blocks, 256, 0, st>>>((const v4f*)src, (v4f*)dst, n4);
+  return hipGetLastError();
+}
+
+}  // namespace lbmdev

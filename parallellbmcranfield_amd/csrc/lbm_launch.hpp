@@ -44,4 +44,7 @@ hipError_t launch_reduce_partials(const double* part, int nparts, double* out, h
 // cres rows of the first m cells -> out[4] = (sum, drag, lift, 0).
 hipError_t launch_ref_residual(const double* cres, int m, int n, double* out, hipStream_t st);
 
+// Plain device copy of `bytes` (multiple of 16).
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+
 }  // namespace lbmdev

@@ -684,4 +684,36 @@ int lbm_solver_timing(const lbm_solver* s, lbm_timing* out) {
 
 size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 0; }
 
+int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs) {
+  if (!gbs || bytes < 16 || reps < 1) return LBM_EINVAL;
+  bytes = bytes / 16 * 16;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = check_hip(hipMalloc(&a, bytes), "hipMalloc");
+  if (!rc) rc = check_hip(hipMalloc(&b, bytes), "hipMalloc");
+  if (!rc) rc = check_hip(hipMemset(a, 0, bytes), "hipMemset");
+  if (!rc) rc = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+  if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
+  if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+  double best = 0.0;
+  for (int r = 0; !rc && r < reps + 1; r++) {
+    rc = check_hip(hipEventRecord(e0, st), "record");
+    if (!rc) rc = check_hip(launch_copy(a, b, bytes, st), "copy");
+    if (!rc) rc = check_hip(hipEventRecord(e1, st), "record");
+    if (!rc) rc = check_hip(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    const double g = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+    if (!rc && r > 0 && g > best) best = g;  // first launch is a warm-up
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  *gbs = best;
+  return rc;
+}
+
 }  // extern "C"
