@@ -63,6 +63,7 @@ struct lbm_solver {
   void* buf[2] = {nullptr, nullptr};
   int cur = 0;  // buf[cur] holds METAF_{t+1}
   uint8_t* d_cls = nullptr;
+  uint8_t* d_tflag = nullptr;
   lbmdev::BNode* d_bn = nullptr;
   int nbn = 0;
   lbmdev::BNode* d_on = nullptr;

@@ -210,15 +210,23 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
 }
 
 // ---------------------------------------------------------------- kernels
+// tflag[t] for the tile of BS consecutive cells starting at bulk_begin + t*BS:
+// 0 no interior node, 1 all interior, 2 mixed (read the per-cell class).
 template <class S, int COLL, int MODE, int BS>
 __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ src,
                                              typename S::T* __restrict__ dst,
-                                             const uint8_t* __restrict__ cls, KParams p, KOut o) {
+                                             const uint8_t* __restrict__ cls,
+                                             const uint8_t* __restrict__ tflag, KParams p,
+                                             KOut o) {
   using R = typename S::R;
   unsigned b = blockIdx.x;
   if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x);
+  const int tf = tflag[b];  // uniform per workgroup
+  if constexpr (MODE != 1) {
+    if (tf == 0) return;
+  }
   const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
-  const bool active = idx < p.bulk_end && cls[idx] == 1;
+  const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (MODE != 1) {
     if (!active) return;

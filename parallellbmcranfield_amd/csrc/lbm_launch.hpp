@@ -10,6 +10,7 @@ struct Launch {
   const void* src = nullptr;   // METAF_t
   void* dst = nullptr;         // METAF_{t+1}
   const uint8_t* cls = nullptr;
+  const uint8_t* tflag = nullptr;  // per bulk tile: 0 none, 1 all interior, 2 mixed
   const BNode* bn = nullptr;   // boundary-list nodes
   int nbn = 0;
   const BNode* on = nullptr;   // open-outlet nodes, grouped by row

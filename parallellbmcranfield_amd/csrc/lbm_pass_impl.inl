@@ -12,13 +12,13 @@ static hipError_t bulk_launch(const Launch& L) {
   T* dst = (T*)L.dst;
   switch (L.block) {
     case 512:
-      k_bulk<S, COLL, MODE, 512><<<L.nb_bulk, 512, 0, L.stream>>>(src, dst, L.cls, L.kp, L.out);
+      k_bulk<S, COLL, MODE, 512><<<L.nb_bulk, 512, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.kp, L.out);
       break;
     case 1024:
-      k_bulk<S, COLL, MODE, 1024><<<L.nb_bulk, 1024, 0, L.stream>>>(src, dst, L.cls, L.kp, L.out);
+      k_bulk<S, COLL, MODE, 1024><<<L.nb_bulk, 1024, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.kp, L.out);
       break;
     default:
-      k_bulk<S, COLL, MODE, 256><<<L.nb_bulk, 256, 0, L.stream>>>(src, dst, L.cls, L.kp, L.out);
+      k_bulk<S, COLL, MODE, 256><<<L.nb_bulk, 256, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.kp, L.out);
       break;
   }
   return hipGetLastError();

@@ -70,6 +70,7 @@ Launch make_launch(lbm_solver* s, int src_buf) {
   L.src = s->buf[src_buf];
   L.dst = s->buf[src_buf ^ 1];
   L.cls = s->d_cls;
+  L.tflag = s->d_tflag;
   L.bn = s->d_bn;
   L.nbn = s->nbn;
   L.on = s->d_on;
@@ -356,6 +357,17 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   kp.flags |= KF_XCD_REMAP;
   s->nb_bnd = (s->nbn + 255) / 256;
   s->nb_out = (s->norows + 63) / 64;
+  // per-tile summary of the classes (tiles of BS cells in launch order)
+  std::vector<uint8_t> tflag((size_t)s->nb_bulk, 0);
+  for (int t = 0; t < s->nb_bulk; t++) {
+    int nb = 0, other = 0;
+    for (int e = 0; e < BS; e++) {
+      const long idx = (long)kp.bulk_begin + (long)t * BS + e;
+      if (idx < kp.bulk_end && cls[idx] == 1) nb++;
+      else other++;
+    }
+    tflag[t] = nb == 0 ? 0 : (other == 0 ? 1 : 2);
+  }
   kp.nb_bulk = s->nb_bulk;
   kp.nb_bnd = s->nb_bnd;
   s->nparts = s->nb_bulk + s->nb_bnd + s->nb_out;
@@ -367,6 +379,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (!rc) rc = dev_alloc(s, &s->buf[0], sbytes);
   if (!rc) rc = dev_alloc(s, &s->buf[1], sbytes);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_cls, (size_t)s->plane);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_tflag, (size_t)s->nb_bulk);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_bn, sizeof(BNode) * std::max(1, s->nbn));
   if (!rc) rc = dev_alloc(s, (void**)&s->d_on, sizeof(BNode) * std::max(1, s->non));
   if (!rc) rc = dev_alloc(s, (void**)&s->d_orow, sizeof(int) * orow.size());
@@ -423,6 +436,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     }
   }
   if (!rc) rc = check_hip(hipMemcpy(s->d_cls, cls.data(), cls.size(), hipMemcpyHostToDevice), "upload classes");
+  if (!rc) rc = check_hip(hipMemcpy(s->d_tflag, tflag.data(), tflag.size(), hipMemcpyHostToDevice), "upload tile flags");
   if (!rc && s->nbn) rc = check_hip(hipMemcpy(s->d_bn, bn.data(), sizeof(BNode) * bn.size(), hipMemcpyHostToDevice), "upload boundary list");
   if (!rc && s->non) rc = check_hip(hipMemcpy(s->d_on, on.data(), sizeof(BNode) * on.size(), hipMemcpyHostToDevice), "upload outlet list");
   if (!rc) rc = check_hip(hipMemcpy(s->d_orow, orow.data(), sizeof(int) * orow.size(), hipMemcpyHostToDevice), "upload outlet rows");
@@ -483,7 +497,7 @@ int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const doubl
 void lbm_solver_destroy(lbm_solver* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_bn, s->d_on, s->d_orow, s->d_side,
+  void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_tflag, s->d_bn, s->d_on, s->d_orow, s->d_side,
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
                   s->d_part, s->d_hist};
   for (void* p : ptrs)

@@ -278,3 +278,21 @@ def test_4097_fp32_deterministic_and_close_to_fp64():
     err = np.sqrt(((ua - ud) ** 2 + (va - vd) ** 2).sum() / (ud ** 2 + vd ** 2).sum())
     print("4097^2 FP32 vs FP64 relative L2 velocity error", err, flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("init_rest", [0, 1])
+def test_config0_cavity_129_bgk_fp64_1000_steps(init_rest):
+    """The plain-cavity configuration (129x129 fluid nodes, BGK, FP64, lid 0.1,
+    nu = 0.128): after 1000 steps u, v, rho are bit-identical to the oracle
+    (the 1e-12 max-abs bar holds with zero difference)."""
+    print("config0 init_rest", init_rest, flush=True)
+    p = O.params(collision=1, uavg=0.1, rho_ini=1.0, viscosity=0.128, init_rest=init_rest)
+    nodes, conn = meshgen.cavity_arrays_fast(129)
+    o = O.Oracle(nodes, conn, p)
+    o.step(1000)
+    case, s = run_gpu(nodes, conn, _lbm_opts_from_oracle(p), 1000)
+    rho, u, v = s.macro()
+    c = o.cells()
+    for name, a, b in (("u", u, c["u"]), ("v", v, c["v"]), ("rho", rho, c["rho"])):
+        assert np.abs(a - b).max() <= 1e-12, name
+        assert np.array_equal(a, b), name
