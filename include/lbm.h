@@ -97,8 +97,12 @@ typedef struct lbm_options {
   float  crit_v, crit_rho;
   int    max_iterations;   /* stop rule's iteration cap (Iterate.c:263 "iter < Iterations") */
   float  f16_scale;        /* LBM_PREC_F16 fixed scale; 0 = 4096 */
-  int    block_size;       /* bulk-kernel workgroup size (64..1024, multiple of 64); 0 = tuned default */
+  int    block_size;       /* bulk-kernel workgroup size: 0 (tuned default), 256, 512 or 1024 */
+  int    tune_flags;       /* LBM_TUNE_* launch-shape switches; 0 = tuned default */
 } lbm_options;
+
+/* Launch-shape switches (results are identical; only speed changes). */
+enum { LBM_TUNE_NO_XCD_REMAP = 1 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;

@@ -26,6 +26,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 INIT_REFERENCE, INIT_REST, INIT_FIELD = 0, 1, 2
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
+TUNE_NO_XCD_REMAP = 1
 
 # Every function include/lbm.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -69,7 +70,7 @@ class Options(C.Structure):
         ("mrt_custom_rates", C.c_int), ("mrt_rates", C.c_double * 9),
         ("resid_kind", C.c_int), ("resid_every", C.c_int), ("use_stop_rule", C.c_int),
         ("crit_v", C.c_float), ("crit_rho", C.c_float), ("max_iterations", C.c_int),
-        ("f16_scale", C.c_float), ("block_size", C.c_int),
+        ("f16_scale", C.c_float), ("block_size", C.c_int), ("tune_flags", C.c_int),
     ]
 
 
