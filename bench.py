@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--collision", default="mrt", choices=["bgk", "trt", "mrt"])
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "f16"])
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--no-xcd-remap", action="store_true")
+    ap.add_argument("--xcd-remap", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
@@ -91,7 +91,7 @@ def main():
     rates = [0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, 1 / tau, 1 / tau]
     kw = dict(collision=coll, uavg=0.1, rho_ini=1.0, viscosity=visc, init=lbm.INIT_REST,
               resid_kind=lbm.RESID_NONE, use_stop_rule=0, block_size=args.block,
-              tune_flags=lbm.TUNE_NO_XCD_REMAP if args.no_xcd_remap else 0)
+              tune_flags=lbm.TUNE_XCD_REMAP if args.xcd_remap else 0)
     if coll == lbm.MRT:
         kw.update(mrt_textbook_meq=1, mrt_rates=rates)
     if coll == lbm.TRT:

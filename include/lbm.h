@@ -101,8 +101,10 @@ typedef struct lbm_options {
   int    tune_flags;       /* LBM_TUNE_* launch-shape switches; 0 = tuned default */
 } lbm_options;
 
-/* Launch-shape switches (results are identical; only speed changes). */
-enum { LBM_TUNE_NO_XCD_REMAP = 1 };
+/* Launch-shape switches (results are identical; only speed changes).
+ * LBM_TUNE_XCD_REMAP: give each XCD a contiguous run of bulk tiles (measured
+ * slower than the hardware's round-robin order on the 4097^2 lattice). */
+enum { LBM_TUNE_XCD_REMAP = 1 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;

@@ -726,11 +726,14 @@ int or_step(or_solver* s, int nsteps)
     boundaries_step(s);
     update_macro_step(s);
     s->iter++;
-    int bad;
+    int bad = 0;
     if (s->p.resid_kind == 2)
       bad = residuals_vel(s);
-    else
-      bad = residuals_ref(s, s->p.resid_kind == 1);
+    else if (s->p.resid_kind == 0)
+      bad = residuals_ref(s, 0);
+    else if (s->p.resid_kind == 1 &&
+             s->iter % (s->p.resid_every > 0 ? s->p.resid_every : 1) == 0)
+      bad = residuals_ref(s, 1);
     if (bad) return 2;
   }
   return 0;

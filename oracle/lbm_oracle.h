@@ -74,8 +74,8 @@ typedef struct or_params {
   int    mrt_textbook;      /* 1: m_eq = rho(-2+3|u|^2), rho(1-3|u|^2) */
   int    mrt_custom_s;      /* 1: use mrt_s[] as relaxation rates */
   double mrt_s[9];
-  int    resid_kind;        /* 0: reference loop, 1: full field, 2: velocity */
-  int    resid_every;       /* velocity residual: K steps between checks */
+  int    resid_kind;        /* 0: reference loop, 1: full field, 2: velocity, 3: none */
+  int    resid_every;       /* full-field / velocity residual: K steps between checks */
   int    outlet1_lid;       /* reserved, keep 0 */
 } or_params;
 

@@ -26,7 +26,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 INIT_REFERENCE, INIT_REST, INIT_FIELD = 0, 1, 2
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
-TUNE_NO_XCD_REMAP = 1
+TUNE_XCD_REMAP = 1
 
 # Every function include/lbm.h declares (checked by tests/test_abi.py).
 EXPORTS = [

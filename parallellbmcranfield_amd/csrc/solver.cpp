@@ -354,7 +354,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   const int BS = o->block_size > 0 ? o->block_size : 256;
   const int count = kp.bulk_end - kp.bulk_begin;
   s->nb_bulk = ((count + BS - 1) / BS + 7) / 8 * 8;
-  if (!(o->tune_flags & LBM_TUNE_NO_XCD_REMAP)) kp.flags |= KF_XCD_REMAP;
+  if (o->tune_flags & LBM_TUNE_XCD_REMAP) kp.flags |= KF_XCD_REMAP;
   s->nb_bnd = (s->nbn + 255) / 256;
   s->nb_out = (s->norows + 63) / 64;
   // per-tile summary of the classes (tiles of BS cells in launch order)

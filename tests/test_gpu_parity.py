@@ -80,7 +80,10 @@ SWITCH_CASES = {
                                 dict(collision=2, init_rest=1, trt_fp64=1, trt_lambda=0.25), 60),
     "trt_lambda_quarter_f32parts": (lambda: meshgen.cavity(20),
                                     dict(collision=2, trt_lambda=0.25), 60),
-    "full_residual": (lambda: meshgen.cavity(20), dict(collision=3, resid_kind=1), 25),
+    "full_residual": (lambda: meshgen.cavity(20), dict(collision=3, resid_kind=1,
+                                                       resid_every=1), 25),
+    "full_residual_every5": (lambda: meshgen.cavity(20), dict(collision=2, resid_kind=1,
+                                                              resid_every=5), 26),
     "velocity_residual": (lambda: meshgen.cavity(20),
                           dict(collision=1, resid_kind=2, resid_every=5), 40),
     "outlet_first_order": (lambda: meshgen.channel(30, 12),
@@ -93,7 +96,7 @@ SWITCH_CASES = {
     "obstacle_curved_mrt_rest": (lambda: meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4),
                                  dict(collision=3, curved=1, init_rest=1), 80),
     "drag_lift_full": (lambda: meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4),
-                       dict(collision=1, curved=1, drag_lift=5, resid_kind=1), 30),
+                       dict(collision=1, curved=1, drag_lift=5, resid_kind=1, resid_every=1), 30),
 }
 
 
@@ -176,7 +179,7 @@ def test_reduced_precision_velocity_error(coll, prec, tol):
 def _text_case(name):
     import golden_util
     d = os.path.join(golden_util.GOLDEN, "text", name)
-    values = [float(x) for x in open(os.path.join(d, "ini.txt")).read().split()]
+    values = open(os.path.join(d, "ini.txt")).read().split()  # keep "%d" fields as written
     return d, values
 
 
@@ -228,7 +231,8 @@ def test_divergence_reports_like_reference(tmp_path):
         s.step(2000)
     assert e.value.code == lbm.DIVERGED
     its, vals = s.residual_history()
-    assert np.all(np.isfinite(vals[:, 0]))
+    # rows before the NaN are reported (inf included, as the reference prints them)
+    assert len(its) > 0 and not np.any(np.isnan(vals[:, 0]))
     s2 = lbm.Solver(case, lbm.make_options(**bad))
     s2.step(300)
     assert s2.residuals()[0] == 0.0

@@ -247,7 +247,8 @@ def test_first_order_outlet_copies_upstream():
 def test_full_field_residual_closed_form():
     """resid_kind 1 sums (F-METAF)^2 over all cells (cuda_step3 intent)."""
     nodes, conn = meshgen.cavity(7).arrays()
-    o = O.Oracle(nodes, conn, O.params(resid_kind=1, uavg=0.1, rho_ini=1.0, viscosity=0.1))
+    o = O.Oracle(nodes, conn, O.params(resid_kind=1, resid_every=1, uavg=0.1, rho_ini=1.0,
+                                       viscosity=0.1))
     o.step(3)
     c = o.cells()
     s = ((c["f"] - c["metaf"]) ** 2).sum()
