@@ -54,10 +54,7 @@ struct lbm_solver {
   const lbm_case* cs = nullptr;  // not owned
   lbm_options opt{};
   int prec = 0, coll = 1;
-  hipStream_t stream = nullptr;   // bulk pass, reductions, copies
-  hipStream_t stream2 = nullptr;  // boundary-list and outlet passes (overlap the bulk pass)
-  hipEvent_t ev_a = nullptr;      // last work on `stream` that touches the state
-  hipEvent_t ev_b = nullptr;      // last work on `stream2`
+  hipStream_t stream = nullptr;
   int device = 0;
   // layout
   int pitch = 0, rows = 0, plane = 0;

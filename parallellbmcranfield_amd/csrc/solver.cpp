@@ -162,10 +162,7 @@ int observe(lbm_solver* s, bool with_f) {
     L.out.f = s->d_f;
     L.out.metaf = s->d_metaf;
   }
-  HC(hipStreamWaitEvent(s->stream, s->ev_b, 0));
   for (int p = 0; p < 3; p++) HC(pass(s, MODE_OBSERVE, p, L));
-  // the next step's boundary pass (second stream) rewrites this buffer
-  HC(hipEventRecord(s->ev_a, s->stream));
   s->observed = s->iter;
   if (with_f) s->observed_f = s->iter;
   return LBM_OK;
@@ -227,10 +224,6 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   int rc = check_hip(hipGetDevice(&s->device), "hipGetDevice");
   if (rc == LBM_OK) rc = check_hip(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking),
                                    "hipStreamCreate");
-  if (rc == LBM_OK) rc = check_hip(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking),
-                                   "hipStreamCreate");
-  if (rc == LBM_OK) rc = check_hip(hipEventCreateWithFlags(&s->ev_a, hipEventDisableTiming), "event");
-  if (rc == LBM_OK) rc = check_hip(hipEventCreateWithFlags(&s->ev_b, hipEventDisableTiming), "event");
   if (rc != LBM_OK) {
     delete s;
     return rc;
@@ -470,8 +463,6 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (e == hipSuccess) e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[1]) : launch_fill_solid_lp(s->prec, L, s->buf[1]);
   if (e == hipSuccess) e = s->prec == 0 ? launch_init_f64(s->coll, s->f0_kind, L) : launch_init_lp(s->prec, s->coll, s->f0_kind, L);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess) e = hipEventRecord(s->ev_a, s->stream);
-  if (e == hipSuccess) e = hipEventRecord(s->ev_b, s->stream2);
   rc = check_hip(e, "initial collision");
   if (rc) {
     lbm_solver_destroy(s);
@@ -506,16 +497,12 @@ int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const doubl
 void lbm_solver_destroy(lbm_solver* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  if (s->stream2) (void)hipStreamSynchronize(s->stream2);
   void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_tflag, s->d_bn, s->d_on, s->d_orow, s->d_side,
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
                   s->d_part, s->d_hist};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
-  if (s->ev_a) (void)hipEventDestroy(s->ev_a);
-  if (s->ev_b) (void)hipEventDestroy(s->ev_b);
-  if (s->stream2) (void)hipStreamDestroy(s->stream2);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -547,21 +534,12 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     Launch L = make_launch(s, s->cur);
     L.kp.res_kind = mode == MODE_RESID ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
     if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
-    // Both passes of step t read METAF_t and write disjoint nodes of METAF_{t+1}:
-    // the bulk pass runs on `stream`, the boundary-list and outlet passes on
-    // `stream2`, each after both passes of step t-1.
-    HC(hipStreamWaitEvent(s->stream2, s->ev_a, 0));
-    HC(hipStreamWaitEvent(s->stream, s->ev_b, 0));
     if (evs) HC(hipEventRecord((*evs)[2 * t], s->stream));
     HC(pass(s, mode, PASS_BULK, L));
     if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
-    Launch L2 = L;
-    L2.stream = s->stream2;
-    HC(pass(s, mode, PASS_BOUNDARY, L2));
-    HC(pass(s, mode, PASS_OUTLET, L2));
-    HC(hipEventRecord(s->ev_b, s->stream2));
+    HC(pass(s, mode, PASS_BOUNDARY, L));
+    HC(pass(s, mode, PASS_OUTLET, L));
     if (eval) {
-      HC(hipStreamWaitEvent(s->stream, s->ev_b, 0));
       double* slot = s->d_hist + (size_t)s->pending * 4;
       if (rk == LBM_RESID_REFERENCE)
         HC(launch_ref_residual(s->d_cres, s->cs->m, s->cs->n, slot, s->stream));
@@ -570,7 +548,6 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
       s->pending_iter.push_back(s->iter);
       s->pending++;
     }
-    HC(hipEventRecord(s->ev_a, s->stream));
     s->cur ^= 1;
     s->iter++;
     ran++;
@@ -582,7 +559,6 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
       }
     }
   }
-  HC(hipStreamWaitEvent(s->stream, s->ev_b, 0));  // later work on `stream` sees the whole step
   int rc = flush_history(s);
   if (done) *done = ran;
   s->timing.steps += ran;
@@ -624,9 +600,7 @@ int lbm_solver_step_timed(lbm_solver* s, int nsteps, double* total_ms, double* b
 
 int lbm_solver_sync(lbm_solver* s) {
   if (!s) return LBM_EINVAL;
-  int rc = check_hip(hipStreamSynchronize(s->stream2), "hipStreamSynchronize");
-  if (!rc) rc = check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-  return rc;
+  return check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
 }
 
 int lbm_solver_iter(const lbm_solver* s) { return s ? s->iter : -1; }
