@@ -103,8 +103,10 @@ typedef struct lbm_options {
 
 /* Launch-shape switches (results are identical; only speed changes).
  * LBM_TUNE_XCD_REMAP: give each XCD a contiguous run of bulk tiles (measured
- * slower than the hardware's round-robin order on the 4097^2 lattice). */
-enum { LBM_TUNE_XCD_REMAP = 1 };
+ * slower than the hardware's round-robin order on the 4097^2 lattice).
+ * LBM_TUNE_SPLIT_BOUNDARY: boundary-list nodes in their own launch after the
+ * bulk pass instead of as the first workgroups of one fused launch. */
+enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;
@@ -197,6 +199,9 @@ int lbm_solver_write_results(lbm_solver* s, const char* path, int format, int fi
 int lbm_solver_timing(const lbm_solver* s, lbm_timing* out);
 /* Bytes of device memory held by the solver. */
 size_t lbm_solver_device_bytes(const lbm_solver* s);
+/* Kernel layout (for profiling): 1 if interior and boundary-list nodes share
+ * one launch per step, 0 if they have separate launches. */
+int lbm_solver_fused(const lbm_solver* s);
 
 /* Bandwidth of a plain device-to-device copy kernel over a buffer of `bytes`
  * (read + write counted), best of `reps` launches: the practical HBM ceiling

@@ -26,7 +26,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 INIT_REFERENCE, INIT_REST, INIT_FIELD = 0, 1, 2
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
-TUNE_XCD_REMAP = 1
+TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY = 1, 2
 
 # Every function include/lbm.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -37,7 +37,7 @@ EXPORTS = [
     "lbm_solver_step_timed", "lbm_solver_sync", "lbm_solver_iter", "lbm_solver_residuals",
     "lbm_solver_residual_history", "lbm_solver_get_macro", "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
-    "lbm_device_copy_gbs", "lbm_run", "lbm_iteration",
+    "lbm_solver_fused", "lbm_device_copy_gbs", "lbm_run", "lbm_iteration",
 ]
 
 
@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
         "lbm_solver_timing": (I, [P, C.POINTER(Timing)]),
         "lbm_solver_device_bytes": (C.c_size_t, [P]),
         "lbm_device_copy_gbs": (I, [C.c_size_t, I, C.POINTER(D)]),
+        "lbm_solver_fused": (I, [P]),
         "lbm_run": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Options), I]),
         "lbm_iteration": (I, [C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
                               C.c_float, I, I, I, I, I, I, I, I, I, C.c_float, C.c_float, I]),
@@ -308,6 +309,10 @@ class Solver:
     def write_results(self, path: str, fmt: int = FORMAT_CSV, final: bool = False,
                       row0_bug: bool = True) -> None:
         _check(lib().lbm_solver_write_results(self.h, path.encode(), fmt, int(final), int(row0_bug)))
+
+    @property
+    def fused(self) -> bool:
+        return bool(lib().lbm_solver_fused(self.h))
 
     def device_bytes(self) -> int:
         return lib().lbm_solver_device_bytes(self.h)

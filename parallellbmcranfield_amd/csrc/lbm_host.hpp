@@ -80,6 +80,7 @@ struct lbm_solver {
   double* d_hist = nullptr;   // [HCAP][4]
   int nparts = 0;
   int nb_bulk = 0, nb_bnd = 0, nb_out = 0;
+  bool fused = true;          // bulk + boundary-list nodes in one launch
   lbmdev::KParams kp{};
   // host state
   int iter = 0;

@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
   const int i = blockIdx.x * 256 + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
-    const BNode nd = bn[i];
+    const BNode& nd = bn[i];  // fields read where used
     R F[9];
     pull_masked<S>(F, nd, src, p);
     apply_bc<S>(F, nd, src, p, nd.idx);
@@ -273,6 +273,55 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
     finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
   }
   if constexpr (MODE == 1) block_sum4<256>(acc[0], acc[1], acc[2], acc[3], o.part, p.nb_bulk + blockIdx.x);
+}
+
+// One launch per step: workgroups [0, nb_bnd) take BS boundary-list nodes
+// each (dispatched first, so their longer latency hides under the bulk
+// tiles), workgroups [nb_bnd, nb_bnd + nb_bulk) are the bulk tiles.
+template <class S, int COLL, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ src,
+                                             typename S::T* __restrict__ dst,
+                                             const uint8_t* __restrict__ cls,
+                                             const uint8_t* __restrict__ tflag,
+                                             const BNode* __restrict__ bn, int nbn, KParams p,
+                                             KOut o) {
+  using R = typename S::R;
+  const int nbb = p.nb_bnd;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if ((int)blockIdx.x < nbb) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
+      const BNode& nd = bn[i];  // fields read where used
+      R F[9];
+      pull_masked<S>(F, nd, src, p);
+      apply_bc<S>(F, nd, src, p, nd.idx);
+      if (nd.flags & BN_FEEDER) {
+        R* side = (R*)o.side + (size_t)nd.side * 9;
+#pragma unroll
+        for (int k = 0; k < 9; k++) side[k] = F[k];
+      }
+      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
+    }
+  } else {
+    unsigned b = blockIdx.x - nbb;
+    if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x - nbb);
+    const int tf = tflag[b];
+    if constexpr (MODE != 1) {
+      if (tf == 0) return;
+    }
+    const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
+    const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+    if constexpr (MODE != 1) {
+      if (!active) return;
+    }
+    if (active) {
+      R F[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+      finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc);
+    }
+  }
+  if constexpr (MODE == 1) block_sum4<BS>(acc[0], acc[1], acc[2], acc[3], o.part, blockIdx.x);
 }
 
 // Open outlet (Iterate.c:533-541, 2nd order; cuda_step3 GpuBoundaries.cu:200-206,
@@ -292,7 +341,7 @@ __global__ __launch_bounds__(64) void k_outlet(const typename S::T* __restrict__
   if (r < nrows) {
     R* side = (R*)o.side;
     for (int t = row_start[r]; t < row_start[r + 1]; t++) {
-      const BNode nd = on[t];
+      const BNode& nd = on[t];
       R F[9];
       pull_masked<S>(F, nd, src, p);
       apply_bc<S>(F, nd, src, p, nd.idx);

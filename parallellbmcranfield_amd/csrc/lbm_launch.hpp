@@ -23,7 +23,7 @@ struct Launch {
   hipStream_t stream = nullptr;
 };
 
-enum Pass { PASS_BULK = 0, PASS_BOUNDARY = 1, PASS_OUTLET = 2 };
+enum Pass { PASS_BULK = 0, PASS_BOUNDARY = 1, PASS_OUTLET = 2, PASS_FUSED = 3 };
 enum Mode { MODE_STEP = 0, MODE_RESID = 1, MODE_OBSERVE = 2 };
 
 // prec: 0 F64, 1 F32, 2 F16.  coll: 1 BGKW, 2 TRT, 3 MRT.

@@ -25,6 +25,18 @@ static hipError_t bulk_launch(const Launch& L) {
 }
 
 template <class S, int COLL, int MODE>
+static hipError_t fused_launch(const Launch& L) {
+  using T = typename S::T;
+  const T* src = (const T*)L.src;
+  T* dst = (T*)L.dst;
+  const int grid = L.kp.nb_bnd + L.nb_bulk;
+  if (grid <= 0) return hipSuccess;
+  k_step<S, COLL, MODE, 256><<<grid, 256, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.bn, L.nbn,
+                                                        L.kp, L.out);
+  return hipGetLastError();
+}
+
+template <class S, int COLL, int MODE>
 static hipError_t pass_mode(int pass, const Launch& L) {
   using T = typename S::T;
   const T* src = (const T*)L.src;
@@ -38,6 +50,8 @@ static hipError_t pass_mode(int pass, const Launch& L) {
       k_boundary<S, COLL, MODE><<<(L.nbn + 255) / 256, 256, 0, L.stream>>>(src, dst, L.bn, L.nbn,
                                                                            L.kp, L.out);
       return hipGetLastError();
+    case PASS_FUSED:
+      return fused_launch<S, COLL, MODE>(L);
     case PASS_OUTLET:
       if (L.norows <= 0) return hipSuccess;
       k_outlet<S, COLL, MODE><<<(L.norows + 63) / 64, 64, 0, L.stream>>>(

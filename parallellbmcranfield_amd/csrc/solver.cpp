@@ -355,6 +355,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   const int count = kp.bulk_end - kp.bulk_begin;
   s->nb_bulk = ((count + BS - 1) / BS + 7) / 8 * 8;
   if (o->tune_flags & LBM_TUNE_XCD_REMAP) kp.flags |= KF_XCD_REMAP;
+  s->fused = BS == 256 && !(o->tune_flags & LBM_TUNE_SPLIT_BOUNDARY);
   s->nb_bnd = (s->nbn + 255) / 256;
   s->nb_out = (s->norows + 63) / 64;
   // per-tile summary of the classes (tiles of BS cells in launch order)
@@ -535,9 +536,14 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     L.kp.res_kind = mode == MODE_RESID ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
     if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
     if (evs) HC(hipEventRecord((*evs)[2 * t], s->stream));
-    HC(pass(s, mode, PASS_BULK, L));
-    if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
-    HC(pass(s, mode, PASS_BOUNDARY, L));
+    if (s->fused) {
+      HC(pass(s, mode, PASS_FUSED, L));  // bulk tiles + boundary-list nodes, one launch
+      if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
+    } else {
+      HC(pass(s, mode, PASS_BULK, L));
+      if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
+      HC(pass(s, mode, PASS_BOUNDARY, L));
+    }
     HC(pass(s, mode, PASS_OUTLET, L));
     if (eval) {
       double* slot = s->d_hist + (size_t)s->pending * 4;
@@ -697,6 +703,8 @@ int lbm_solver_timing(const lbm_solver* s, lbm_timing* out) {
 }
 
 size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 0; }
+
+int lbm_solver_fused(const lbm_solver* s) { return s && s->fused ? 1 : 0; }
 
 int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs) {
   if (!gbs || bytes < 16 || reps < 1) return LBM_EINVAL;
