@@ -74,6 +74,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--xcd-remap", action="store_true")
     ap.add_argument("--split-boundary", action="store_true")
+    ap.add_argument("--xcd-group", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
@@ -93,7 +94,8 @@ def main():
     kw = dict(collision=coll, uavg=0.1, rho_ini=1.0, viscosity=visc, init=lbm.INIT_REST,
               resid_kind=lbm.RESID_NONE, use_stop_rule=0, block_size=args.block,
               tune_flags=(lbm.TUNE_XCD_REMAP if args.xcd_remap else 0) |
-              (lbm.TUNE_SPLIT_BOUNDARY if args.split_boundary else 0))
+              (lbm.TUNE_SPLIT_BOUNDARY if args.split_boundary else 0),
+              xcd_group=args.xcd_group)
     if coll == lbm.MRT:
         kw.update(mrt_textbook_meq=1, mrt_rates=rates)
     if coll == lbm.TRT:

@@ -99,6 +99,7 @@ typedef struct lbm_options {
   float  f16_scale;        /* LBM_PREC_F16 fixed scale; 0 = 4096 */
   int    block_size;       /* bulk-kernel workgroup size: 0 (tuned default), 256, 512 or 1024 */
   int    tune_flags;       /* LBM_TUNE_* launch-shape switches; 0 = tuned default */
+  int    xcd_group;        /* > 0: runs of this many adjacent bulk tiles per XCD; 0 = hardware order */
 } lbm_options;
 
 /* Launch-shape switches (results are identical; only speed changes).

@@ -71,6 +71,7 @@ class Options(C.Structure):
         ("resid_kind", C.c_int), ("resid_every", C.c_int), ("use_stop_rule", C.c_int),
         ("crit_v", C.c_float), ("crit_rho", C.c_float), ("max_iterations", C.c_int),
         ("f16_scale", C.c_float), ("block_size", C.c_int), ("tune_flags", C.c_int),
+        ("xcd_group", C.c_int),
     ]
 
 

@@ -295,5 +295,12 @@ __device__ inline void macro_dev(const float g[9], float rho0, float& drho, floa
 __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
   return (b & 7u) * (nb >> 3) + (b >> 3);
 }
+// Grouped order: runs of G consecutive tiles go to one XCD, so adjacent tiles
+// share their edge lines in one L2 while all XCDs still stream neighbouring
+// addresses.  Needs gridDim.x % (8 G) == 0.
+__device__ inline unsigned xcd_group_block(unsigned b, unsigned G) {
+  const unsigned r = b >> 3, x = b & 7u;
+  return (r / G) * (8u * G) + x * G + (r % G);
+}
 
 }  // namespace lbmdev

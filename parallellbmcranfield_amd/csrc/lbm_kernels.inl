@@ -221,6 +221,7 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
   using R = typename S::R;
   unsigned b = blockIdx.x;
   if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x);
+  else if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
   const int tf = tflag[b];  // uniform per workgroup
   if constexpr (MODE != 1) {
     if (tf == 0) return;
@@ -305,6 +306,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
   } else {
     unsigned b = blockIdx.x - nbb;
     if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x - nbb);
+    else if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
     const int tf = tflag[b];
     if constexpr (MODE != 1) {
       if (tf == 0) return;

@@ -28,6 +28,7 @@ struct KParams {
   int drag_lift;      // boundary id for the drag/lift placeholder (0 = off)
   int nb_bulk;        // number of bulk workgroups (partials offset of the boundary pass)
   int nb_bnd;         // number of boundary workgroups
+  int xcd_group;      // > 0: grouped XCD tile order with this run length
   // physics
   double omega, omega_a, rho0;
   float omega_f, omega_a_f, rho0_f;

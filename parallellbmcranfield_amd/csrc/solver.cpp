@@ -353,7 +353,10 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   // ---- grids
   const int BS = o->block_size > 0 ? o->block_size : 256;
   const int count = kp.bulk_end - kp.bulk_begin;
-  s->nb_bulk = ((count + BS - 1) / BS + 7) / 8 * 8;
+  const int G = o->xcd_group > 0 ? o->xcd_group : 0;
+  const int align = 8 * (G > 0 ? G : 1);
+  s->nb_bulk = ((count + BS - 1) / BS + align - 1) / align * align;
+  kp.xcd_group = G;
   if (o->tune_flags & LBM_TUNE_XCD_REMAP) kp.flags |= KF_XCD_REMAP;
   s->fused = BS == 256 && !(o->tune_flags & LBM_TUNE_SPLIT_BOUNDARY);
   s->nb_bnd = (s->nbn + 255) / 256;
