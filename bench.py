@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--xcd-remap", action="store_true")
     ap.add_argument("--split-boundary", action="store_true")
     ap.add_argument("--xcd-group", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--sample-every", type=int, default=8,
+                    help="HIP events around every N-th step-kernel launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
@@ -94,7 +97,8 @@ def main():
     kw = dict(collision=coll, uavg=0.1, rho_ini=1.0, viscosity=visc, init=lbm.INIT_REST,
               resid_kind=lbm.RESID_NONE, use_stop_rule=0, block_size=args.block,
               tune_flags=(lbm.TUNE_XCD_REMAP if args.xcd_remap else 0) |
-              (lbm.TUNE_SPLIT_BOUNDARY if args.split_boundary else 0),
+              (lbm.TUNE_SPLIT_BOUNDARY if args.split_boundary else 0) |
+              (lbm.TUNE_NO_GRAPH if args.no_graph else 0),
               xcd_group=args.xcd_group)
     if coll == lbm.MRT:
         kw.update(mrt_textbook_meq=1, mrt_rates=rates)
@@ -113,7 +117,7 @@ def main():
     s.sync()
     s.sync()
     t0 = time.perf_counter()
-    total_ms, bulk_ms = s.step_timed(args.steps)
+    total_ms, kern_ms = s.step_timed(args.steps, args.sample_every)
     s.sync()
     t1 = time.perf_counter()
     wall = t1 - t0
@@ -125,7 +129,7 @@ def main():
     mlups = nf * args.steps / wall / 1e6
     esize = {lbm.PREC_F64: 8, lbm.PREC_F32: 4, lbm.PREC_F16: 2}[prec]
     bytes_per_launch = 2 * 9 * esize * n_bulk
-    avg_bulk_s = bulk_ms / 1e3 / args.steps
+    avg_bulk_s = kern_ms / 1e3
     achieved = bytes_per_launch / avg_bulk_s / 1e9
     traffic = None
     if os.path.exists(PMC_SUMMARY):
@@ -180,6 +184,7 @@ def main():
             "kernel": "k_step (bulk tiles + boundary-list nodes)" if fused else "k_bulk",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": round(avg_bulk_s * 1e6, 2),
+            "launches_timed": (args.steps + args.sample_every - 1) // args.sample_every,
             "copy_kernel_gbs": round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,

@@ -99,15 +99,17 @@ typedef struct lbm_options {
   float  f16_scale;        /* LBM_PREC_F16 fixed scale; 0 = 4096 */
   int    block_size;       /* bulk-kernel workgroup size: 0 (tuned default), 256, 512 or 1024 */
   int    tune_flags;       /* LBM_TUNE_* launch-shape switches; 0 = tuned default */
-  int    xcd_group;        /* > 0: runs of this many adjacent bulk tiles per XCD; 0 = hardware order */
+  int    xcd_group;        /* runs of this many adjacent bulk tiles per XCD (1 = hardware order); 0 = tuned default */
 } lbm_options;
 
 /* Launch-shape switches (results are identical; only speed changes).
  * LBM_TUNE_XCD_REMAP: give each XCD a contiguous run of bulk tiles (measured
  * slower than the hardware's round-robin order on the 4097^2 lattice).
  * LBM_TUNE_SPLIT_BOUNDARY: boundary-list nodes in their own launch after the
- * bulk pass instead of as the first workgroups of one fused launch. */
-enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2 };
+ * bulk pass instead of as the first workgroups of one fused launch.
+ * LBM_TUNE_NO_GRAPH: launch every step directly instead of replaying captured
+ * two-step HIP graphs for steps without residual evaluation. */
+enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2, LBM_TUNE_NO_GRAPH = 4 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;
@@ -177,10 +179,11 @@ void lbm_solver_destroy(lbm_solver* s);
  * with use_stop_rule it stops early when the loop condition fails.
  * *done (may be NULL) receives the number of iterations run. */
 int lbm_solver_step(lbm_solver* s, int nsteps, int* done);
-/* Same, timing every bulk-kernel launch with HIP events on the solver's
- * stream.  total_ms: wall time of the nsteps on the stream; bulk_ms: sum of
- * the bulk-kernel durations. */
-int lbm_solver_step_timed(lbm_solver* s, int nsteps, double* total_ms, double* bulk_ms);
+/* Same, timing the step kernel with HIP events on the solver's stream around
+ * every `sample_every`-th launch (1 = every launch).  total_ms: stream time of
+ * the nsteps; kernel_ms: average duration of one sampled step-kernel launch. */
+int lbm_solver_step_timed(lbm_solver* s, int nsteps, int sample_every, double* total_ms,
+                          double* kernel_ms);
 int lbm_solver_sync(lbm_solver* s);
 int lbm_solver_iter(const lbm_solver* s);
 /* Residuals[0..3] after the last iteration (Iterate.c:300, :259-260 before any). */

@@ -26,7 +26,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 INIT_REFERENCE, INIT_REST, INIT_FIELD = 0, 1, 2
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
-TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY = 1, 2
+TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY, TUNE_NO_GRAPH = 1, 2, 4
 
 # Every function include/lbm.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -118,7 +118,7 @@ def lib() -> C.CDLL:
         "lbm_solver_create_field": (I, [P, C.POINTER(Options), P, P, P, C.POINTER(P)]),
         "lbm_solver_destroy": (None, [P]),
         "lbm_solver_step": (I, [P, I, C.POINTER(I)]),
-        "lbm_solver_step_timed": (I, [P, I, C.POINTER(D), C.POINTER(D)]),
+        "lbm_solver_step_timed": (I, [P, I, I, C.POINTER(D), C.POINTER(D)]),
         "lbm_solver_sync": (I, [P]),
         "lbm_solver_iter": (I, [P]),
         "lbm_solver_residuals": (I, [P, P]),
@@ -264,10 +264,13 @@ class Solver:
         _check(lib().lbm_solver_step(self.h, nsteps, C.byref(done)))
         return done.value
 
-    def step_timed(self, nsteps: int):
-        tot, bulk = C.c_double(), C.c_double()
-        _check(lib().lbm_solver_step_timed(self.h, nsteps, C.byref(tot), C.byref(bulk)))
-        return tot.value, bulk.value
+    def step_timed(self, nsteps: int, sample_every: int = 1):
+        """Run nsteps; returns (stream ms for all steps, average ms of one
+        sampled step-kernel launch)."""
+        tot, kern = C.c_double(), C.c_double()
+        _check(lib().lbm_solver_step_timed(self.h, nsteps, sample_every, C.byref(tot),
+                                           C.byref(kern)))
+        return tot.value, kern.value
 
     def sync(self) -> None:
         _check(lib().lbm_solver_sync(self.h))

@@ -81,6 +81,8 @@ struct lbm_solver {
   int nparts = 0;
   int nb_bulk = 0, nb_bnd = 0, nb_out = 0;
   bool fused = true;          // bulk + boundary-list nodes in one launch
+  bool use_graph = true;      // replay captured two-step graphs for plain steps
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by starting buffer parity
   lbmdev::KParams kp{};
   // host state
   int iter = 0;

@@ -353,12 +353,14 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   // ---- grids
   const int BS = o->block_size > 0 ? o->block_size : 256;
   const int count = kp.bulk_end - kp.bulk_begin;
-  const int G = o->xcd_group > 0 ? o->xcd_group : 0;
-  const int align = 8 * (G > 0 ? G : 1);
+  // grouped XCD order, runs of 4 tiles (measured best on the 4097^2 lattice)
+  const int G = o->xcd_group > 0 ? o->xcd_group : 4;
+  const int align = 8 * G;
   s->nb_bulk = ((count + BS - 1) / BS + align - 1) / align * align;
   kp.xcd_group = G;
   if (o->tune_flags & LBM_TUNE_XCD_REMAP) kp.flags |= KF_XCD_REMAP;
   s->fused = BS == 256 && !(o->tune_flags & LBM_TUNE_SPLIT_BOUNDARY);
+  s->use_graph = !(o->tune_flags & LBM_TUNE_NO_GRAPH);
   s->nb_bnd = (s->nbn + 255) / 256;
   s->nb_out = (s->norows + 63) / 64;
   // per-tile summary of the classes (tiles of BS cells in launch order)
@@ -480,6 +482,38 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   return LBM_OK;
 }
 
+// Plain step (no residual work) of one buffer parity: the launches a graph holds.
+static int launch_plain_step(lbm_solver* s, int src_buf) {
+  Launch L = make_launch(s, src_buf);
+  L.kp.res_kind = RK_NONE;
+  if (s->fused) {
+    HC(pass(s, MODE_STEP, PASS_FUSED, L));
+  } else {
+    HC(pass(s, MODE_STEP, PASS_BULK, L));
+    HC(pass(s, MODE_STEP, PASS_BOUNDARY, L));
+  }
+  HC(pass(s, MODE_STEP, PASS_OUTLET, L));
+  return LBM_OK;
+}
+
+// Capture steps (parity, parity^1): after replay the state is back in
+// buf[parity] two iterations later.  Kernel nodes only (no memset nodes).
+static int build_graph(lbm_solver* s, int parity) {
+  hipGraph_t g = nullptr;
+  HC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_plain_step(s, parity);
+  if (rc == LBM_OK) rc = launch_plain_step(s, parity ^ 1);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc != LBM_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HC(e);
+  e = hipGraphInstantiate(&s->gexec[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return check_hip(e, "hipGraphInstantiate");
+}
+
 extern "C" {
 
 int lbm_solver_create(const lbm_case* c, const lbm_options* o, lbm_solver** out) {
@@ -507,11 +541,14 @@ void lbm_solver_destroy(lbm_solver* s) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  for (hipGraphExec_t g : s->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
-static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_t>* evs) {
+static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_t>* evs,
+                     int sample_every = 1) {
   if (done) *done = 0;
   if (s->diverged) {
     set_error("solver diverged");
@@ -532,19 +569,38 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
         break;
     }
     const int rk = o.resid_kind;
-    const bool eval = rk == LBM_RESID_REFERENCE ||
-                      ((rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY) && (s->iter + 1) % K == 0);
+    auto evals_at = [&](int it) {  // does the step that ends at iteration it+1 evaluate?
+      return rk == LBM_RESID_REFERENCE ||
+             ((rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY) && (it + 1) % K == 0);
+    };
+    const bool eval = evals_at(s->iter);
+    // Two plain steps in a row: replay the captured graph.
+    if (s->use_graph && t + 1 < nsteps && !eval && !evals_at(s->iter + 1) && !stop_can_trigger &&
+        (!o.use_stop_rule || s->iter + 2 <= o.max_iterations) &&
+        !(evs && (t % sample_every == 0 || (t + 1) % sample_every == 0))) {
+      if (!s->gexec[s->cur]) {
+        int rc = build_graph(s, s->cur);
+        if (rc) return rc;
+      }
+      HC(hipGraphLaunch(s->gexec[s->cur], s->stream));
+      s->iter += 2;
+      ran += 2;
+      t += 1;
+      continue;
+    }
     const int mode = (eval && (rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY)) ? MODE_RESID : MODE_STEP;
     Launch L = make_launch(s, s->cur);
     L.kp.res_kind = mode == MODE_RESID ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
     if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
-    if (evs) HC(hipEventRecord((*evs)[2 * t], s->stream));
+    const bool timed = evs && (t % sample_every == 0);
+    const size_t ei = 2 * (size_t)(t / (sample_every > 0 ? sample_every : 1));
+    if (timed) HC(hipEventRecord((*evs)[ei], s->stream));
     if (s->fused) {
       HC(pass(s, mode, PASS_FUSED, L));  // bulk tiles + boundary-list nodes, one launch
-      if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
+      if (timed) HC(hipEventRecord((*evs)[ei + 1], s->stream));
     } else {
       HC(pass(s, mode, PASS_BULK, L));
-      if (evs) HC(hipEventRecord((*evs)[2 * t + 1], s->stream));
+      if (timed) HC(hipEventRecord((*evs)[ei + 1], s->stream));
       HC(pass(s, mode, PASS_BOUNDARY, L));
     }
     HC(pass(s, mode, PASS_OUTLET, L));
@@ -579,30 +635,36 @@ int lbm_solver_step(lbm_solver* s, int nsteps, int* done) {
   return step_impl(s, nsteps, done, nullptr);
 }
 
-int lbm_solver_step_timed(lbm_solver* s, int nsteps, double* total_ms, double* bulk_ms) {
+int lbm_solver_step_timed(lbm_solver* s, int nsteps, int sample_every, double* total_ms,
+                          double* kernel_ms) {
   if (!s || nsteps < 0) return LBM_EINVAL;
-  while ((int)s->ev.size() < 2 * nsteps + 2) {
+  if (sample_every < 1) sample_every = 1;
+  const int nsamp = (nsteps + sample_every - 1) / sample_every;
+  while ((int)s->ev.size() < 2 * nsamp + 2) {
     hipEvent_t e;
     HC(hipEventCreate(&e));
     s->ev.push_back(e);
   }
-  std::vector<hipEvent_t> evs(s->ev.begin(), s->ev.begin() + 2 * nsteps);
-  hipEvent_t e0 = s->ev[2 * nsteps], e1 = s->ev[2 * nsteps + 1];
+  std::vector<hipEvent_t> evs(s->ev.begin(), s->ev.begin() + 2 * nsamp);
+  hipEvent_t e0 = s->ev[2 * nsamp], e1 = s->ev[2 * nsamp + 1];
   HC(hipEventRecord(e0, s->stream));
   int done = 0;
-  int rc = step_impl(s, nsteps, &done, &evs);
+  int rc = step_impl(s, nsteps, &done, &evs, sample_every);
   HC(hipEventRecord(e1, s->stream));
   HC(hipEventSynchronize(e1));
   float ms = 0.f;
   HC(hipEventElapsedTime(&ms, e0, e1));
   double bsum = 0.0;
-  for (int t = 0; t < done; t++) {
+  int nb = 0;
+  for (int t = 0; t < done; t += sample_every) {
     float b = 0.f;
-    HC(hipEventElapsedTime(&b, evs[2 * t], evs[2 * t + 1]));
+    const size_t ei = 2 * (size_t)(t / sample_every);
+    HC(hipEventElapsedTime(&b, evs[ei], evs[ei + 1]));
     bsum += b;
+    nb++;
   }
   if (total_ms) *total_ms = ms;
-  if (bulk_ms) *bulk_ms = bsum;
+  if (kernel_ms) *kernel_ms = nb ? bsum / nb : 0.0;
   s->timing.bulk_ms += bsum;
   return rc;
 }
