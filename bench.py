@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MLUPS (% HBM roofline) at 4097² MRT; time-to-1e-6 residual, learned vs rest"
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "bulk_traffic_pmc.json")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_traffic_pmc.json")
 
 
 def log(*a):
@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--sample-every", type=int, default=8,
                     help="HIP events around every N-th step-kernel launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -132,7 +132,9 @@ def main():
     avg_bulk_s = kern_ms / 1e3
     achieved = bytes_per_launch / avg_bulk_s / 1e9
     traffic = None
-    if os.path.exists(PMC_SUMMARY):
+    default_shape = (args.block in (0, 256) and not args.xcd_remap and not args.split_boundary
+                     and args.xcd_group in (0, 4))
+    if default_shape and os.path.exists(PMC_SUMMARY):
         try:
             pm = json.load(open(PMC_SUMMARY))
             key = f"{args.collision}_{args.precision}_{N}"
