@@ -17,7 +17,8 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "lib", "liblbm.so")
+# LBM_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("LBM_LIB") or os.path.join(_PKG, "lib", "liblbm.so")
 CLI_PATH = os.path.join(_PKG, "bin", "lbm_solve")
 
 OK, EIO, EPARSE, DIVERGED, EHIP, EINVAL, ENOMEM = range(7)

@@ -54,8 +54,10 @@ def _run(cmd, quiet):
         print("\n".join(lines))
 
 
-def build(quiet: bool = True) -> str:
-    """Compile every unit and link liblbm.so and lbm_solve. Returns the .so path."""
+def build(quiet: bool = True, lib_path: str | None = None, defines=()) -> str:
+    """Compile every unit and link liblbm.so and lbm_solve. Returns the .so path.
+    lib_path/defines build a variant library (e.g. -DLBM_NT_LOADS=1) without
+    touching the default one or the CLI."""
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(BINDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
@@ -64,15 +66,19 @@ def build(quiet: bool = True) -> str:
         obj = os.path.join(OBJDIR, src + ".o")
         path = os.path.join(CSRC, src)
         if kind == "hip":
-            cmd = [HIPCC] + HIP_COMMON + extra + ["-c", path, "-o", obj]
+            cmd = [HIPCC] + HIP_COMMON + extra + list(defines) + ["-c", path, "-o", obj]
         else:
             cmd = ["g++"] + HOST_FLAGS + extra + ["-c", path, "-o", obj]
         _run(cmd, quiet)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    target = lib_path or LIB
+    tmp = target + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-Wl,-soname,liblbm.so"], quiet)
-    os.replace(tmp, LIB)
+    os.replace(tmp, target)
+    if lib_path:
+        shutil.rmtree(OBJDIR, ignore_errors=True)
+        return target
     cli_obj = os.path.join(OBJDIR, "lbm_solve.o")
     _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "lbm_solve.cpp"), "-o", cli_obj], quiet)
     _run(["g++", "-o", CLI, cli_obj, "-L", LIBDIR, "-llbm", "-Wl,-rpath,$ORIGIN/../lib",
@@ -83,5 +89,8 @@ def build(quiet: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    print(build(quiet=False))
+    if len(sys.argv) > 2:  # _build.py OUT.so -DNAME=VALUE ...
+        print(build(quiet=False, lib_path=os.path.abspath(sys.argv[1]), defines=sys.argv[2:]))
+    else:
+        print(build(quiet=False))
     sys.exit(0)

@@ -48,25 +48,38 @@ __device__ constexpr int MM[9][9] = {
     {0, 0, 0, 0, 0, 1, -1, 1, -1}};
 
 // ---------------------------------------------------------------- storage
+// Loads of the distributions: every value is read once per step (plus the
+// neighbouring tile's edge line), so LBM_NT_LOADS=1 marks them non-temporal.
+#ifndef LBM_NT_LOADS
+#define LBM_NT_LOADS 0
+#endif
+template <class T>
+__device__ inline T ld(const T* p) {
+  if constexpr (LBM_NT_LOADS)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
 struct StF64 {
   using T = double;
   using R = double;  // arithmetic type
   static constexpr bool DEV = false;
-  __device__ static R load(const T* p, const KParams&) { return *p; }
+  __device__ static R load(const T* p, const KParams&) { return ld(p); }
   __device__ static void store(T* p, R v, const KParams&) { __builtin_nontemporal_store(v, p); }
 };
 struct StF32 {
   using T = float;
   using R = float;
   static constexpr bool DEV = true;
-  __device__ static R load(const T* p, const KParams&) { return *p; }
+  __device__ static R load(const T* p, const KParams&) { return ld(p); }
   __device__ static void store(T* p, R v, const KParams&) { __builtin_nontemporal_store(v, p); }
 };
 struct StF16 {
   using T = _Float16;
   using R = float;
   static constexpr bool DEV = true;
-  __device__ static R load(const T* p, const KParams& kp) { return (float)(*p) * kp.inv_scale_f; }
+  __device__ static R load(const T* p, const KParams& kp) { return (float)ld(p) * kp.inv_scale_f; }
   __device__ static void store(T* p, R v, const KParams& kp) {
     __builtin_nontemporal_store((_Float16)(v * kp.scale_f), p);
   }
