@@ -44,7 +44,8 @@ typedef enum lbm_status {
 enum { LBM_BGKW = 1, LBM_TRT = 2, LBM_MRT = 3 };
 enum { LBM_INLET_PARABOLIC = 1, LBM_INLET_UNIFORM = 2, LBM_INLET_PULSATILE = 3 };
 enum { LBM_CURVED_ON = 1, LBM_CURVED_OFF = 2 };
-enum { LBM_OUTLET_VELOCITY = 1, LBM_OUTLET_OPEN2 = 2, LBM_OUTLET_OPEN1 = 3 };
+enum { LBM_OUTLET_VELOCITY = 1, LBM_OUTLET_OPEN2 = 2, LBM_OUTLET_OPEN1 = 3,
+       LBM_OUTLET_PRESSURE = 4 /* Zou-He rho = outlet_rho; not in the reference */ };
 enum { LBM_FORMAT_CSV = 1, LBM_FORMAT_TECPLOT = 2 };
 
 /* Storage / arithmetic of the distributions. */
@@ -100,6 +101,9 @@ typedef struct lbm_options {
   int    block_size;       /* bulk-kernel workgroup size: 0 (tuned default), 256, 512 or 1024 */
   int    tune_flags;       /* LBM_TUNE_* launch-shape switches; 0 = tuned default */
   int    xcd_group;        /* runs of this many adjacent bulk tiles per XCD (1 = hardware order); 0 = tuned default */
+  double outlet_rho;       /* LBM_OUTLET_PRESSURE density; 0 = rho_ini */
+  int    force_id;         /* momentum-exchange force on the wall links of this boundary id (0 = off) */
+  int    force_every;      /* force sampled every K steps (lbm_solver_force_history) */
 } lbm_options;
 
 /* Launch-shape switches (results are identical; only speed changes).
@@ -191,6 +195,11 @@ int lbm_solver_residuals(lbm_solver* s, double out[4]);
 /* Residual rows recorded since the last call (for residuals.dat): iters[k],
  * vals[4k..4k+3].  *count receives how many were written (<= cap). */
 int lbm_solver_residual_history(lbm_solver* s, int cap, int* iters, double* vals, int* count);
+/* Momentum-exchange force (Fx, Fy) on the wall links of boundary id
+ * force_id, sampled every force_every steps, since the last call: iters[k],
+ * fxy[2k..2k+1].  Replaces the reference's drag/lift placeholder
+ * (serial/CellFunctions.c:489-493) for bodies; no counterpart there. */
+int lbm_solver_force_history(lbm_solver* s, int cap, int* iters, double* fxy, int* count);
 /* U, V, Rho of every cell after the last iteration (n*m each; may be NULL). */
 int lbm_solver_get_macro(lbm_solver* s, double* rho, double* u, double* v);
 /* F (post-boundary distributions) and METAF (post-collision) of every cell

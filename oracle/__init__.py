@@ -43,7 +43,7 @@ class Params(C.Structure):
         ("init_rest", C.c_int), ("trt_fp64", C.c_int), ("trt_lambda", C.c_double),
         ("mrt_textbook", C.c_int), ("mrt_custom_s", C.c_int),
         ("mrt_s", C.c_double * 9), ("resid_kind", C.c_int), ("resid_every", C.c_int),
-        ("outlet1_lid", C.c_int),
+        ("outlet1_lid", C.c_int), ("outlet_rho", C.c_double),
     ]
 
 
@@ -81,6 +81,7 @@ def lib():
         L.or_write_results.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
         L.or_mrt_init.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_d2q9.argtypes = [C.c_void_p] * 5 + [C.c_int]
+        L.or_force.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         _lib = L
     return _lib
 
@@ -155,6 +156,11 @@ class Oracle:
     def residuals(self) -> np.ndarray:
         out = np.zeros(4)
         lib().or_residuals(self.h, out.ctypes.data)
+        return out
+
+    def force(self, bid: int) -> np.ndarray:
+        out = np.zeros(2)
+        lib().or_force(self.h, bid, out.ctypes.data)
         return out
 
     def omegas(self) -> np.ndarray:

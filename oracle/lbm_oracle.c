@@ -463,6 +463,19 @@ static void boundaries_step(or_solver* s)
             c->f[8] = 2 * a[8] - b[8];
           }
           break;
+        case 4: /* pressure outlet, Zou & He (1997), east side; not in the reference */
+          if (c->bc[1] == 3) {
+            double* F = c->f;
+            const double ro = s->p.outlet_rho > 0 ? s->p.outlet_rho : (double)s->p.rho_ini;
+            double num = F[0] + F[2] + F[4] + 2.0 * (F[1] + F[5] + F[8]);
+            double ux = num / ro - 1.0;
+            double t = ro * ux / 6.0;
+            double h = 0.5 * (F[2] - F[4]);
+            F[3] = F[1] - 4.0 * t;
+            F[7] = F[5] + h - t;
+            F[6] = F[8] - h - t;
+          }
+          break;
         case 3: /* 1st order, cuda/cuda_step3/GpuBoundaries.cu:200-206 */
           if (c->bc[1] == 3 && idx >= 1) {
             const double* a = s->cells[idx - 1].f;
@@ -748,6 +761,24 @@ void or_dims(const or_solver* s, int* n, int* m, double* delta, int* num_inlet,
   if (num_inlet) *num_inlet = s->num_inlet;
   if (min_inlet_y) *min_inlet_y = s->min_inlet_y;
   if (max_inlet_y) *max_inlet_y = s->max_inlet_y;
+}
+
+void or_force(const or_solver* s, int bid, double out[2])
+{
+  const long nc = (long)s->n * s->m;
+  double fx = 0.0, fy = 0.0;
+  for (long i = 0; i < nc; i++) {
+    const or_cell* c = &s->cells[i];
+    if (c->fluid != 1 || c->boundary_id != bid) continue;
+    for (int k = 1; k < 9; k++)
+      if (c->bc[k] == 1) {
+        double g = c->metaf[k] + c->f[s->opp[k]];
+        fx += s->cx[k] * g;
+        fy += s->cy[k] * g;
+      }
+  }
+  out[0] = fx;
+  out[1] = fy;
 }
 
 int or_iter(const or_solver* s) { return s->iter; }

@@ -62,7 +62,8 @@ typedef struct or_params {
   int    inlet_profile;     /* 1 parabolic, 2 uniform, 3 pulsatile (no-op) */
   int    collision;         /* 1 BGKW, 2 TRT, 3 MRT */
   int    curved;            /* 1 curved (Bouzidi) walls, 2 half-way bounce-back */
-  int    outlet_profile;    /* 1 velocity outlet, 2 open 2nd order, 3 open 1st order */
+  int    outlet_profile;    /* 1 velocity outlet, 2 open 2nd order, 3 open 1st order,
+                               4 pressure outlet (Zou-He, rho = outlet_rho) */
   int    iterations;
   int    autosave_every, autosave_after, postproc, drag_lift;
   float  crit_v, crit_rho;
@@ -77,6 +78,7 @@ typedef struct or_params {
   int    resid_kind;        /* 0: reference loop, 1: full field, 2: velocity, 3: none */
   int    resid_every;       /* full-field / velocity residual: K steps between checks */
   int    outlet1_lid;       /* reserved, keep 0 */
+  double outlet_rho;        /* outlet_profile 4: outlet density (0 -> rho_ini) */
 } or_params;
 
 typedef struct or_solver or_solver;
@@ -108,6 +110,12 @@ int  or_cell_size(void);                   /* sizeof(or_cell) */
 /* Copies macros (n*m each) and distributions (n*m*9, cell-major). */
 void or_get_macro(const or_solver* s, double* rho, double* u, double* v);
 void or_get_f(const or_solver* s, double* f, double* metaf);
+
+/* Momentum-exchange force on the wall links (BC_ID == 1) of the fluid cells
+ * whose BoundaryID is bid, from the current F (post-boundary) and METAF:
+ * sum of c_k (METAF_k + F_opp(k)).  Not in the reference (its drag/lift is a
+ * placeholder, serial/CellFunctions.c:489-493). */
+void or_force(const or_solver* s, int bid, double out[2]);
 
 /* Writers with the reference's formats (serial/FileIO.c:298-427). */
 int  or_write_results(const or_solver* s, const char* path, int final_layout,

@@ -22,6 +22,7 @@ LIB_PATH = os.environ.get("LBM_LIB") or os.path.join(_PKG, "lib", "liblbm.so")
 CLI_PATH = os.path.join(_PKG, "bin", "lbm_solve")
 
 OK, EIO, EPARSE, DIVERGED, EHIP, EINVAL, ENOMEM = range(7)
+OUTLET_VELOCITY, OUTLET_OPEN2, OUTLET_OPEN1, OUTLET_PRESSURE = 1, 2, 3, 4
 BGKW, TRT, MRT = 1, 2, 3
 PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 INIT_REFERENCE, INIT_REST, INIT_FIELD = 0, 1, 2
@@ -36,7 +37,8 @@ EXPORTS = [
     "lbm_case_write", "lbm_case_info_get", "lbm_case_cells", "lbm_case_free",
     "lbm_solver_create", "lbm_solver_create_field", "lbm_solver_destroy", "lbm_solver_step",
     "lbm_solver_step_timed", "lbm_solver_sync", "lbm_solver_iter", "lbm_solver_residuals",
-    "lbm_solver_residual_history", "lbm_solver_get_macro", "lbm_solver_get_f",
+    "lbm_solver_residual_history", "lbm_solver_force_history", "lbm_solver_get_macro",
+    "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
     "lbm_solver_fused", "lbm_device_copy_gbs", "lbm_run", "lbm_iteration",
 ]
@@ -72,7 +74,8 @@ class Options(C.Structure):
         ("resid_kind", C.c_int), ("resid_every", C.c_int), ("use_stop_rule", C.c_int),
         ("crit_v", C.c_float), ("crit_rho", C.c_float), ("max_iterations", C.c_int),
         ("f16_scale", C.c_float), ("block_size", C.c_int), ("tune_flags", C.c_int),
-        ("xcd_group", C.c_int),
+        ("xcd_group", C.c_int), ("outlet_rho", C.c_double), ("force_id", C.c_int),
+        ("force_every", C.c_int),
     ]
 
 
@@ -124,6 +127,7 @@ def lib() -> C.CDLL:
         "lbm_solver_iter": (I, [P]),
         "lbm_solver_residuals": (I, [P, P]),
         "lbm_solver_residual_history": (I, [P, I, P, P, C.POINTER(I)]),
+        "lbm_solver_force_history": (I, [P, I, P, P, C.POINTER(I)]),
         "lbm_solver_get_macro": (I, [P, P, P, P]),
         "lbm_solver_get_f": (I, [P, P, P]),
         "lbm_solver_write_results": (I, [P, C.c_char_p, I, I, I]),
@@ -299,6 +303,21 @@ class Solver:
                 break
         return np.concatenate(its), np.concatenate(vals)
 
+    def force_history(self):
+        """(iterations, forces[k, 2]) sampled since the last call."""
+        its, vals = [], []
+        while True:
+            cap = 4096
+            i = np.zeros(cap, np.int32)
+            v = np.zeros((cap, 2))
+            cnt = C.c_int()
+            _check(lib().lbm_solver_force_history(self.h, cap, _ptr(i), _ptr(v), C.byref(cnt)))
+            its.append(i[:cnt.value])
+            vals.append(v[:cnt.value])
+            if cnt.value < cap:
+                break
+        return np.concatenate(its), np.concatenate(vals)
+
     def macro(self):
         nc = self.n * self.m
         rho, u, v = np.zeros(nc), np.zeros(nc), np.zeros(nc)
@@ -359,6 +378,29 @@ def device_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:
     g = C.c_double()
     _check(lib().lbm_device_copy_gbs(nbytes, reps, C.byref(g)))
     return g.value
+
+
+def strouhal(lift: np.ndarray, sample_steps: int, diameter: float, velocity: float,
+             discard: float = 0.5) -> float:
+    """Strouhal number f D / U from a lift series sampled every `sample_steps`
+    lattice steps: the dominant frequency of the (mean-removed, Hann-windowed)
+    last (1 - discard) of the series, refined by parabolic interpolation of
+    the spectral peak."""
+    x = np.asarray(lift, dtype=np.float64)
+    x = x[int(len(x) * discard):]
+    x = x - x.mean()
+    n = len(x)
+    if n < 16:
+        raise ValueError("lift series too short")
+    spec = np.abs(np.fft.rfft(x * np.hanning(n)))
+    k = int(np.argmax(spec[1:]) + 1)
+    delta = 0.0
+    if 1 <= k < len(spec) - 1:
+        a, b, c = np.log(spec[k - 1] + 1e-300), np.log(spec[k] + 1e-300), np.log(spec[k + 1] + 1e-300)
+        den = a - 2 * b + c
+        delta = 0.5 * (a - c) / den if den != 0 else 0.0
+    f = (k + delta) / (n * sample_steps)  # cycles per lattice step
+    return f * diameter / velocity
 
 
 def gpu_available() -> bool:

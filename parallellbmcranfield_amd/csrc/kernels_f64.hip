@@ -28,28 +28,29 @@ hipError_t launch_init_f64(int coll, int f0_kind, const Launch& L) {
 hipError_t launch_fill_solid_f64(const Launch& L, void* buf) { return fill_impl<StF64>(L, buf); }
 
 // ---------------------------------------------------------------- reductions
-// Sum of nparts x 4 partials in a fixed order: thread t adds parts t, t+256,
+// Sum of nparts x 6 partials in a fixed order: thread t adds parts t, t+256,
 // ... in sequence, then a fixed-shape LDS tree.  Same result on every run.
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ part,
                                                          int nparts, double* __restrict__ out) {
-  __shared__ double red[4][256];
+  constexpr int W = 6;
+  __shared__ double red[W][256];
   const int t = threadIdx.x;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  double s[W] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = t; i < nparts; i += 256) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) s[c] = s[c] + part[(size_t)i * 4 + c];
+    for (int c = 0; c < W; c++) s[c] = s[c] + part[(size_t)i * W + c];
   }
 #pragma unroll
-  for (int c = 0; c < 4; c++) red[c][t] = s[c];
+  for (int c = 0; c < W; c++) red[c][t] = s[c];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (t < w) {
 #pragma unroll
-      for (int c = 0; c < 4; c++) red[c][t] = red[c][t] + red[c][t + w];
+      for (int c = 0; c < W; c++) red[c][t] = red[c][t] + red[c][t + w];
     }
     __syncthreads();
   }
-  if (t < 4) out[t] = red[t][0];
+  if (t < W) out[t] = red[t][0];
 }
 
 // serial/ComputeResiduals.c:19-30: for i < m, for j < n, for k < 9,

@@ -94,6 +94,9 @@ struct lbm_solver {
   std::vector<double> hist_val;
   int pending = 0;            // device hist entries not yet copied
   std::vector<int> pending_iter;
+  std::vector<int> pending_what;  // bit 0 residual row, bit 1 force row
+  std::vector<int> force_iter;
+  std::vector<double> force_val;  // fx, fy per row
   bool diverged = false;
   int diverged_iter = -1;
   std::vector<double> rho0, u0, v0;  // initial macros (reference order), for t = 0

@@ -23,28 +23,23 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Deterministic per-workgroup sum of 4 values -> part[slot*4 + c].
+// Deterministic per-workgroup sum of NP values -> part[slot*NP + c].
+constexpr int NP = 6;  // 4 residual terms + momentum-exchange force (x, y)
 template <int BS>
-__device__ inline void block_sum4(double a0, double a1, double a2, double a3, double* part,
-                                  int slot) {
-  __shared__ double red[BS / 64][4];
-  a0 = wave_sum(a0);
-  a1 = wave_sum(a1);
-  a2 = wave_sum(a2);
-  a3 = wave_sum(a3);
+__device__ inline void block_sum(const double (&a)[NP], double* part, int slot) {
+  __shared__ double red[BS / 64][NP];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[wv][0] = a0;
-    red[wv][1] = a1;
-    red[wv][2] = a2;
-    red[wv][3] = a3;
+#pragma unroll
+  for (int c = 0; c < NP; c++) {
+    const double v = wave_sum(a[c]);
+    if (lane == 0) red[wv][c] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < NP) {
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < BS / 64; w++) s += red[w][threadIdx.x];
-    part[(size_t)slot * 4 + threadIdx.x] = s;
+    part[(size_t)slot * NP + threadIdx.x] = s;
   }
 }
 
@@ -109,6 +104,19 @@ __device__ inline void apply_bc(typename S::R F[9], const BNode& nd, const typen
     for (int k = 0; k < 9; k++)
       if (nd.bc[k] == 1) F[OPP[k]] = F[k];
   }
+  if (p.outlet_profile == 4 && nd.bc[1] == 3) {
+    // East pressure outlet, Zou & He (1997): rho = rho_out, v = 0; unknown
+    // populations 3, 6, 7 from mass and momentum balance.
+    const R ro = S::DEV ? (R)p.rho_out_f : (R)p.rho_out;
+    R num = F[0] + F[2] + F[4] + (R)2.0 * (F[1] + F[5] + F[8]);
+    if constexpr (S::DEV) num = num + roff;
+    const R ux = num / ro - (R)1.0;
+    const R t = ro * ux / (R)6.0;
+    const R h = (R)0.5 * (F[2] - F[4]);
+    F[3] = F[1] - (R)4.0 * t;
+    F[7] = F[5] + h - t;
+    F[6] = F[8] - h - t;
+  }
   if (p.outlet_profile == 1 && nd.bc[1] == 3) {
     // OutletBoundaries: serial/BoundaryConditions.c:67-81
     R num = F[0] + F[2] + F[4] + (R)2.0 * (F[1] + F[5] + F[8]);
@@ -126,7 +134,7 @@ __device__ inline void apply_bc(typename S::R F[9], const BNode& nd, const typen
 template <class S, int COLL, int MODE, bool BND>
 __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
                                    typename S::T* dst, const KParams& p, const KOut& o, int idx,
-                                   const BNode* nd, double acc[4]) {
+                                   const BNode* nd, double (&acc)[NP]) {
   using R = typename S::R;
   R Rh, U, V, dr = 0;
   if constexpr (S::DEV) {
@@ -142,6 +150,24 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
       const double rr = (double)Rh;
       drag = rr / 3 * (20 - nd->x) / 5;
       lift = rr / 3 * (20 - nd->y) / 5;
+    }
+  }
+  // momentum exchange on the links of boundary id force_id: the body receives
+  // c_k (f*_k(x) + f_opp(k)(x, t+1)) per wall link (Ladd / Mei et al.)
+  if constexpr (MODE == 1 && BND) {
+    if (p.force_on && nd->boundary_id == p.force_id) {
+      double fx = 0.0, fy = 0.0;
+#pragma unroll
+      for (int k = 1; k < 9; k++) {
+        if (nd->bc[k] == 1) {
+          const double mk = absval<S>(S::load(src + k * p.plane + idx, p), k, p);
+          const double fo = absval<S>(F[OPP[k]], OPP[k], p);
+          fx += (double)CX[k] * (mk + fo);
+          fy += (double)CY[k] * (mk + fo);
+        }
+      }
+      acc[4] += fx;
+      acc[5] += fy;
     }
   }
   // residual terms of step t
@@ -228,7 +254,7 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
   }
   const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
   const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if constexpr (MODE != 1) {
     if (!active) return;
   }
@@ -238,7 +264,7 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
     for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
     finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc);
   }
-  if constexpr (MODE == 1) block_sum4<BS>(acc[0], acc[1], acc[2], acc[3], o.part, blockIdx.x);
+  if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
 
 template <class S>
@@ -260,7 +286,7 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
                                                   KOut o) {
   using R = typename S::R;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
     const BNode& nd = bn[i];  // fields read where used
     R F[9];
@@ -273,7 +299,7 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
     }
     finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
   }
-  if constexpr (MODE == 1) block_sum4<256>(acc[0], acc[1], acc[2], acc[3], o.part, p.nb_bulk + blockIdx.x);
+  if constexpr (MODE == 1) block_sum<256>(acc, o.part, p.nb_bulk + blockIdx.x);
 }
 
 // One launch per step: workgroups [0, nb_bnd) take BS boundary-list nodes
@@ -288,7 +314,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
                                              KOut o) {
   using R = typename S::R;
   const int nbb = p.nb_bnd;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if ((int)blockIdx.x < nbb) {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
@@ -323,7 +349,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
       finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc);
     }
   }
-  if constexpr (MODE == 1) block_sum4<BS>(acc[0], acc[1], acc[2], acc[3], o.part, blockIdx.x);
+  if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
 
 // Open outlet (Iterate.c:533-541, 2nd order; cuda_step3 GpuBoundaries.cu:200-206,
@@ -339,7 +365,7 @@ __global__ __launch_bounds__(64) void k_outlet(const typename S::T* __restrict__
                                                KParams p, KOut o) {
   using R = typename S::R;
   const int r = blockIdx.x * 64 + threadIdx.x;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (r < nrows) {
     R* side = (R*)o.side;
     for (int t = row_start[r]; t < row_start[r + 1]; t++) {
@@ -365,7 +391,7 @@ __global__ __launch_bounds__(64) void k_outlet(const typename S::T* __restrict__
     }
   }
   if constexpr (MODE == 1)
-    block_sum4<64>(acc[0], acc[1], acc[2], acc[3], o.part, p.nb_bulk + p.nb_bnd + blockIdx.x);
+    block_sum<64>(acc, o.part, p.nb_bulk + p.nb_bnd + blockIdx.x);
 }
 
 // First collision (serial/Iterate.c:268 on the state CellIni leaves): from

@@ -24,14 +24,16 @@ struct KParams {
   int flags;
   int res_kind;       // ResKind for this pass
   int curved;         // 1 interpolated (Bouzidi) walls, else half-way bounce-back
-  int outlet_profile; // 1 velocity, 2 open 2nd order, 3 open 1st order
+  int outlet_profile; // 1 velocity, 2 open 2nd order, 3 open 1st order, 4 pressure
+  int force_id;       // boundary id whose links give the momentum-exchange force
+  int force_on;       // this pass accumulates the force (MODE 1 only)
   int drag_lift;      // boundary id for the drag/lift placeholder (0 = off)
   int nb_bulk;        // number of bulk workgroups (partials offset of the boundary pass)
   int nb_bnd;         // number of boundary workgroups
   int xcd_group;      // > 0: grouped XCD tile order with this run length
   // physics
-  double omega, omega_a, rho0;
-  float omega_f, omega_a_f, rho0_f;
+  double omega, omega_a, rho0, rho_out;
+  float omega_f, omega_a_f, rho0_f, rho_out_f;
   float scale_f, inv_scale_f;
   double stmiv[9][9];  // FP64 MRT: M^-1 S exactly as MRTInitializer forms it
   float mrt_e[9];      // FP32 MRT: s_l / |M_l|^2
@@ -72,7 +74,7 @@ struct KOut {
   double* v;
   double* f;        // padded [9][plane] post-boundary F (observe, optional)
   double* metaf;    // padded [9][plane] own METAF (observe, optional)
-  double* part;     // per-workgroup residual partials [nb][4]
+  double* part;     // per-workgroup partials [nb][6]: residual terms, force x, y
   double* snap;     // velocity residual snapshot, padded [3][plane] (rho, u, v)
   double* cres;     // reference-residual cells: [m][12] (9 diffs, drag, lift, written flag)
   void* side;       // outlet side buffer [slots][9] (compute type)

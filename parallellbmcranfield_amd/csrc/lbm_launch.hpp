@@ -39,7 +39,7 @@ hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& 
 hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L);
 hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf);
 
-// Deterministic FP64 sum of nparts x 4 partials -> out[4].
+// Deterministic FP64 sum of nparts x 6 partials -> out[6].
 hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st);
 // The reference residual's sweep (serial/ComputeResiduals.c:19-30) over the
 // cres rows of the first m cells -> out[4] = (sum, drag, lift, 0).

@@ -32,7 +32,8 @@ using namespace lbmhost;
 
 namespace {
 
-constexpr int HCAP = 1024;  // device residual-history slots
+constexpr int HCAP = 1024;  // device history slots
+constexpr int HW = 8;       // slot: raw residual [0..3], force [4..5], unused [6..7]
 
 int dev_alloc(lbm_solver* s, void** p, size_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -124,13 +125,21 @@ bool finalize_resid(lbm_solver* s, const double raw[4], double r[4]) {
 
 int flush_history(lbm_solver* s) {
   if (s->pending == 0) return LBM_OK;
-  std::vector<double> raw((size_t)s->pending * 4);
+  std::vector<double> raw((size_t)s->pending * HW);
   HC(hipMemcpyAsync(raw.data(), s->d_hist, raw.size() * sizeof(double), hipMemcpyDeviceToHost,
                     s->stream));
   HC(hipStreamSynchronize(s->stream));
   for (int i = 0; i < s->pending; i++) {
+    const double* row = &raw[(size_t)i * HW];
+    const int what = s->pending_what[i];
+    if ((what & 2) && !s->diverged) {
+      s->force_iter.push_back(s->pending_iter[i]);
+      s->force_val.push_back(row[4]);
+      s->force_val.push_back(row[5]);
+    }
+    if (!(what & 1)) continue;
     double r[4];
-    const bool ok = finalize_resid(s, &raw[(size_t)i * 4], r);
+    const bool ok = finalize_resid(s, row, r);
     if (s->diverged) continue;  // rows after the divergence are not reported
     std::memcpy(s->resid, r, sizeof r);
     if (!ok) {
@@ -144,6 +153,7 @@ int flush_history(lbm_solver* s) {
   }
   s->pending = 0;
   s->pending_iter.clear();
+  s->pending_what.clear();
   return s->diverged ? LBM_DIVERGED : LBM_OK;
 }
 
@@ -250,6 +260,9 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   kp.omega_a_f = (float)omega_a;
   kp.rho0 = (double)o->rho_ini;
   kp.rho0_f = o->rho_ini;
+  kp.rho_out = o->outlet_rho > 0 ? o->outlet_rho : (double)o->rho_ini;
+  kp.rho_out_f = (float)kp.rho_out;
+  kp.force_id = o->force_id;
   const float scale = o->f16_scale > 0 ? o->f16_scale : 4096.0f;
   kp.scale_f = scale;
   kp.inv_scale_f = 1.0f / scale;
@@ -393,8 +406,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (!rc) rc = dev_alloc(s, (void**)&s->d_rho, sizeof(double) * (size_t)s->plane);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_u, sizeof(double) * (size_t)s->plane);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_v, sizeof(double) * (size_t)s->plane);
-  if (!rc) rc = dev_alloc(s, (void**)&s->d_part, sizeof(double) * 4 * (size_t)s->nparts);
-  if (!rc) rc = dev_alloc(s, (void**)&s->d_hist, sizeof(double) * 4 * HCAP);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_part, sizeof(double) * 6 * (size_t)s->nparts);
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_hist, sizeof(double) * HW * HCAP);
   if (!rc && o->resid_kind == LBM_RESID_REFERENCE)
     rc = dev_alloc(s, (void**)&s->d_cres, sizeof(double) * 12 * (size_t)m);
   if (!rc && o->resid_kind == LBM_RESID_VELOCITY)
@@ -573,9 +586,15 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
       return rk == LBM_RESID_REFERENCE ||
              ((rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY) && (it + 1) % K == 0);
     };
-    const bool eval = evals_at(s->iter);
+    const bool reval = evals_at(s->iter);
+    auto forces_at = [&](int it) {
+      return o.force_id != 0 && o.force_every > 0 && (it + 1) % o.force_every == 0;
+    };
+    const bool feval = forces_at(s->iter);
+    const bool eval = reval || feval;
     // Two plain steps in a row: replay the captured graph.
-    if (s->use_graph && t + 1 < nsteps && !eval && !evals_at(s->iter + 1) && !stop_can_trigger &&
+    if (s->use_graph && t + 1 < nsteps && !eval && !evals_at(s->iter + 1) &&
+        !forces_at(s->iter + 1) && !stop_can_trigger &&
         (!o.use_stop_rule || s->iter + 2 <= o.max_iterations) &&
         !(evs && (t % sample_every == 0 || (t + 1) % sample_every == 0))) {
       if (!s->gexec[s->cur]) {
@@ -588,9 +607,11 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
       t += 1;
       continue;
     }
-    const int mode = (eval && (rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY)) ? MODE_RESID : MODE_STEP;
+    const bool rpart = reval && (rk == LBM_RESID_FULL || rk == LBM_RESID_VELOCITY);
+    const int mode = (rpart || feval) ? MODE_RESID : MODE_STEP;
     Launch L = make_launch(s, s->cur);
-    L.kp.res_kind = mode == MODE_RESID ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
+    L.kp.res_kind = rpart ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
+    L.kp.force_on = feval ? 1 : 0;
     if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
     const bool timed = evs && (t % sample_every == 0);
     const size_t ei = 2 * (size_t)(t / (sample_every > 0 ? sample_every : 1));
@@ -605,18 +626,18 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     }
     HC(pass(s, mode, PASS_OUTLET, L));
     if (eval) {
-      double* slot = s->d_hist + (size_t)s->pending * 4;
-      if (rk == LBM_RESID_REFERENCE)
+      double* slot = s->d_hist + (size_t)s->pending * HW;
+      if (mode == MODE_RESID) HC(launch_reduce_partials(s->d_part, s->nparts, slot, s->stream));
+      if (reval && rk == LBM_RESID_REFERENCE)
         HC(launch_ref_residual(s->d_cres, s->cs->m, s->cs->n, slot, s->stream));
-      else
-        HC(launch_reduce_partials(s->d_part, s->nparts, slot, s->stream));
       s->pending_iter.push_back(s->iter);
+      s->pending_what.push_back((reval ? 1 : 0) | (feval ? 2 : 0));
       s->pending++;
     }
     s->cur ^= 1;
     s->iter++;
     ran++;
-    if (s->pending == HCAP || (eval && stop_can_trigger)) {
+    if (s->pending == HCAP || (reval && stop_can_trigger)) {
       int rc = flush_history(s);
       if (rc) {
         if (done) *done = ran;
@@ -694,6 +715,24 @@ int lbm_solver_residual_history(lbm_solver* s, int cap, int* iters, double* vals
   }
   s->hist_iter.erase(s->hist_iter.begin(), s->hist_iter.begin() + k);
   s->hist_val.erase(s->hist_val.begin(), s->hist_val.begin() + 4 * (size_t)k);
+  *count = k;
+  return LBM_OK;
+}
+
+int lbm_solver_force_history(lbm_solver* s, int cap, int* iters, double* fxy, int* count) {
+  if (!s || cap < 0 || !count) return LBM_EINVAL;
+  (void)flush_history(s);
+  const int have = (int)s->force_iter.size();
+  const int k = std::min(cap, have);
+  for (int i = 0; i < k; i++) {
+    if (iters) iters[i] = s->force_iter[i];
+    if (fxy) {
+      fxy[2 * i] = s->force_val[(size_t)2 * i];
+      fxy[2 * i + 1] = s->force_val[(size_t)2 * i + 1];
+    }
+  }
+  s->force_iter.erase(s->force_iter.begin(), s->force_iter.begin() + k);
+  s->force_val.erase(s->force_val.begin(), s->force_val.begin() + 2 * (size_t)k);
   *count = k;
   return LBM_OK;
 }

@@ -95,6 +95,11 @@ SWITCH_CASES = {
                                              uavg=0.05), 80),
     "obstacle_curved_mrt_rest": (lambda: meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4),
                                  dict(collision=3, curved=1, init_rest=1), 80),
+    "pressure_outlet_mrt": (lambda: meshgen.channel(30, 12),
+                            dict(collision=3, outlet_profile=4, outlet_rho=1.01, uavg=0.05), 80),
+    "pressure_outlet_parabolic_curved": (lambda: meshgen.channel(30, 12),
+                                         dict(collision=1, outlet_profile=4, inlet_profile=1,
+                                              curved=1, uavg=0.05), 60),
     "drag_lift_full": (lambda: meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4),
                        dict(collision=1, curved=1, drag_lift=5, resid_kind=1, resid_every=1), 30),
 }
@@ -107,7 +112,8 @@ def _lbm_opts_from_oracle(p: O.Params):
     extra = dict(init=lbm.INIT_REST if p.init_rest else lbm.INIT_REFERENCE,
                  trt_fp64_parts=p.trt_fp64, trt_lambda=p.trt_lambda,
                  mrt_textbook_meq=p.mrt_textbook, resid_kind=p.resid_kind,
-                 resid_every=max(p.resid_every, 1), max_iterations=p.iterations)
+                 resid_every=max(p.resid_every, 1), max_iterations=p.iterations,
+                 outlet_rho=p.outlet_rho)
     if p.mrt_custom_s:
         extra["mrt_rates"] = list(p.mrt_s)
     return lbm.make_options(**kw, **extra)
@@ -328,3 +334,71 @@ def test_graph_replay_matches_eager_launches(prec, geom):
     fa, ma = sg.distributions()
     fb, mb = se.distributions()
     assert np.array_equal(fa, fb) and np.array_equal(ma, mb)
+
+
+def test_momentum_exchange_force_matches_oracle():
+    """Force history (boundary id 5 = the obstacle) sampled every 3 steps equals
+    the oracle's momentum exchange at the same iterations (rtol 1e-12: the
+    device sums links in another order)."""
+    nodes, conn = meshgen.cavity_obstacle(24, 12.3, 11.6, 4.4).arrays()
+    p = O.params(collision=3, curved=1, uavg=0.1, rho_ini=1.0, viscosity=0.04, init_rest=1)
+    o = O.Oracle(nodes, conn, p)
+    opts = _lbm_opts_from_oracle(p)
+    opts.force_id, opts.force_every = 5, 3
+    case, s = run_gpu(nodes, conn, opts, 0)
+    expect = {}
+    for it in range(1, 31):
+        o.step(1)
+        if it % 3 == 0:
+            expect[it - 1] = o.force(5)
+    s.step(30)
+    its, fxy = s.force_history()
+    assert list(its) == sorted(expect)
+    for i, f in zip(its, fxy):
+        assert np.allclose(f, expect[int(i)], rtol=1e-12, atol=1e-15), (i, f, expect[int(i)])
+    assert np.abs(fxy).max() > 1e-4
+
+
+def _cylinder_run(prec, steps, every):
+    NX, NY, D = 2049, 513, 64
+    case = lbm.Case.channel(NX, NY, 512.0, 260.0, D / 2)
+    umax = 0.05
+    uavg = umax / 1.5                     # parabolic inlet: peak 1.5 * Uavg
+    nu = uavg * D / 100.0                 # Re = 100 on the mean inlet velocity
+    tau = 3 * np.float64(np.float32(nu)) + 0.5
+    opts = lbm.make_options(collision=lbm.MRT, uavg=uavg, rho_ini=1.0, viscosity=nu,
+                            inlet_profile=1, curved=1, outlet_profile=lbm.OUTLET_PRESSURE,
+                            init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE,
+                            mrt_textbook_meq=1, mrt_rates=[0, 1.4, 1.4, 0, 1.2, 0, 1.2,
+                                                           1 / tau, 1 / tau],
+                            force_id=5, force_every=every, precision=prec, outlet_rho=1.0)
+    s = lbm.Solver(case, opts)
+    s.step(steps)
+    its, f = s.force_history()
+    return its, f, uavg, D
+
+
+def test_config2_cylinder_vortex_shedding():
+    """Channel 2049 x 513 with a D = 64 cylinder at (512, 260), parabolic inlet
+    (u_max = 0.05), rho = 1 pressure outlet, interpolated walls, MRT, Re = 100:
+    the lift oscillates with a Strouhal number in the confined-cylinder range,
+    and FP32 storage reproduces the FP64 Strouhal number and mean drag."""
+    steps, every = 100000, 10
+    out = {}
+    for prec in (lbm.PREC_F64, lbm.PREC_F32):
+        print("cylinder run", prec, flush=True)
+        its, f, U, D = _cylinder_run(prec, steps, every)
+        fx, fy = f[:, 0], f[:, 1]
+        half = len(fy) // 2
+        st = lbm.strouhal(fy, every, D, U)
+        cd = 2 * fx[half:].mean() / (1.0 * U * U * D)
+        cl_amp = 2 * (fy[half:].max() - fy[half:].min()) / 2 / (U * U * D)
+        out[prec] = (st, cd, cl_amp)
+        print(f"precision {prec}: St = {st:.4f}  Cd = {cd:.4f}  Cl amplitude = {cl_amp:.4f}",
+              flush=True)
+    st64, cd64, cl64 = out[lbm.PREC_F64]
+    st32, cd32, cl32 = out[lbm.PREC_F32]
+    assert cl64 > 0.05                        # the wake sheds
+    assert 0.15 < st64 < 0.35
+    assert abs(st32 - st64) < 0.01 * st64
+    assert abs(cd32 - cd64) < 0.01 * abs(cd64)

@@ -287,3 +287,54 @@ def test_writers_match_reference_text():
         o.step(10)
         o.write_results(os.path.join(t, "f.csv"), final=True)
         assert open(os.path.join(t, "f.csv")).read() == open(os.path.join(d, "FinalData.csv")).read()
+
+
+# ------------------------------------------------ config-2 physics (not in the reference)
+def test_pressure_outlet_closed_forms():
+    """Zou-He pressure outlet: at every outlet node (BC_ID[1] = 3, not a
+    corner) the post-boundary density equals rho_out and j_y = 0."""
+    NX, NY = 30, 12
+    nodes, conn = meshgen.channel(NX, NY).arrays()
+    o = O.Oracle(nodes, conn, O.params(outlet_profile=4, outlet_rho=1.02, uavg=0.05, rho_ini=1.0,
+                                       viscosity=0.05, collision=3))
+    o.step(40)
+    c = o.cells()
+    n = NX + 2
+    for j in range(2, NY):  # skip corner rows
+        cell = c[j * n + NX]
+        assert cell["bc"][1] == 3 and cell["corner"] == 0
+        F = cell["f"]
+        assert abs(F.sum() - 1.02) < 1e-13
+        jy = F[2] - F[4] + F[5] + F[6] - F[7] - F[8]
+        assert abs(jy) < 1e-15
+        assert abs(cell["rho"] - 1.02) < 1e-13
+
+
+def test_force_closed_forms():
+    """Momentum exchange in a closed box at rest: the bottom wall feels the
+    lattice pressure rho/3 per unit length downward (N nodes plus the corner
+    diagonals), the top wall the opposite, and all walls together nothing."""
+    N = 16
+    nodes, conn = meshgen.cavity(N, lid_type=1).arrays()
+    o = O.Oracle(nodes, conn, O.params(collision=1, rho_ini=1.0, uavg=0.0, init_rest=1,
+                                       viscosity=0.1, curved=1))
+    o.step(3)
+    f = {b: o.force(b) for b in (1, 2, 3, 4)}
+    total = sum(f.values())
+    assert np.abs(total).max() < 1e-12
+    # west / east walls are mirror images (their corners carry the same ids)
+    assert abs(f[1][0] + f[3][0]) < 1e-12 and abs(f[1][1] - f[3][1]) < 1e-12
+    # bottom pushed down, top pushed up, by rho/3 per node plus corner links
+    assert -(N + 3) / 3 < f[4][1] < -(N - 3) / 3
+    assert (N - 3) / 3 < f[2][1] < (N + 3) / 3
+    assert abs(f[4][0]) < 1e-12 and abs(f[2][0]) < 1e-12
+
+
+def test_strouhal_of_a_sinusoid():
+    import parallellbmcranfield_amd as lbm
+    steps, every = 100000, 10
+    t = np.arange(0, steps, every)
+    period = 1234.5
+    lift = 0.3 * np.sin(2 * np.pi * t / period) + 0.01 * np.cos(2 * np.pi * t / 97.0) + 5.0
+    st = lbm.strouhal(lift, every, diameter=64.0, velocity=0.04)
+    assert abs(st - 64.0 / (period * 0.04)) < 2e-3 * st
