@@ -212,6 +212,13 @@ int lbm_solver_write_results(lbm_solver* s, const char* path, int format, int fi
 int lbm_solver_timing(const lbm_solver* s, lbm_timing* out);
 /* Bytes of device memory held by the solver. */
 size_t lbm_solver_device_bytes(const lbm_solver* s);
+/* Binary checkpoint of the solver state (both distribution buffers, the
+ * iteration, residuals, velocity-residual snapshot): the restart the
+ * reference declares but never implements (ReadFormerData,
+ * serial/FileIO.c:287; ReadResults, serial/include/FilesIO.h:25).  A file
+ * loads into a solver of the same case, precision and residual kind. */
+int lbm_solver_save(lbm_solver* s, const char* path);
+int lbm_solver_load(lbm_solver* s, const char* path);
 /* Kernel layout (for profiling): 1 if interior and boundary-list nodes share
  * one launch per step, 0 if they have separate launches. */
 int lbm_solver_fused(const lbm_solver* s);
@@ -229,6 +236,11 @@ int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs);
  * LBM_DIVERGED where the reference printed DIVERGENCE! and exited 1. */
 int lbm_run(const char* ini_file, const char* node_file, const char* bc_file,
             const char* out_dir, const lbm_options* opt_overrides, int verbose);
+/* lbm_run that first restores restart_file (if not NULL) and finally saves
+ * checkpoint_file (if not NULL) with lbm_solver_save. */
+int lbm_run_checkpointed(const char* ini_file, const char* node_file, const char* bc_file,
+                         const char* out_dir, const lbm_options* opt_overrides,
+                         const char* restart_file, const char* checkpoint_file, int verbose);
 
 /* The reference's Iteration() signature (serial/include/Iterate.h:21-27),
  * writing Results/ in the current directory. */

@@ -40,7 +40,8 @@ EXPORTS = [
     "lbm_solver_residual_history", "lbm_solver_force_history", "lbm_solver_get_macro",
     "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
-    "lbm_solver_fused", "lbm_device_copy_gbs", "lbm_run", "lbm_iteration",
+    "lbm_solver_fused", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs", "lbm_run",
+    "lbm_run_checkpointed", "lbm_iteration",
 ]
 
 
@@ -135,6 +136,10 @@ def lib() -> C.CDLL:
         "lbm_solver_device_bytes": (C.c_size_t, [P]),
         "lbm_device_copy_gbs": (I, [C.c_size_t, I, C.POINTER(D)]),
         "lbm_solver_fused": (I, [P]),
+        "lbm_solver_save": (I, [P, C.c_char_p]),
+        "lbm_solver_load": (I, [P, C.c_char_p]),
+        "lbm_run_checkpointed": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                     C.POINTER(Options), C.c_char_p, C.c_char_p, I]),
         "lbm_run": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Options), I]),
         "lbm_iteration": (I, [C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
                               C.c_float, I, I, I, I, I, I, I, I, I, C.c_float, C.c_float, I]),
@@ -337,6 +342,14 @@ class Solver:
     @property
     def fused(self) -> bool:
         return bool(lib().lbm_solver_fused(self.h))
+
+    def save(self, path: str) -> None:
+        """Binary checkpoint (lbm_solver_save)."""
+        _check(lib().lbm_solver_save(self.h, path.encode()))
+
+    def load(self, path: str) -> None:
+        """Restore a checkpoint written by save() for the same case/precision."""
+        _check(lib().lbm_solver_load(self.h, path.encode()))
 
     def device_bytes(self) -> int:
         return lib().lbm_solver_device_bytes(self.h)

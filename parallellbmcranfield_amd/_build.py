@@ -31,7 +31,7 @@ HIP_COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang",
               "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
               "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROCM, "include"), "-I", CSRC,
-              "-Wall", "-Wno-unused-function"]
+              "-Wall", "-Wno-unused-function", "-pthread"]
 
 UNITS = [
     ("kernels_f64.hip", "hip", ["-ffp-contract=off"]),

@@ -18,7 +18,8 @@ static void usage() {
                "                 [--precision f64|f32|f16] [--init reference|rest]\n"
                "                 [--resid reference|full|velocity|none] [--resid-every K]\n"
                "                 [--trt-lambda X] [--trt-fp64] [--mrt-textbook]\n"
-               "                 [--mrt-rates s0,s1,...,s8] [--quiet]\n");
+               "                 [--mrt-rates s0,s1,...,s8] [--restart FILE]\n"
+               "                 [--checkpoint FILE] [--quiet]\n");
 }
 
 int main(int argc, char** argv) {
@@ -26,6 +27,8 @@ int main(int argc, char** argv) {
   const char* nodes = "Mesh/D2node.dat";
   const char* bc = "Mesh/BCconnectors.dat";
   const char* out = "";
+  const char* restart = nullptr;
+  const char* checkpoint = nullptr;
   int verbose = 1;
   bool custom = false;
   lbm_ini ini;
@@ -63,7 +66,9 @@ int main(int argc, char** argv) {
       }
       have_rates = true;
       custom = true;
-    } else if (!std::strcmp(argv[a], "--quiet")) verbose = 0;
+    } else if (!std::strcmp(argv[a], "--restart")) restart = next();
+    else if (!std::strcmp(argv[a], "--checkpoint")) checkpoint = next();
+    else if (!std::strcmp(argv[a], "--quiet")) verbose = 0;
     else { usage(); return 2; }
   }
   lbm_options opt;
@@ -91,7 +96,8 @@ int main(int argc, char** argv) {
       for (int k = 0; k < 9; k++) opt.mrt_rates[k] = rates[k];
     }
   }
-  const int rc = lbm_run(ini_file, nodes, bc, out, custom ? &opt : nullptr, verbose);
+  const int rc = lbm_run_checkpointed(ini_file, nodes, bc, out, custom ? &opt : nullptr, restart,
+                                      checkpoint, verbose);
   if (rc == LBM_DIVERGED) return 1;  // serial/ComputeResiduals.c:50
   return rc == LBM_OK ? 0 : 2;
 }

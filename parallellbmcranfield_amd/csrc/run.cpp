@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lbm_host.hpp"
@@ -27,6 +29,62 @@ std::string join(const std::string& dir, const char* name) {
 
 }  // namespace
 
+namespace {
+
+// Formats rows [r0, r1) of a result file into `out` (row -> one text line).
+template <class RowFn>
+void format_rows(long r0, long r1, RowFn row, std::string& out) {
+  char line[512];
+  for (long r = r0; r < r1; r++) {
+    const int len = row(r, line, sizeof line);
+    out.append(line, (size_t)(len < (int)sizeof line ? len : (int)sizeof line - 1));
+  }
+}
+
+// Writes `nrows` lines produced by `row` after `header`, formatting chunks of
+// rows on several threads and writing them in order (same bytes as a
+// sequential fprintf loop).
+template <class RowFn>
+int write_lines(const char* path, const std::string& header, long nrows, RowFn row) {
+  FILE* fp = std::fopen(path, "w");
+  if (!fp) {
+    set_error(std::string("cannot write ") + path);
+    return LBM_EIO;
+  }
+  std::fwrite(header.data(), 1, header.size(), fp);
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 16) nt = 16;
+  if (nrows < 20000) nt = 1;
+  const long chunk = 1 << 18;  // rows per block
+  std::vector<std::string> bufs(nt);
+  for (long base = 0; base < nrows; base += chunk * (long)nt) {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+      const long r0 = base + (long)t * chunk;
+      const long r1 = std::min(nrows, r0 + chunk);
+      bufs[t].clear();
+      if (r0 >= r1) continue;
+      if (nt == 1)
+        format_rows(r0, r1, row, bufs[t]);
+      else
+        th.emplace_back([&, t, r0, r1] { format_rows(r0, r1, row, bufs[t]); });
+    }
+    for (auto& x : th) x.join();
+    for (unsigned t = 0; t < nt; t++)
+      if (!bufs[t].empty()) std::fwrite(bufs[t].data(), 1, bufs[t].size(), fp);
+  }
+  const bool bad = std::ferror(fp) != 0;
+  std::fclose(fp);
+  if (bad) {
+    set_error(std::string("write failed: ") + path);
+    return LBM_EIO;
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
 extern "C" int lbm_solver_write_results(lbm_solver* s, const char* path, int format,
                                         int final_layout, int row0_bug) {
   if (!s || !path) return LBM_EINVAL;
@@ -41,53 +99,49 @@ extern "C" int lbm_solver_write_results(lbm_solver* s, const char* path, int for
     rc = lbm_solver_get_f(s, f.data(), nullptr);
     if (rc) return rc;
   }
-  FILE* fp = std::fopen(path, "w");
-  if (!fp) {
-    set_error(std::string("cannot write ") + path);
-    return LBM_EIO;
-  }
-  std::vector<char> iobuf(1 << 20);
-  std::setvbuf(fp, iobuf.data(), _IOFBF, iobuf.size());
   const bool csv = format != LBM_FORMAT_TECPLOT;
+  char head[512];
+  std::string header;
   // serial/FileIO.c:298-348 (WriteResults) and :354-427 (WriteFinalResults)
   if (!final_layout) {
     if (csv) {
-      std::fprintf(fp, "x,y,u,v,vel_mag,rho,press,fluid\n");
+      header = "x,y,u,v,vel_mag,rho,press,fluid\n";
     } else {
-      std::fprintf(fp, "Title = \"LBM results\"\n");
-      std::fprintf(fp, "Variables = \"x\",\"y\",\"u\",\"v\",\"u mag\",\"rho\",\"press\",\"fluid\"\n");
-      std::fprintf(fp, "Zone i=%d, j=%d, f=point\n", n, m);
+      header = "Title = \"LBM results\"\n";
+      header += "Variables = \"x\",\"y\",\"u\",\"v\",\"u mag\",\"rho\",\"press\",\"fluid\"\n";
+      std::snprintf(head, sizeof head, "Zone i=%d, j=%d, f=point\n", n, m);
+      header += head;
     }
     const char* fmt = csv ? "%f, %f, %f, %f, %f, %f, %f, %d\n" : "%f %f %f %f %f %f %f %d\n";
-    for (long i = 0; i < nc; i++)
-      std::fprintf(fp, fmt, c->x(i), c->y(i), u[i], v[i], std::sqrt(u[i] * u[i] + v[i] * v[i]),
-                   rho[i], rho[i] / 3.0, c->fluid[i]);
-  } else {
-    if (csv) {
-      std::fprintf(fp, "x,y,u,v,vel_mag,rho,press,fluid,f0,f1,f2,f3,f4,f5,f6,f7,f8\n");
-    } else {
-      std::fprintf(fp, "Title = \"LBM results\"\n");
-      std::fprintf(fp, "Variables = \"x\",\"y\",\"u\",\"v\",\"u mag\",\"rho\",\"press\",\"fluid\",\"f0\",\"f1\",\"f2\",\"f3\",\"f4\",\"f5\",\"f6\",\"f7\",\"f8\"\n");
-      std::fprintf(fp, "Zone i=%d, j=%d, f=point\n", n, m);
-    }
-    const char* fmt = csv ? "%f, %f, %f, %f, %f, %f, %f, %d, %f, %f, %f, %f, %f, %f, %f, %f, %f\n"
-                          : "%f %f %f %f %f %f %f %d %f %f %f %f %f %f %f %f %f\n";
-    for (int j = 0; j < m; j++)
-      for (int i = 0; i < n; i++) {
-        const long q = row0_bug ? i : (long)j * n + i;  // FileIO.c:368 indexes Cells+i
-        const double* F = &f[(size_t)q * 9];
-        std::fprintf(fp, fmt, c->x(q), c->y(q), u[q], v[q], std::sqrt(u[q] * u[q] + v[q] * v[q]),
-                     rho[q], rho[q] / 3, c->fluid[q], F[0], F[1], F[2], F[3], F[4], F[5], F[6],
-                     F[7], F[8]);
-      }
+    return write_lines(path, header, nc, [&](long i, char* buf, size_t cap) {
+      return std::snprintf(buf, cap, fmt, c->x(i), c->y(i), u[i], v[i],
+                           std::sqrt(u[i] * u[i] + v[i] * v[i]), rho[i], rho[i] / 3.0,
+                           c->fluid[i]);
+    });
   }
-  std::fclose(fp);
-  return LBM_OK;
+  if (csv) {
+    header = "x,y,u,v,vel_mag,rho,press,fluid,f0,f1,f2,f3,f4,f5,f6,f7,f8\n";
+  } else {
+    header = "Title = \"LBM results\"\n";
+    header += "Variables = \"x\",\"y\",\"u\",\"v\",\"u mag\",\"rho\",\"press\",\"fluid\",\"f0\",\"f1\",\"f2\",\"f3\",\"f4\",\"f5\",\"f6\",\"f7\",\"f8\"\n";
+    std::snprintf(head, sizeof head, "Zone i=%d, j=%d, f=point\n", n, m);
+    header += head;
+  }
+  const char* fmt = csv ? "%f, %f, %f, %f, %f, %f, %f, %d, %f, %f, %f, %f, %f, %f, %f, %f, %f\n"
+                        : "%f %f %f %f %f %f %f %d %f %f %f %f %f %f %f %f %f\n";
+  return write_lines(path, header, nc, [&](long r, char* buf, size_t cap) {
+    const long q = row0_bug ? (r % n) : r;  // FileIO.c:368 indexes Cells+i for every row j
+    const double* F = &f[(size_t)q * 9];
+    return std::snprintf(buf, cap, fmt, c->x(q), c->y(q), u[q], v[q],
+                         std::sqrt(u[q] * u[q] + v[q] * v[q]), rho[q], rho[q] / 3,
+                         c->fluid[q], F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8]);
+  });
 }
 
 // serial/Iterate.c:17-410 with serial/main.c:20-93 around it.
 static int run_impl(const lbm_ini& ini, const char* node_file, const char* bc_file,
-                    const char* out_dir, const lbm_options* overrides, int verbose) {
+                    const char* out_dir, const lbm_options* overrides, int verbose,
+                    const char* restart_file = nullptr, const char* checkpoint_file = nullptr) {
   const double t_start = now_s();
   const std::string res_dir = join(out_dir ? out_dir : "", "Results");
   if (verbose) {
@@ -172,6 +226,16 @@ static int run_impl(const lbm_ini& ini, const char* node_file, const char* bc_fi
     std::fclose(log);
     lbm_case_free(cs);
     return rc;
+  }
+  if (restart_file) {
+    rc = lbm_solver_load(s, restart_file);
+    if (rc) {
+      std::fprintf(stderr, "%s\n", lbm_last_error());
+      std::fclose(log);
+      lbm_solver_destroy(s);
+      lbm_case_free(cs);
+      return rc;
+    }
   }
   const double t_init = now_s() - t_init0;
   std::fprintf(log, "\n:::: Initialization done! ::::\n");
@@ -299,6 +363,10 @@ static int run_impl(const lbm_ini& ini, const char* node_file, const char* bc_fi
       std::fprintf(tf, "tWriting %f\n", t_writing);
       std::fclose(tf);
     }
+    if (checkpoint_file) {
+      rc = lbm_solver_save(s, checkpoint_file);
+      if (rc) goto fail;
+    }
     std::string final_out = res_dir + (csv ? "/FinalData.csv" : "/FinalData.dat");
     if (ini.postproc_prog == 1 || ini.postproc_prog == 2) {
       // FileIO.c:354-427, row-0 repetition included
@@ -334,6 +402,20 @@ extern "C" int lbm_run(const char* ini_file, const char* node_file, const char* 
     return rc;
   }
   return run_impl(ini, node_file, bc_file, out_dir, overrides, verbose);
+}
+
+extern "C" int lbm_run_checkpointed(const char* ini_file, const char* node_file,
+                                    const char* bc_file, const char* out_dir,
+                                    const lbm_options* overrides, const char* restart_file,
+                                    const char* checkpoint_file, int verbose) {
+  lbm_ini ini;
+  int rc = lbm_read_ini(ini_file ? ini_file : "SetUpData.ini", &ini);
+  if (rc) {
+    std::fprintf(stderr, "%s\n", lbm_last_error());
+    return rc;
+  }
+  return run_impl(ini, node_file, bc_file, out_dir, overrides, verbose, restart_file,
+                  checkpoint_file);
 }
 
 extern "C" int lbm_iteration(const char* NodeDataFile, const char* BCconnectorDataFile,

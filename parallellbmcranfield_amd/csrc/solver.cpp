@@ -810,6 +810,111 @@ size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 
 
 int lbm_solver_fused(const lbm_solver* s) { return s && s->fused ? 1 : 0; }
 
+// ---------------------------------------------------------------- checkpoint
+namespace {
+struct CkptHeader {
+  char magic[8];  // "LBMCKPT1"
+  int abi, n, m, prec, coll, pitch, plane, iter, cur, resid_kind, has_snap;
+  double resid[4];
+};
+
+int copy_out(lbm_solver* s, FILE* f, const void* dev, size_t bytes) {
+  const size_t CH = (size_t)256 << 20;
+  std::vector<char> tmp(std::min(bytes, CH));
+  for (size_t off = 0; off < bytes; off += CH) {
+    const size_t len = std::min(CH, bytes - off);
+    HC(hipMemcpy(tmp.data(), (const char*)dev + off, len, hipMemcpyDeviceToHost));
+    if (std::fwrite(tmp.data(), 1, len, f) != len) {
+      set_error("checkpoint write failed");
+      return LBM_EIO;
+    }
+  }
+  return LBM_OK;
+}
+
+int copy_in(lbm_solver* s, FILE* f, void* dev, size_t bytes) {
+  (void)s;
+  const size_t CH = (size_t)256 << 20;
+  std::vector<char> tmp(std::min(bytes, CH));
+  for (size_t off = 0; off < bytes; off += CH) {
+    const size_t len = std::min(CH, bytes - off);
+    if (std::fread(tmp.data(), 1, len, f) != len) {
+      set_error("checkpoint file truncated");
+      return LBM_EPARSE;
+    }
+    HC(hipMemcpy((char*)dev + off, tmp.data(), len, hipMemcpyHostToDevice));
+  }
+  return LBM_OK;
+}
+}  // namespace
+
+int lbm_solver_save(lbm_solver* s, const char* path) {
+  if (!s || !path) return LBM_EINVAL;
+  int rc = flush_history(s);
+  if (rc && rc != LBM_DIVERGED) return rc;
+  HC(hipStreamSynchronize(s->stream));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    set_error(std::string("cannot write ") + path);
+    return LBM_EIO;
+  }
+  CkptHeader h;
+  std::memset(&h, 0, sizeof h);
+  std::memcpy(h.magic, "LBMCKPT1", 8);
+  h.abi = LBM_ABI_VERSION;
+  h.n = s->cs->n;
+  h.m = s->cs->m;
+  h.prec = s->prec;
+  h.coll = s->coll;
+  h.pitch = s->pitch;
+  h.plane = s->plane;
+  h.iter = s->iter;
+  h.cur = s->cur;
+  h.resid_kind = s->opt.resid_kind;
+  h.has_snap = s->d_snap ? 1 : 0;
+  std::memcpy(h.resid, s->resid, sizeof h.resid);
+  rc = std::fwrite(&h, sizeof h, 1, f) == 1 ? LBM_OK : LBM_EIO;
+  const size_t sb = (size_t)9 * s->plane * s->elem;
+  if (!rc) rc = copy_out(s, f, s->buf[0], sb);
+  if (!rc) rc = copy_out(s, f, s->buf[1], sb);
+  if (!rc && s->d_snap) rc = copy_out(s, f, s->d_snap, sizeof(double) * 3 * (size_t)s->plane);
+  std::fclose(f);
+  return rc;
+}
+
+int lbm_solver_load(lbm_solver* s, const char* path) {
+  if (!s || !path) return LBM_EINVAL;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) {
+    set_error(std::string("cannot open ") + path);
+    return LBM_EIO;
+  }
+  CkptHeader h;
+  int rc = std::fread(&h, sizeof h, 1, f) == 1 ? LBM_OK : LBM_EPARSE;
+  if (!rc && (std::memcmp(h.magic, "LBMCKPT1", 8) != 0 || h.n != s->cs->n || h.m != s->cs->m ||
+              h.prec != s->prec || h.pitch != s->pitch || h.plane != s->plane ||
+              h.has_snap != (s->d_snap ? 1 : 0))) {
+    set_error("checkpoint does not match this case / precision / residual kind");
+    rc = LBM_EINVAL;
+  }
+  HC(hipStreamSynchronize(s->stream));
+  const size_t sb = (size_t)9 * s->plane * s->elem;
+  if (!rc) rc = copy_in(s, f, s->buf[0], sb);
+  if (!rc) rc = copy_in(s, f, s->buf[1], sb);
+  if (!rc && s->d_snap) rc = copy_in(s, f, s->d_snap, sizeof(double) * 3 * (size_t)s->plane);
+  std::fclose(f);
+  if (rc) return rc;
+  s->iter = h.iter;
+  s->cur = h.cur;
+  std::memcpy(s->resid, h.resid, sizeof h.resid);
+  s->observed = s->observed_f = -1;
+  s->pending = 0;
+  s->pending_iter.clear();
+  s->pending_what.clear();
+  s->diverged = false;
+  return LBM_OK;
+}
+
 int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs) {
   if (!gbs || bytes < 16 || reps < 1) return LBM_EINVAL;
   bytes = bytes / 16 * 16;

@@ -402,3 +402,44 @@ def test_config2_cylinder_vortex_shedding():
     assert 0.15 < st64 < 0.35
     assert abs(st32 - st64) < 0.01 * st64
     assert abs(cd32 - cd64) < 0.01 * abs(cd64)
+
+
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32])
+def test_checkpoint_restart_is_exact(prec, tmp_path):
+    """save at t=40, keep stepping to 90; a new solver loaded from the file and
+    stepped 50 more reaches the same state bit for bit (velocity residual
+    snapshot included)."""
+    nodes, conn = meshgen.cavity(40).arrays()
+    opts = lbm.make_options(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.03,
+                            init=lbm.INIT_REST, precision=prec, resid_kind=lbm.RESID_VELOCITY,
+                            resid_every=10, use_stop_rule=0)
+    case = lbm.Case.from_arrays(nodes, conn)
+    a = lbm.Solver(case, opts)
+    a.step(40)
+    a.save(str(tmp_path / "ck.bin"))
+    a.step(50)
+    b = lbm.Solver(case, opts)
+    b.load(str(tmp_path / "ck.bin"))
+    assert b.iter == 40
+    b.step(50)
+    for x, y in zip(a.macro(), b.macro()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.residuals(), b.residuals())
+    c = lbm.Solver(lbm.Case.cavity(41), opts)
+    with pytest.raises(lbm.LbmError):
+        c.load(str(tmp_path / "ck.bin"))
+
+
+def test_parallel_writer_matches_oracle_bytes(tmp_path):
+    """Result files of a 160^2 lattice (threaded formatter) are byte-identical
+    to the oracle's sequential fprintf writer on the same FP64 state."""
+    N = 160
+    nodes, conn = meshgen.cavity_arrays_fast(N)
+    p = O.params(collision=1, uavg=0.1, rho_ini=1.0, viscosity=0.05)
+    o = O.Oracle(nodes, conn, p)
+    o.step(20)
+    case, s = run_gpu(nodes, conn, _lbm_opts_from_oracle(p), 20)
+    for final, bug in ((False, True), (True, True), (True, False)):
+        o.write_results(str(tmp_path / "o.csv"), final, bug)
+        s.write_results(str(tmp_path / "g.csv"), lbm.FORMAT_CSV, final, bug)
+        assert (tmp_path / "o.csv").read_bytes() == (tmp_path / "g.csv").read_bytes(), (final, bug)
