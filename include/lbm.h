@@ -159,6 +159,21 @@ int lbm_case_cavity(int N, int lid_type, lbm_case** out);
  * (radius r > 0 at (cx, cy) in node units) whose links carry the exact cut
  * distance q and boundary id 5. */
 int lbm_case_channel(int NX, int NY, double cx, double cy, double r, lbm_case** out);
+/* Mesh generator for batched sweeps: an NX x NY lid-driven cavity (n = NX+2,
+ * m = NY+3, Delta = 1/max(NX, NY)) with nrect solid rectangles given in
+ * fluid-node units as rects[4r..4r+3] = (i0, j0, i1, j1), inclusive; the
+ * rectangles' links are half-way bounce-back walls of boundary id 5.  No
+ * counterpart in the reference (generalises cases/coarse/Mesh to any
+ * aspect ratio and obstacles). */
+int lbm_case_cavity_ex(int NX, int NY, int lid_type, const int* rects, int nrect,
+                       lbm_case** out);
+/* Packs `count` cases side by side (shelf packing, tallest first) into one
+ * lattice for a batched solver.  Instances keep their own coordinates and
+ * boundary links; rects (may be NULL) receives (x0, y0, n, m) per instance
+ * in the packed lattice.  No counterpart in the reference. */
+int lbm_case_pack(const lbm_case* const* cases, int count, lbm_case** out, int* rects);
+/* Number of instances of a packed case (0 for an ordinary one). */
+int lbm_case_instances(const lbm_case* c);
 /* Writes the case as Mesh/D2node.dat + Mesh/BCconnectors.dat text. */
 int lbm_case_write(const lbm_case* c, const char* node_file, const char* bc_file);
 int lbm_case_info_get(const lbm_case* c, lbm_case_info* out);
@@ -175,6 +190,25 @@ int lbm_solver_create(const lbm_case* c, const lbm_options* o, lbm_solver** out)
 /* As lbm_solver_create with LBM_INIT_FIELD: rho/u/v are n*m host arrays. */
 int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const double* rho,
                             const double* u, const double* v, lbm_solver** out);
+/* Per-instance parameters of a batched solver. */
+typedef struct lbm_instance {
+  float viscosity; /* lattice viscosity: omega = 1/(3 nu + 1/2) */
+  float uavg;      /* lid speed of the instance (inlet profile 2) */
+} lbm_instance;
+/* Batched solver over a packed case (lbm_case_pack): every instance advances
+ * in the same step launches with its own omega and lid speed, FP32 or FP16
+ * storage.  Every resid_every steps each instance's velocity residual
+ * ||u(t) - u(t-K)|| / (K ||u(t)||) is computed; an instance below crit_v stops
+ * advancing.  With use_stop_rule the batch stops when all have.  No
+ * counterpart in the reference (its solver runs one lattice per process). */
+int lbm_solver_create_batched(const lbm_case* c, const lbm_options* o, const lbm_instance* inst,
+                              int count, lbm_solver** out);
+/* Per-instance state of a batched solver: conv_iter (iteration at which the
+ * instance converged, -1 still running, -2 diverged), conv_seconds (wall time
+ * from the first step to the detection), resid (last residual).  Any pointer
+ * may be NULL; the first min(count, instances) entries are written. */
+int lbm_batch_status(lbm_solver* s, int count, int* conv_iter, double* conv_seconds,
+                     double* resid);
 void lbm_solver_destroy(lbm_solver* s);
 
 /* Replaces one or more passes of the main loop body (serial/Iterate.c:263-332:

@@ -34,6 +34,8 @@ TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY, TUNE_NO_GRAPH = 1, 2, 4
 EXPORTS = [
     "lbm_last_error", "lbm_abi_version", "lbm_read_ini", "lbm_options_from_ini",
     "lbm_case_load", "lbm_case_from_arrays", "lbm_case_cavity", "lbm_case_channel",
+    "lbm_case_cavity_ex", "lbm_case_pack", "lbm_case_instances",
+    "lbm_solver_create_batched", "lbm_batch_status",
     "lbm_case_write", "lbm_case_info_get", "lbm_case_cells", "lbm_case_free",
     "lbm_solver_create", "lbm_solver_create_field", "lbm_solver_destroy", "lbm_solver_step",
     "lbm_solver_step_timed", "lbm_solver_sync", "lbm_solver_iter", "lbm_solver_residuals",
@@ -88,6 +90,10 @@ class CaseInfo(C.Structure):
     ]
 
 
+class Instance(C.Structure):
+    _fields_ = [("viscosity", C.c_float), ("uavg", C.c_float)]
+
+
 class Timing(C.Structure):
     _fields_ = [("bulk_ms", C.c_double), ("boundary_ms", C.c_double), ("outlet_ms", C.c_double),
                 ("residual_ms", C.c_double), ("observe_ms", C.c_double), ("init_ms", C.c_double),
@@ -115,6 +121,11 @@ def lib() -> C.CDLL:
         "lbm_case_from_arrays": (I, [P, I, P, I, C.POINTER(P)]),
         "lbm_case_cavity": (I, [I, I, C.POINTER(P)]),
         "lbm_case_channel": (I, [I, I, D, D, D, C.POINTER(P)]),
+        "lbm_case_cavity_ex": (I, [I, I, I, P, I, C.POINTER(P)]),
+        "lbm_case_pack": (I, [P, I, C.POINTER(P), P]),
+        "lbm_case_instances": (I, [P]),
+        "lbm_solver_create_batched": (I, [P, C.POINTER(Options), P, I, C.POINTER(P)]),
+        "lbm_batch_status": (I, [P, I, P, P, P]),
         "lbm_case_write": (I, [P, C.c_char_p, C.c_char_p]),
         "lbm_case_info_get": (I, [P, C.POINTER(CaseInfo)]),
         "lbm_case_cells": (I, [P] * 11),
@@ -229,6 +240,33 @@ class Case:
         _check(lib().lbm_case_channel(NX, NY, cx, cy, r, C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def cavity_ex(cls, NX: int, NY: int, rects=(), lid_type: int = 2) -> "Case":
+        """NX x NY lid-driven cavity with solid rectangles (i0, j0, i1, j1),
+        inclusive fluid-node indices."""
+        r = np.ascontiguousarray(np.asarray(rects, dtype=np.int32).reshape(-1, 4))
+        h = C.c_void_p()
+        _check(lib().lbm_case_cavity_ex(NX, NY, lid_type, _ptr(r) if r.size else None,
+                                        r.shape[0], C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def pack(cls, cases) -> "Case":
+        """Pack several cases into one lattice for a batched solver; the
+        instance rectangles (x0, y0, n, m) are in ``.rects``."""
+        arr = (C.c_void_p * len(cases))(*[c.h.value if isinstance(c.h, C.c_void_p) else c.h
+                                          for c in cases])
+        rects = np.zeros((len(cases), 4), np.int32)
+        h = C.c_void_p()
+        _check(lib().lbm_case_pack(C.cast(arr, C.c_void_p), len(cases), C.byref(h), _ptr(rects)))
+        self = cls(h)
+        self.rects = rects
+        return self
+
+    @property
+    def instances(self) -> int:
+        return lib().lbm_case_instances(self.h)
+
     def write(self, node_file: str, bc_file: str) -> None:
         _check(lib().lbm_case_write(self.h, node_file.encode(), bc_file.encode()))
 
@@ -256,11 +294,17 @@ class Case:
 class Solver:
     """Device state of one lattice and its time loop (serial/Iterate.c:168-332)."""
 
-    def __init__(self, case: Case, opts: Options, field=None):
+    def __init__(self, case: Case, opts: Options, field=None, instances=None):
+        """instances: for a packed case, a sequence of (viscosity, lid speed)."""
         self.case = case  # keep alive: the solver references the case
         self.opts = opts
         h = C.c_void_p()
-        if field is None:
+        if instances is not None:
+            arr = (Instance * len(instances))(*[Instance(float(nu), float(u))
+                                                for nu, u in instances])
+            _check(lib().lbm_solver_create_batched(case.h, C.byref(opts), C.cast(arr, C.c_void_p),
+                                                   len(instances), C.byref(h)))
+        elif field is None:
             _check(lib().lbm_solver_create(case.h, C.byref(opts), C.byref(h)))
         else:
             rho, u, v = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in field)
@@ -281,6 +325,16 @@ class Solver:
         _check(lib().lbm_solver_step_timed(self.h, nsteps, sample_every, C.byref(tot),
                                            C.byref(kern)))
         return tot.value, kern.value
+
+    def batch_status(self):
+        """(conv_iter, conv_seconds, residual) arrays, one entry per instance;
+        conv_iter -1 = still running, -2 = diverged."""
+        k = lib().lbm_case_instances(self.case.h)
+        it = np.zeros(k, np.int32)
+        sec = np.zeros(k)
+        res = np.zeros(k)
+        _check(lib().lbm_batch_status(self.h, k, _ptr(it), _ptr(sec), _ptr(res)))
+        return it, sec, res
 
     def sync(self) -> None:
         _check(lib().lbm_solver_sync(self.h))

@@ -1,5 +1,6 @@
 // case_io.cpp — case input (ini, mesh files), mesh generators and the
 // per-cell set-up of the reference's CellIni, restated in O(N).
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -386,6 +387,123 @@ int lbm_case_cavity(int N, int lid_type, lbm_case** out) {
   return LBM_OK;
 }
 
+int lbm_case_cavity_ex(int NX, int NY, int lid_type, const int* rects, int nrect,
+                       lbm_case** out) {
+  if (!out || NX < 2 || NY < 2 || NX > 30000 || NY > 30000 || nrect < 0 || (nrect && !rects))
+    return LBM_EINVAL;
+  lbm_case* c = new lbm_case();
+  const int n = NX + 2, m = NY + 3;
+  auto in_rect = [=](int i, int j) {
+    for (int r = 0; r < nrect; r++)
+      if (i >= rects[4 * r] && i <= rects[4 * r + 2] && j >= rects[4 * r + 1] && j <= rects[4 * r + 3])
+        return true;
+    return false;
+  };
+  auto fluid = [=](int i, int j) {
+    return i >= 1 && i <= NX && j >= 1 && j <= NY && !in_rect(i, j);
+  };
+  auto classify = [=](int, int, int, int ii, int jj) -> GenLink {
+    if (ii >= 1 && ii <= NX && jj >= 1 && jj <= NY) return {1, 5, 0.5};  // obstacle
+    if (jj == 0) return {1, 4, 0.5};
+    if (ii == NX + 1 && jj == NY + 1) return {1, 3, 0.5};
+    if (jj == NY + 1) return {lid_type, 2, 0.5};
+    if (ii == 0) return {1, 1, 0.5};
+    return {1, 3, 0.5};
+  };
+  gen_mesh(c, n, m, 1.0 / (NX > NY ? NX : NY), fluid, classify);
+  int rc = finish_case(c);
+  if (rc != LBM_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return LBM_OK;
+}
+
+int lbm_case_pack(const lbm_case* const* cases, int count, lbm_case** out, int* rects) {
+  if (!cases || count <= 0 || count > 65000 || !out) return LBM_EINVAL;
+  // shelf packing, tallest first
+  std::vector<int> order(count);
+  long area = 0;
+  int maxw = 0;
+  for (int k = 0; k < count; k++) {
+    if (!cases[k]) return LBM_EINVAL;
+    order[k] = k;
+    area += (long)cases[k]->n * cases[k]->m;
+    maxw = std::max(maxw, cases[k]->n);
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return cases[a]->m > cases[b]->m; });
+  const int W = std::max(maxw, (int)std::ceil(std::sqrt((double)area * 1.15)));
+  std::vector<int> x0(count), y0(count);
+  int x = 0, y = 0, shelf = 0;
+  for (int k : order) {
+    const lbm_case* c = cases[k];
+    if (x + c->n > W) {
+      y += shelf;
+      x = 0;
+      shelf = 0;
+    }
+    x0[k] = x;
+    y0[k] = y;
+    x += c->n;
+    shelf = std::max(shelf, c->m);
+  }
+  const int n = W, m = y + shelf;
+  if ((long)n * m > (1L << 30)) return LBM_EINVAL;
+  lbm_case* p = new lbm_case();
+  const long nc = (long)n * m;
+  p->n = n;
+  p->m = m;
+  p->num_nodes = (int)nc;
+  p->nodes.assign((size_t)nc * 5, 0.0f);
+  p->id.resize((size_t)nc);
+  p->fluid.assign((size_t)nc, 0);
+  p->bcell.assign((size_t)nc, -1);
+  p->cell_inst.assign((size_t)nc, -1);
+  p->ninst = count;
+  p->inst_rect.resize((size_t)count * 4);
+  for (long q = 0; q < nc; q++) {
+    p->id[q] = (int32_t)q;
+    p->nodes[(size_t)q * 5] = (float)(q % n);
+    p->nodes[(size_t)q * 5 + 1] = (float)(q / n);
+  }
+  for (int k = 0; k < count; k++) {
+    const lbm_case* c = cases[k];
+    p->inst_rect[4 * k] = x0[k];
+    p->inst_rect[4 * k + 1] = y0[k];
+    p->inst_rect[4 * k + 2] = c->n;
+    p->inst_rect[4 * k + 3] = c->m;
+    if (rects) {
+      rects[4 * k] = x0[k];
+      rects[4 * k + 1] = y0[k];
+      rects[4 * k + 2] = c->n;
+      rects[4 * k + 3] = c->m;
+    }
+    for (int j = 0; j < c->m; j++)
+      for (int i = 0; i < c->n; i++) {
+        const long src = (long)j * c->n + i;
+        const long dst = (long)(j + y0[k]) * n + (i + x0[k]);
+        float* r = &p->nodes[(size_t)dst * 5];
+        r[2] = (float)c->x(src);
+        r[3] = (float)c->y(src);
+        r[4] = (float)c->fluid[src];
+        p->fluid[dst] = c->fluid[src];
+        p->cell_inst[dst] = (int16_t)k;
+        if (c->bcell[src] >= 0) {
+          p->bcell[dst] = (int32_t)p->bcells.size();
+          p->bcells.push_back(c->bcells[c->bcell[src]]);
+        }
+      }
+    p->num_fluid += c->num_fluid;
+    p->num_boundary += c->num_boundary;
+    p->num_conn += c->num_conn;
+  }
+  p->delta = cases[0]->delta;
+  *out = p;
+  return LBM_OK;
+}
+
 int lbm_case_channel(int NX, int NY, double ocx, double ocy, double r, lbm_case** out) {
   if (!out || NX < 3 || NY < 2) return LBM_EINVAL;
   lbm_case* c = new lbm_case();
@@ -426,6 +544,10 @@ int lbm_case_channel(int NX, int NY, double ocx, double ocy, double r, lbm_case*
 
 int lbm_case_write(const lbm_case* c, const char* node_file, const char* bc_file) {
   if (!c || !node_file || !bc_file) return LBM_EINVAL;
+  if (c->ninst) {
+    set_error("a packed case has no single reference-format mesh");
+    return LBM_EINVAL;
+  }
   FILE* f = std::fopen(node_file, "w");
   if (!f) {
     set_error(std::string("cannot write ") + node_file);
@@ -450,6 +572,8 @@ int lbm_case_write(const lbm_case* c, const char* node_file, const char* bc_file
   std::fclose(f);
   return LBM_OK;
 }
+
+int lbm_case_instances(const lbm_case* c) { return c ? c->ninst : 0; }
 
 int lbm_case_info_get(const lbm_case* c, lbm_case_info* o) {
   if (!c || !o) return LBM_EINVAL;

@@ -87,6 +87,49 @@ __global__ void k_ref_residual(const double* __restrict__ cres, int m, int n,
   out[3] = 0.0;
 }
 
+// Velocity-change sums of each batched instance over its storage rectangle:
+// out[2k] = sum |u(t) - u(t-K)|^2, out[2k+1] = sum |u(t)|^2, from the per-cell
+// terms the residual step leaves in planes 3 and 4 of the snapshot.  One
+// workgroup per instance, fixed order: the sums are deterministic.
+__global__ __launch_bounds__(256) void k_inst_resid(const double* __restrict__ snap, int plane,
+                                                    int pitch, const int* __restrict__ rect,
+                                                    const uint8_t* __restrict__ active,
+                                                    double* __restrict__ out) {
+  const int k = blockIdx.x;
+  const int x0 = rect[4 * k], y0 = rect[4 * k + 1], w = rect[4 * k + 2], h = rect[4 * k + 3];
+  double a = 0.0, b = 0.0;
+  if (active[k]) {
+    const long cnt = (long)w * h;
+    for (long e = threadIdx.x; e < cnt; e += 256) {
+      const long r = e / w, c = e - r * w;
+      const long idx = (y0 + r) * (long)pitch + x0 + c;
+      a += snap[3 * (size_t)plane + idx];
+      b += snap[4 * (size_t)plane + idx];
+    }
+  }
+  __shared__ double red[4][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane == 0) {
+    red[wv][0] = a;
+    red[wv][1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int w2 = 0; w2 < 4; w2++) s += red[w2][threadIdx.x];
+    out[2 * k + threadIdx.x] = s;
+  }
+}
+
+hipError_t launch_inst_resid(const double* snap, int plane, int pitch, const int* rect,
+                             const uint8_t* active, int ninst, double* out, hipStream_t st) {
+  if (ninst <= 0) return hipSuccess;
+  k_inst_resid<<<ninst, 256, 0, st>>>(snap, plane, pitch, rect, active, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st) {
   k_reduce_partials<<<1, 256, 0, st>>>(part, nparts, out);
   return hipGetLastError();

@@ -181,11 +181,37 @@ __device__ inline void collide_f64(const double F[9], double R, double U, double
   if constexpr (COLL == 3) collide_mrt_f64(F, R, U, V, p, out);
 }
 
+// Relaxation parameters of the deviation-form kernels: the solver's global
+// values, or one instance's row of the batched parameter table.
+struct CollPar {
+  float om, oma;  // omega (= 1/tau), TRT omega-minus
+  float e[9];     // MRT: s_l / |M_l|^2
+  int flags;      // KF_MRT_TEXTBOOK
+};
+
+__device__ inline CollPar coll_par(const KParams& p, int inst) {
+  CollPar c;
+  c.flags = p.flags;
+  if (inst < 0) {
+    c.om = p.omega_f;
+    c.oma = p.omega_a_f;
+#pragma unroll
+    for (int l = 0; l < 9; l++) c.e[l] = p.mrt_e[l];
+  } else {
+    const float* r = p.inst_par + (size_t)inst * INST_PAR_W;
+    c.om = r[0];
+    c.oma = r[1];
+#pragma unroll
+    for (int l = 0; l < 9; l++) c.e[l] = r[2 + l];
+  }
+  return c;
+}
+
 // Deviation-form FP32 kernels: g_k = f_k - w_k rho0; drho = sum g; rho = rho0 + drho.
 // feq_k - w_k rho0 = w_k (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
 __device__ inline void collide_bgk_dev(const float g[9], float drho, float rho, float u, float v,
-                                       const KParams& p, float out[9]) {
-  const float om = p.omega_f;
+                                       const CollPar& p, float out[9]) {
+  const float om = p.om;
   const float usq15 = 1.5f * (u * u + v * v);
 #pragma unroll
   for (int k = 0; k < 9; k++) {
@@ -196,8 +222,8 @@ __device__ inline void collide_bgk_dev(const float g[9], float drho, float rho, 
 }
 
 __device__ inline void collide_trt_dev(const float g[9], float drho, float rho, float u, float v,
-                                       const KParams& p, float out[9]) {
-  const float op = p.omega_f, om = p.omega_a_f;
+                                       const CollPar& p, float out[9]) {
+  const float op = p.om, om = p.oma;
   const float usq15 = 1.5f * (u * u + v * v);
   // rest population: symmetric only
   {
@@ -222,7 +248,7 @@ __device__ inline void collide_trt_dev(const float g[9], float drho, float rho, 
 }
 
 __device__ inline void collide_mrt_dev(const float g[9], float drho, float rho, float u, float v,
-                                       const KParams& p, float out[9]) {
+                                       const CollPar& p, float out[9]) {
   // moments of g (rows of M)
   const float s14 = g[1] + g[2] + g[3] + g[4];
   const float s58 = g[5] + g[6] + g[7] + g[8];
@@ -250,7 +276,7 @@ __device__ inline void collide_mrt_dev(const float g[9], float drho, float rho, 
   // e_l = s_l d_l / |M_l|^2 ; out = g - M^T e
   float e[9];
 #pragma unroll
-  for (int l = 0; l < 9; l++) e[l] = p.mrt_e[l] * d[l];
+  for (int l = 0; l < 9; l++) e[l] = p.e[l] * d[l];
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     float acc = 0.f;
@@ -266,7 +292,7 @@ __device__ inline void collide_mrt_dev(const float g[9], float drho, float rho, 
 
 template <int COLL>
 __device__ inline void collide_dev(const float g[9], float drho, float rho, float u, float v,
-                                   const KParams& p, float out[9]) {
+                                   const CollPar& p, float out[9]) {
   if constexpr (COLL == 1) collide_bgk_dev(g, drho, rho, u, v, p, out);
   if constexpr (COLL == 2) collide_trt_dev(g, drho, rho, u, v, p, out);
   if constexpr (COLL == 3) collide_mrt_dev(g, drho, rho, u, v, p, out);

@@ -39,6 +39,11 @@ struct lbm_case {
   std::vector<int32_t> bcell;   // index into bcells or -1
   std::vector<lbmhost::BCell> bcells;
   int num_fluid = 0, num_boundary = 0;
+  // packed cases (lbm_case_pack): instance of each cell (-1: filler) and each
+  // instance's rectangle (x0, y0, n, m) in the packed grid
+  std::vector<int16_t> cell_inst;
+  std::vector<int> inst_rect;
+  int ninst = 0;
 
   double x(long lin) const { return (double)nodes[(size_t)id[lin] * 5 + 2]; }
   double y(long lin) const { return (double)nodes[(size_t)id[lin] * 5 + 3]; }
@@ -104,6 +109,18 @@ struct lbm_solver {
   lbm_timing timing{};
   std::vector<hipEvent_t> ev;  // timing pool
   size_t dev_bytes = 0;
+  // batched instances
+  int ninst = 0;
+  float* d_inst_par = nullptr;
+  uint16_t* d_tile_inst = nullptr;
+  uint16_t* d_node_inst = nullptr;
+  uint8_t* d_inst_active = nullptr;
+  int* d_inst_rect = nullptr;        // storage rectangles [ninst][4]
+  double* d_inst_res = nullptr;      // [ninst][4]
+  std::vector<uint8_t> inst_active;
+  std::vector<int> inst_conv_iter;
+  std::vector<double> inst_conv_s, inst_res;
+  double t_first_step = -1.0;
 };
 
 namespace lbmhost {

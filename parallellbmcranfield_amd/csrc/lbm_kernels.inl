@@ -134,7 +134,7 @@ __device__ inline void apply_bc(typename S::R F[9], const BNode& nd, const typen
 template <class S, int COLL, int MODE, bool BND>
 __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
                                    typename S::T* dst, const KParams& p, const KOut& o, int idx,
-                                   const BNode* nd, double (&acc)[NP]) {
+                                   const BNode* nd, double (&acc)[NP], int inst = -1) {
   using R = typename S::R;
   R Rh, U, V, dr = 0;
   if constexpr (S::DEV) {
@@ -209,6 +209,10 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
       o.snap[idx] = rr;
       o.snap[p.plane + idx] = uu;
       o.snap[2 * p.plane + idx] = vv;
+      if (p.ninst) {  // per-cell terms for the per-instance sums
+        o.snap[3 * (size_t)p.plane + idx] = (uu - u0) * (uu - u0) + (vv - v0) * (vv - v0);
+        o.snap[4 * (size_t)p.plane + idx] = uu * uu + vv * vv;
+      }
     }
   }
   if constexpr (MODE == 2) {
@@ -227,7 +231,7 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
   } else {
     R out[9];
     if constexpr (S::DEV)
-      collide_dev<COLL>(F, dr, Rh, U, V, p, out);
+      collide_dev<COLL>(F, dr, Rh, U, V, coll_par(p, inst), out);
     else
       collide_f64<COLL>(F, Rh, U, V, p, out);
 #pragma unroll
@@ -236,6 +240,25 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
 }
 
 // ---------------------------------------------------------------- kernels
+// Batched instances: the instance of a bulk node (per tile when the tile is
+// all interior, else per cell) and whether it is still being advanced.
+__device__ inline int bulk_instance(const KParams& p, unsigned b, int tf, int idx, bool& active) {
+  if (!p.ninst) return -1;
+  int inst;
+  if (tf == 1) {
+    inst = p.tile_inst[b];
+  } else {
+    inst = active ? (int)p.node_inst[idx] : -1;
+    if (inst == 0xffff) inst = -1;
+  }
+  if (inst >= 0 && p.inst_active && !p.inst_active[inst]) active = false;
+  return inst;
+}
+
+__device__ inline bool bnode_live(const KParams& p, const BNode& nd) {
+  return !p.ninst || nd.inst < 0 || !p.inst_active || p.inst_active[nd.inst];
+}
+
 // tflag[t] for the tile of BS consecutive cells starting at bulk_begin + t*BS:
 // 0 no interior node, 1 all interior, 2 mixed (read the per-cell class).
 template <class S, int COLL, int MODE, int BS>
@@ -253,7 +276,8 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
     if (tf == 0) return;
   }
   const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
-  const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+  bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+  const int inst = bulk_instance(p, b, tf, idx, active);
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if constexpr (MODE != 1) {
     if (!active) return;
@@ -262,7 +286,7 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
     R F[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
-    finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc);
+    finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst);
   }
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
@@ -287,7 +311,7 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
   using R = typename S::R;
   const int i = blockIdx.x * 256 + threadIdx.x;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
+  if (i < nbn && !(bn[i].flags & BN_OUTLET) && bnode_live(p, bn[i])) {
     const BNode& nd = bn[i];  // fields read where used
     R F[9];
     pull_masked<S>(F, nd, src, p);
@@ -297,7 +321,7 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
 #pragma unroll
       for (int k = 0; k < 9; k++) side[k] = F[k];
     }
-    finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
+    finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst);
   }
   if constexpr (MODE == 1) block_sum<256>(acc, o.part, p.nb_bulk + blockIdx.x);
 }
@@ -317,7 +341,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if ((int)blockIdx.x < nbb) {
     const int i = blockIdx.x * BS + threadIdx.x;
-    if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
+    if (i < nbn && !(bn[i].flags & BN_OUTLET) && bnode_live(p, bn[i])) {
       const BNode& nd = bn[i];  // fields read where used
       R F[9];
       pull_masked<S>(F, nd, src, p);
@@ -327,7 +351,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
 #pragma unroll
         for (int k = 0; k < 9; k++) side[k] = F[k];
       }
-      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
+      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst);
     }
   } else {
     unsigned b = blockIdx.x - nbb;
@@ -338,7 +362,8 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
       if (tf == 0) return;
     }
     const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
-    const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+    bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+    const int inst = bulk_instance(p, b, tf, idx, active);
     if constexpr (MODE != 1) {
       if (!active) return;
     }
@@ -346,7 +371,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
       R F[9];
 #pragma unroll
       for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
-      finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc);
+      finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst);
     }
   }
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
@@ -427,7 +452,9 @@ __global__ __launch_bounds__(256) void k_init(typename S::T* __restrict__ dst,
     float g[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) g[k] = (float)(F0[k] - W64[k] * p.rho0);
-    collide_dev<COLL>(g, (float)(rho - p.rho0), (float)rho, (float)u, (float)v, p, out);
+    const int inst = p.ninst ? (int)p.node_inst[idx] : -1;
+    collide_dev<COLL>(g, (float)(rho - p.rho0), (float)rho, (float)u, (float)v,
+                      coll_par(p, inst == 0xffff ? -1 : inst), out);
   } else {
     collide_f64<COLL>(F0, rho, u, v, p, out);
   }

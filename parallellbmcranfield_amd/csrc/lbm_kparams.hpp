@@ -4,6 +4,8 @@
 
 namespace lbmdev {
 
+constexpr int INST_PAR_W = 12;  // batched parameter row: omega, omega-, e[9], unused
+
 enum KFlags : int {
   KF_TRT_FP64 = 1,       // TRT parts in FP64 (reference rounds them to binary32)
   KF_MRT_TEXTBOOK = 2,   // textbook MRT e/eps equilibria
@@ -37,6 +39,12 @@ struct KParams {
   float scale_f, inv_scale_f;
   double stmiv[9][9];  // FP64 MRT: M^-1 S exactly as MRTInitializer forms it
   float mrt_e[9];      // FP32 MRT: s_l / |M_l|^2
+  // batched instances (deviation storage only); ninst == 0 otherwise
+  int ninst;
+  const float* inst_par;        // [ninst][INST_PAR_W]
+  const uint16_t* tile_inst;    // instance of each all-interior bulk tile
+  const uint16_t* node_inst;    // instance of each padded cell (0xffff: none)
+  const uint8_t* inst_active;   // 0: converged, frozen
 };
 
 // Boundary-node record (host builds, kernel reads).  One per fluid node that
@@ -54,7 +62,7 @@ struct BNode {
   uint16_t pad1;
   int32_t boundary_id;
   int32_t feed1, feed2;  // side slots of the (i-1) and (i-2) feeders (outlet nodes), -1 = solid
-  int32_t pad2;
+  int32_t inst;     // batched instance, -1 when not batched
   double q[9];      // Q[k]
   double uo;        // Uo
   double x, y;      // CoordX, CoordY (drag/lift placeholder)

@@ -43,6 +43,8 @@ hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf);
 hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st);
 // The reference residual's sweep (serial/ComputeResiduals.c:19-30) over the
 // cres rows of the first m cells -> out[4] = (sum, drag, lift, 0).
+hipError_t launch_inst_resid(const double* snap, int plane, int pitch, const int* rect,
+                             const uint8_t* active, int ninst, double* out, hipStream_t st);
 hipError_t launch_ref_residual(const double* cres, int m, int n, double* out, hipStream_t st);
 
 // Plain device copy of `bytes` (multiple of 16).

@@ -1,6 +1,7 @@
 // solver.cpp — device state, node classification and the step loop behind
 // the lbm_solver_* entry points.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -168,6 +169,7 @@ int observe(lbm_solver* s, bool with_f) {
   }
   Launch L = make_launch(s, s->cur ^ 1);  // previous buffer holds METAF_t
   L.dst = nullptr;
+  L.kp.inst_active = nullptr;  // observe converged instances too
   if (with_f) {
     L.out.f = s->d_f;
     L.out.metaf = s->d_metaf;
@@ -189,9 +191,20 @@ void depad(const lbm_solver* s, const T* padded, T* out, long plane_off) {
 }  // namespace
 
 static int solver_create(const lbm_case* c, const lbm_options* o, const double* frho,
-                         const double* fu, const double* fv, lbm_solver** out) {
+                         const double* fu, const double* fv, const lbm_instance* inst,
+                         int ninst, lbm_solver** out) {
   if (!c || !o || !out) return LBM_EINVAL;
   *out = nullptr;
+  if (c->ninst != ninst || (ninst > 0 && !inst)) {
+    set_error(c->ninst ? "a packed case needs lbm_solver_create_batched with one parameter "
+                         "set per instance"
+                       : "instance parameters given for a case that is not packed");
+    return LBM_EINVAL;
+  }
+  if (ninst > 0 && o->precision == LBM_PREC_F64) {
+    set_error("batched instances use FP32 or FP16 storage");
+    return LBM_EINVAL;
+  }
   if (o->collision < 1 || o->collision > 3) {
     set_error("collision model must be 1 (BGKW), 2 (TRT) or 3 (MRT)");
     return LBM_EINVAL;
@@ -218,6 +231,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   s->cs = c;
   s->opt = *o;
   if (s->opt.resid_every < 1) s->opt.resid_every = 1;
+  // batched instances converge one by one on their own velocity residual
+  if (ninst > 0) s->opt.resid_kind = LBM_RESID_VELOCITY;
   s->prec = o->precision;
   s->coll = o->collision;
   s->elem = s->prec == 0 ? 8 : (s->prec == 1 ? 4 : 2);
@@ -275,6 +290,12 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (o->trt_fp64_parts) kp.flags |= KF_TRT_FP64;
   if (o->mrt_textbook_meq) kp.flags |= KF_MRT_TEXTBOOK;
 
+  // Uo of a cell: the inlet profile, or the lid speed of its instance
+  auto cell_uo = [&](long lin) -> double {
+    if (c->ninst && c->cell_inst[lin] >= 0 && o->inlet_profile == 2)
+      return (double)inst[c->cell_inst[lin]].uavg;
+    return inlet_uo(c, o, lin);
+  };
   // ---- node classes, boundary list, open-outlet rows
   std::vector<uint8_t> cls((size_t)s->plane, 0);
   const long nc = (long)n * m;
@@ -335,7 +356,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     r.boundary = (int8_t)(B ? B->boundary : 0);
     r.stream = (uint16_t)(B ? B->stream : 0x1ff);
     r.boundary_id = B ? B->boundary_id : 0;
-    r.uo = inlet_uo(c, o, lin);
+    r.uo = cell_uo(lin);
+    r.inst = c->ninst ? (int32_t)c->cell_inst[lin] : -1;
     r.x = c->x(lin);
     r.y = c->y(lin);
     if (lin < m) r.flags |= BN_COMPAT_RES;
@@ -388,6 +410,51 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     tflag[t] = nb == 0 ? 0 : (other == 0 ? 1 : 2);
   }
   kp.nb_bulk = s->nb_bulk;
+  // ---- batched instances: parameter rows, instance of each cell / pure tile
+  std::vector<float> inst_par;
+  std::vector<uint16_t> node_inst, tile_inst;
+  std::vector<int> srect;
+  if (ninst > 0) {
+    s->ninst = ninst;
+    inst_par.assign((size_t)ninst * INST_PAR_W, 0.0f);
+    for (int k = 0; k < ninst; k++) {
+      const double om = 1.0 / (3. * (double)inst[k].viscosity + 0.5);
+      double oma = 8 * (2 - om) / (8 - om);
+      if (o->trt_lambda > 0) oma = 1.0 / (o->trt_lambda / (1.0 / om - 0.5) + 0.5);
+      double smk[9];
+      std::memcpy(smk, sm, sizeof smk);
+      smk[7] = smk[8] = om;  // stress moments relax at 1/tau of the instance
+      float* r = &inst_par[(size_t)k * INST_PAR_W];
+      r[0] = (float)om;
+      r[1] = (float)oma;
+      for (int l = 0; l < 9; l++) r[2 + l] = (float)(smk[l] / norm[l]);
+    }
+    node_inst.assign((size_t)s->plane, 0xffff);
+    for (long lin = 0; lin < nc; lin++) {
+      const int ii = (int)(lin % n), jj = (int)(lin / n);
+      if (c->cell_inst[lin] >= 0)
+        node_inst[(size_t)(jj + 1) * s->pitch + (ii + 1)] = (uint16_t)c->cell_inst[lin];
+    }
+    tile_inst.assign((size_t)s->nb_bulk, 0);
+    for (int t = 0; t < s->nb_bulk; t++) {
+      const long idx = (long)kp.bulk_begin + (long)t * BS;
+      if (tflag[t] != 1) continue;
+      tile_inst[t] = node_inst[idx];
+      for (int e = 1; e < BS; e++)
+        if (node_inst[idx + e] != tile_inst[t]) tflag[t] = 2;  // spans two instances
+    }
+    srect.resize((size_t)ninst * 4);
+    for (int k = 0; k < ninst; k++) {
+      srect[4 * k] = c->inst_rect[4 * k] + 1;  // storage column of the rectangle
+      srect[4 * k + 1] = c->inst_rect[4 * k + 1] + 1;
+      srect[4 * k + 2] = c->inst_rect[4 * k + 2];
+      srect[4 * k + 3] = c->inst_rect[4 * k + 3];
+    }
+    s->inst_active.assign((size_t)ninst, 1);
+    s->inst_conv_iter.assign((size_t)ninst, -1);
+    s->inst_conv_s.assign((size_t)ninst, -1.0);
+    s->inst_res.assign((size_t)ninst, 100.0);
+  }
   kp.nb_bnd = s->nb_bnd;
   s->nparts = s->nb_bulk + s->nb_bnd + s->nb_out;
 
@@ -408,10 +475,30 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (!rc) rc = dev_alloc(s, (void**)&s->d_v, sizeof(double) * (size_t)s->plane);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_part, sizeof(double) * 6 * (size_t)s->nparts);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_hist, sizeof(double) * HW * HCAP);
-  if (!rc && o->resid_kind == LBM_RESID_REFERENCE)
+  if (!rc && s->opt.resid_kind == LBM_RESID_REFERENCE)
     rc = dev_alloc(s, (void**)&s->d_cres, sizeof(double) * 12 * (size_t)m);
-  if (!rc && o->resid_kind == LBM_RESID_VELOCITY)
-    rc = dev_alloc(s, (void**)&s->d_snap, sizeof(double) * 3 * (size_t)s->plane);
+  if (!rc && s->opt.resid_kind == LBM_RESID_VELOCITY)
+    rc = dev_alloc(s, (void**)&s->d_snap, sizeof(double) * (ninst ? 5 : 3) * (size_t)s->plane);
+  if (!rc && ninst > 0)
+    rc = check_hip(hipMemset(s->d_snap + 3 * (size_t)s->plane, 0, sizeof(double) * 2 * (size_t)s->plane), "clear snapshot");
+  if (!rc && ninst > 0) {
+    rc = dev_alloc(s, (void**)&s->d_inst_par, sizeof(float) * inst_par.size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_node_inst, sizeof(uint16_t) * node_inst.size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_tile_inst, sizeof(uint16_t) * tile_inst.size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_inst_active, (size_t)ninst);
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_inst_rect, sizeof(int) * srect.size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_inst_res, sizeof(double) * 4 * (size_t)ninst);
+    if (!rc) rc = check_hip(hipMemcpy(s->d_inst_par, inst_par.data(), sizeof(float) * inst_par.size(), hipMemcpyHostToDevice), "upload");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_node_inst, node_inst.data(), sizeof(uint16_t) * node_inst.size(), hipMemcpyHostToDevice), "upload");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_tile_inst, tile_inst.data(), sizeof(uint16_t) * tile_inst.size(), hipMemcpyHostToDevice), "upload");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_inst_active, s->inst_active.data(), (size_t)ninst, hipMemcpyHostToDevice), "upload");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_inst_rect, srect.data(), sizeof(int) * srect.size(), hipMemcpyHostToDevice), "upload");
+    kp.ninst = ninst;
+    kp.inst_par = s->d_inst_par;
+    kp.tile_inst = s->d_tile_inst;
+    kp.node_inst = s->d_node_inst;
+    kp.inst_active = s->d_inst_active;
+  }
   if (rc) {
     lbm_solver_destroy(s);
     return rc;
@@ -423,7 +510,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   s->u0.assign((size_t)nc, 0.0);
   s->v0.assign((size_t)nc, 0.0);
   for (long lin = 0; lin < nc; lin++) {
-    s->u0[lin] = inlet_uo(c, o, lin);
+    s->u0[lin] = cell_uo(lin);
     if (c->fluid[lin] != 1) continue;
     if (o->init == LBM_INIT_REST) {
       s->u0[lin] = 0.0;
@@ -534,7 +621,7 @@ int lbm_solver_create(const lbm_case* c, const lbm_options* o, lbm_solver** out)
     set_error("LBM_INIT_FIELD needs lbm_solver_create_field");
     return LBM_EINVAL;
   }
-  return solver_create(c, o, nullptr, nullptr, nullptr, out);
+  return solver_create(c, o, nullptr, nullptr, nullptr, nullptr, c ? c->ninst : 0, out);
 }
 
 int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const double* rho,
@@ -542,7 +629,17 @@ int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const doubl
   if (!o) return LBM_EINVAL;
   lbm_options oo = *o;
   oo.init = LBM_INIT_FIELD;
-  return solver_create(c, &oo, rho, u, v, out);
+  return solver_create(c, &oo, rho, u, v, nullptr, c ? c->ninst : 0, out);
+}
+
+int lbm_solver_create_batched(const lbm_case* c, const lbm_options* o, const lbm_instance* inst,
+                              int count, lbm_solver** out) {
+  if (!c || !o || !inst || count <= 0) return LBM_EINVAL;
+  if (o->init == LBM_INIT_FIELD) {
+    set_error("batched solvers start from the reference or rest state");
+    return LBM_EINVAL;
+  }
+  return solver_create(c, o, nullptr, nullptr, nullptr, inst, count, out);
 }
 
 void lbm_solver_destroy(lbm_solver* s) {
@@ -550,7 +647,8 @@ void lbm_solver_destroy(lbm_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_tflag, s->d_bn, s->d_on, s->d_orow, s->d_side,
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
-                  s->d_part, s->d_hist};
+                  s->d_part, s->d_hist, s->d_inst_par, s->d_tile_inst, s->d_node_inst,
+                  s->d_inst_active, s->d_inst_rect, s->d_inst_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -560,8 +658,72 @@ void lbm_solver_destroy(lbm_solver* s) {
   delete s;
 }
 
+static double wall_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Per-instance velocity residuals of a batched solver after a residual step:
+// instances below crit_v stop advancing (their state is copied into the other
+// buffer so that later steps and observations see one fixed state).
+static int batch_eval(lbm_solver* s) {
+  const int ni = s->ninst;
+  HC(launch_inst_resid(s->d_snap, s->plane, s->pitch, s->d_inst_rect, s->d_inst_active, ni,
+                       s->d_inst_res, s->stream));
+  std::vector<double> raw(2 * (size_t)ni);
+  HC(hipMemcpyAsync(raw.data(), s->d_inst_res, raw.size() * sizeof(double), hipMemcpyDeviceToHost,
+                    s->stream));
+  HC(hipStreamSynchronize(s->stream));
+  const double now = wall_seconds() - s->t_first_step;
+  const double K = (double)s->opt.resid_every;
+  const double crit = s->opt.crit_v;
+  double worst = 0.0;
+  bool changed = false;
+  std::vector<int> rect(4 * (size_t)ni);
+  for (int k = 0; k < ni; k++) {
+    if (!s->inst_active[k]) continue;
+    const double a = raw[2 * k], b = raw[2 * k + 1];
+    const double r = b > 0 ? std::sqrt(a) / (K * std::sqrt(b)) : std::sqrt(a);
+    s->inst_res[k] = r;
+    if (!(r == r)) {  // this instance diverged: stop it, the others go on
+      s->inst_active[k] = 0;
+      s->inst_conv_iter[k] = -2;
+      s->inst_conv_s[k] = now;
+      changed = true;
+      continue;
+    }
+    if (crit > 0 && r < crit) {
+      s->inst_active[k] = 0;
+      s->inst_conv_iter[k] = s->iter;
+      s->inst_conv_s[k] = now;
+      changed = true;
+      const int x0 = s->cs->inst_rect[4 * k] + 1, y0 = s->cs->inst_rect[4 * k + 1] + 1;
+      const int w = s->cs->inst_rect[4 * k + 2], h = s->cs->inst_rect[4 * k + 3];
+      char* from = (char*)s->buf[s->cur];
+      char* to = (char*)s->buf[s->cur ^ 1];
+      for (int q = 0; q < 9; q++) {
+        const size_t off = ((size_t)q * s->plane + (size_t)y0 * s->pitch + x0) * s->elem;
+        HC(hipMemcpy2DAsync(to + off, s->pitch * s->elem, from + off, s->pitch * s->elem,
+                            (size_t)w * s->elem, (size_t)h, hipMemcpyDeviceToDevice, s->stream));
+      }
+      continue;
+    }
+    worst = std::max(worst, r);
+  }
+  if (changed) {
+    HC(hipMemcpyAsync(s->d_inst_active, s->inst_active.data(), (size_t)ni, hipMemcpyHostToDevice,
+                      s->stream));
+    HC(hipStreamSynchronize(s->stream));
+  }
+  s->resid[0] = s->resid[1] = worst;
+  s->resid[2] = s->resid[3] = 0.0;
+  s->hist_iter.push_back(s->iter);
+  s->hist_val.insert(s->hist_val.end(), s->resid, s->resid + 4);
+  return LBM_OK;
+}
+
 static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_t>* evs,
                      int sample_every = 1) {
+  if (s->ninst && s->t_first_step < 0) s->t_first_step = wall_seconds();
   if (done) *done = 0;
   if (s->diverged) {
     set_error("solver diverged");
@@ -625,18 +787,26 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
       HC(pass(s, mode, PASS_BOUNDARY, L));
     }
     HC(pass(s, mode, PASS_OUTLET, L));
-    if (eval) {
+    const bool row_r = reval && !s->ninst;  // batched residuals are per instance
+    if (row_r || feval) {
       double* slot = s->d_hist + (size_t)s->pending * HW;
       if (mode == MODE_RESID) HC(launch_reduce_partials(s->d_part, s->nparts, slot, s->stream));
-      if (reval && rk == LBM_RESID_REFERENCE)
+      if (row_r && rk == LBM_RESID_REFERENCE)
         HC(launch_ref_residual(s->d_cres, s->cs->m, s->cs->n, slot, s->stream));
       s->pending_iter.push_back(s->iter);
-      s->pending_what.push_back((reval ? 1 : 0) | (feval ? 2 : 0));
+      s->pending_what.push_back((row_r ? 1 : 0) | (feval ? 2 : 0));
       s->pending++;
     }
     s->cur ^= 1;
     s->iter++;
     ran++;
+    if (s->ninst && reval) {
+      int rc = batch_eval(s);
+      if (rc) {
+        if (done) *done = ran;
+        return rc;
+      }
+    }
     if (s->pending == HCAP || (reval && stop_can_trigger)) {
       int rc = flush_history(s);
       if (rc) {
@@ -850,6 +1020,10 @@ int copy_in(lbm_solver* s, FILE* f, void* dev, size_t bytes) {
 
 int lbm_solver_save(lbm_solver* s, const char* path) {
   if (!s || !path) return LBM_EINVAL;
+  if (s->ninst) {
+    set_error("checkpoints hold single-lattice solvers only");
+    return LBM_EINVAL;
+  }
   int rc = flush_history(s);
   if (rc && rc != LBM_DIVERGED) return rc;
   HC(hipStreamSynchronize(s->stream));
@@ -884,6 +1058,10 @@ int lbm_solver_save(lbm_solver* s, const char* path) {
 
 int lbm_solver_load(lbm_solver* s, const char* path) {
   if (!s || !path) return LBM_EINVAL;
+  if (s->ninst) {
+    set_error("checkpoints hold single-lattice solvers only");
+    return LBM_EINVAL;
+  }
   FILE* f = std::fopen(path, "rb");
   if (!f) {
     set_error(std::string("cannot open ") + path);
@@ -948,3 +1126,19 @@ int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs) {
 }
 
 }  // extern "C"
+
+int lbm_batch_status(lbm_solver* s, int count, int* conv_iter, double* conv_seconds,
+                     double* resid) {
+  if (!s || count < 0) return LBM_EINVAL;
+  if (!s->ninst) {
+    set_error("not a batched solver");
+    return LBM_EINVAL;
+  }
+  const int k = std::min(count, s->ninst);
+  for (int i = 0; i < k; i++) {
+    if (conv_iter) conv_iter[i] = s->inst_conv_iter[i];
+    if (conv_seconds) conv_seconds[i] = s->inst_conv_s[i];
+    if (resid) resid[i] = s->inst_res[i];
+  }
+  return LBM_OK;
+}

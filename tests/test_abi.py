@@ -131,6 +131,66 @@ def test_channel_generator_matches_test_generator():
     assert np.all(q > 0) and np.all(q <= 1.0 + 1e-6) and q.std() > 0.05
 
 
+def test_cavity_ex_square_equals_shipped_cavity():
+    """An obstacle-free square cavity_ex is the shipped cavity cell for cell."""
+    for N in (5, 16, 33):
+        a = lbm.Case.cavity_ex(N, N).cells()
+        b = lbm.Case.cavity(N).cells()
+        for f in a:
+            assert np.array_equal(a[f], b[f]), (N, f)
+
+
+def test_cavity_ex_rectangles_and_aspect():
+    NX, NY = 30, 18
+    rects = [(4, 3, 9, 7), (20, 10, 25, 16)]
+    case = lbm.Case.cavity_ex(NX, NY, rects)
+    assert (case.n, case.m) == (NX + 2, NY + 3)
+    c = case.cells()
+    fl = c["fluid"].reshape(case.m, case.n)
+    for i0, j0, i1, j1 in rects:
+        assert not fl[j0:j1 + 1, i0:i1 + 1].any()
+    interior = fl[1:NY + 1, 1:NX + 1]
+    blocked = sum((i1 - i0 + 1) * (j1 - j0 + 1) for i0, j0, i1, j1 in rects)
+    assert interior.sum() == NX * NY - blocked
+    # fluid nodes next to a rectangle carry wall links (type 1) of boundary id 5
+    bc = c["bc"].reshape(case.m, case.n, 9)
+    bid = c["boundary_id"].reshape(case.m, case.n)
+    assert bc[5, 3, 1] == 1 and bid[5, 3] == 5      # west face of the first rectangle
+    assert bc[2, 6, 2] == 1 and bid[2, 6] == 5      # below it: link north into it
+    # Delta from the longer side
+    assert abs(c["x"].reshape(case.m, case.n)[1, 2] - c["x"].reshape(case.m, case.n)[1, 1]
+               - 1.0 / NX) < 1e-6
+
+
+def test_pack_keeps_every_instance_cell_for_cell():
+    cases = [lbm.Case.cavity_ex(20, 12, [(3, 3, 6, 6)]), lbm.Case.cavity_ex(9, 25),
+             lbm.Case.cavity_ex(14, 14), lbm.Case.cavity_ex(40, 8, [(10, 2, 12, 5)])]
+    packed = lbm.Case.pack(cases)
+    assert packed.instances == 4 and cases[0].instances == 0
+    pc = packed.cells()
+    owner = np.full((packed.m, packed.n), -1)
+    for k, (c, (x0, y0, w, h)) in enumerate(zip(cases, packed.rects)):
+        assert (w, h) == (c.n, c.m)
+        assert (owner[y0:y0 + h, x0:x0 + w] == -1).all(), "rectangles overlap"
+        owner[y0:y0 + h, x0:x0 + w] = k
+        cc = c.cells()
+        for f in ("fluid", "corner", "stream", "boundary", "boundary_id", "bc", "q", "x", "y"):
+            a = pc[f].reshape(packed.m, packed.n, -1)[y0:y0 + h, x0:x0 + w]
+            b = cc[f].reshape(c.m, c.n, -1)
+            assert np.array_equal(a, b), (k, f)
+    assert not pc["fluid"].reshape(packed.m, packed.n)[owner < 0].any()
+    # fluid nodes of different instances are never neighbours
+    fl = pc["fluid"].reshape(packed.m, packed.n).astype(bool)
+    for dj, di in ((0, 1), (1, 0), (1, 1), (1, -1)):
+        a = owner[max(dj, 0):packed.m, max(di, 0):packed.n + min(di, 0)]
+        b = owner[:packed.m - dj, max(-di, 0):packed.n - max(di, 0)]
+        fa = fl[max(dj, 0):packed.m, max(di, 0):packed.n + min(di, 0)]
+        fb = fl[:packed.m - dj, max(-di, 0):packed.n - max(di, 0)]
+        assert not (fa & fb & (a != b)).any()
+    with pytest.raises(lbm.LbmError):
+        packed.write("/tmp/never_n.dat", "/tmp/never_c.dat")
+
+
 def test_ini_reader(tmp_path):
     txt = "0.08\n0.0\n5\n0.01\n2\n3\n2\n2\n5000\n100\n10000\n1\n0\n-1.0\n-1.0\n0\n\n:01: Mean U\n"
     (tmp_path / "SetUpData.ini").write_text(txt)

@@ -1,0 +1,149 @@
+"""Batched multi-instance solver on the GPU, through the C ABI.
+
+Bar: an instance of a packed lattice advances exactly as the same cavity run
+alone (FP32 / FP16 storage, MRT and BGK: bit-identical fields), converged
+instances stop at the iteration the single-lattice solver stops at, and the
+single-lattice cavity_ex path is itself bit-exact against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import parallellbmcranfield_amd as lbm
+
+pytestmark = pytest.mark.gpu
+
+# (NX, NY, rectangles, viscosity, lid speed)
+SET = [
+    (40, 28, [(5, 5, 10, 9)], 0.02, 0.1),
+    (23, 47, [], 0.05, 0.08),
+    (64, 33, [(30, 4, 34, 20), (50, 25, 55, 30)], 0.01, 0.1),
+    (17, 17, [], 0.1, 0.05),
+]
+
+
+def batch_opts(prec, coll=3, **kw):
+    return lbm.make_options(collision=coll, uavg=0.1, rho_ini=1.0, viscosity=0.02,
+                            precision=prec, init=lbm.INIT_REST,
+                            resid_kind=lbm.RESID_VELOCITY, **kw)
+
+
+def single(NX, NY, rects, nu, u, prec, coll, steps, **kw):
+    case = lbm.Case.cavity_ex(NX, NY, rects)
+    o = lbm.make_options(collision=coll, uavg=u, rho_ini=1.0, viscosity=nu, precision=prec,
+                         init=lbm.INIT_REST, resid_kind=lbm.RESID_VELOCITY, **kw)
+    s = lbm.Solver(case, o)
+    s.step(steps)
+    return case, s
+
+
+def crop(a, packed, k):
+    x0, y0, w, h = packed.rects[k]
+    return a.reshape(packed.m, packed.n)[y0:y0 + h, x0:x0 + w].ravel()
+
+
+@pytest.mark.parametrize("prec,coll", [(lbm.PREC_F32, 3), (lbm.PREC_F16, 3), (lbm.PREC_F32, 1)])
+def test_batched_instances_equal_single_runs(prec, coll):
+    print("batched vs single", prec, coll, flush=True)
+    cases = [lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in SET]
+    packed = lbm.Case.pack(cases)
+    s = lbm.Solver(packed, batch_opts(prec, coll, resid_every=50), instances=[(nu, u) for *_, nu, u in SET])
+    steps = 301
+    assert s.step(steps) == steps
+    rho, u, v = s.macro()
+    it, _, res = s.batch_status()
+    assert (it == -1).all() and np.all(res > 0)
+    for k, (nx, ny, r, nu, ul) in enumerate(SET):
+        _, s1 = single(nx, ny, r, nu, ul, prec, coll, steps, resid_every=50)
+        r1, u1, v1 = s1.macro()
+        assert np.array_equal(crop(rho, packed, k), r1), k
+        assert np.array_equal(crop(u, packed, k), u1), k
+        assert np.array_equal(crop(v, packed, k), v1), k
+        # the single run's velocity residual is the instance's residual
+        assert abs(s1.residuals()[0] - res[k]) <= 1e-9 * res[k]
+
+
+def test_batched_convergence_matches_single_runs():
+    print("batched convergence", flush=True)
+    crit, K = 2e-4, 20
+    cases = [lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in SET]
+    packed = lbm.Case.pack(cases)
+    o = batch_opts(lbm.PREC_F32, resid_every=K, crit_v=crit, use_stop_rule=1,
+                   max_iterations=20000)
+    s = lbm.Solver(packed, o, instances=[(nu, u) for *_, nu, u in SET])
+    s.step(20000)
+    it, sec, res = s.batch_status()
+    assert (it > 0).all(), it
+    assert s.iter == it.max()
+    assert np.all(np.diff(sec[np.argsort(it)]) >= 0)
+    rho, u, v = s.macro()
+    for k, (nx, ny, r, nu, ul) in enumerate(SET):
+        case = lbm.Case.cavity_ex(nx, ny, r)
+        o1 = lbm.make_options(collision=3, uavg=ul, rho_ini=1.0, viscosity=nu,
+                              precision=lbm.PREC_F32, init=lbm.INIT_REST,
+                              resid_kind=lbm.RESID_VELOCITY, resid_every=K, crit_v=crit,
+                              use_stop_rule=1, max_iterations=20000)
+        s1 = lbm.Solver(case, o1)
+        s1.step(20000)
+        assert s1.iter == it[k], (k, s1.iter, it[k])
+        # a stopped instance holds the state it converged at; observed one
+        # step on, as the single lattice advanced once more
+        o1.use_stop_rule = 0
+        s2 = lbm.Solver(case, o1)
+        s2.step(int(it[k]) + 1)
+        r2, u2, v2 = s2.macro()
+        assert np.array_equal(crop(u, packed, k), u2), k
+        assert np.array_equal(crop(v, packed, k), v2), k
+        assert np.array_equal(crop(rho, packed, k), r2), k
+
+
+def test_converged_instance_stays_fixed():
+    print("frozen instance", flush=True)
+    SET2 = [(16, 16, [], 0.1, 0.05), (72, 72, [], 0.004, 0.1), (30, 20, [(4, 4, 8, 8)], 0.02, 0.1)]
+    packed = lbm.Case.pack([lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in SET2])
+    o = batch_opts(lbm.PREC_F32, resid_every=10, crit_v=1e-3)
+    s = lbm.Solver(packed, o, instances=[(nu, u) for *_, nu, u in SET2])
+    while not (s.batch_status()[0] >= 0).any():
+        s.step(10)
+        assert s.iter < 50000
+    it0 = s.batch_status()[0].copy()
+    frozen = it0 >= 0
+    a = [[crop(x, packed, k).copy() for x in s.macro()] for k in range(len(SET2))]
+    s.step(7)  # odd count: both buffers of a stopped instance must hold its state
+    b = [[crop(x, packed, k) for x in s.macro()] for k in range(len(SET2))]
+    it1 = s.batch_status()[0]
+    for k in range(len(SET2)):
+        same = all(np.array_equal(x, y) for x, y in zip(a[k], b[k]))
+        if frozen[k]:
+            assert same and it1[k] == it0[k], k
+        else:
+            assert not same, k  # running instances keep advancing
+
+
+def test_cavity_ex_fp64_bit_exact_vs_oracle(tmp_path):
+    print("cavity_ex vs oracle", flush=True)
+    case = lbm.Case.cavity_ex(26, 15, [(6, 4, 9, 8)])
+    nodes, bc = str(tmp_path / "n.dat"), str(tmp_path / "c.dat")
+    case.write(nodes, bc)
+    p = O.params(collision=3, uavg=0.1, rho_ini=1.0, viscosity=0.03, init_rest=1,
+                 resid_kind=3)
+    ref = O.Oracle.from_files(nodes, bc, p)
+    ref.step(150)
+    c2 = lbm.Case.load(nodes, bc)
+    o = lbm.make_options(collision=3, uavg=0.1, rho_ini=1.0, viscosity=0.03,
+                         init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE)
+    s = lbm.Solver(c2, o)
+    s.step(150)
+    for a, b in zip(s.macro(), ref.macro()):
+        assert np.array_equal(a, b)
+
+
+def test_batched_rejects_bad_combinations():
+    packed = lbm.Case.pack([lbm.Case.cavity_ex(8, 8), lbm.Case.cavity_ex(9, 9)])
+    with pytest.raises(lbm.LbmError):  # one parameter set per instance
+        lbm.Solver(packed, batch_opts(lbm.PREC_F32))
+    with pytest.raises(lbm.LbmError):  # FP64 batches are not offered
+        lbm.Solver(packed, batch_opts(lbm.PREC_F64), instances=[(0.1, 0.1)] * 2)
+    s = lbm.Solver(packed, batch_opts(lbm.PREC_F32), instances=[(0.1, 0.1)] * 2)
+    with pytest.raises(lbm.LbmError):
+        s.save("/tmp/never.ckpt")
