@@ -1,0 +1,160 @@
+"""Batched sweeps of lid-driven cavities on one GPU.
+
+A sweep is a set of cavities of uneven size, aspect ratio and Reynolds number,
+each with a few solid rectangles, packed into one lattice
+(``lbm_case_pack``) and advanced together by the batched solver
+(``lbm_solver_create_batched``): every instance runs until its own velocity
+residual ||u(t) - u(t-K)|| / (K ||u(t)||) falls below the criterion, then
+stops while the others go on.  The same cavities can be run one at a time
+with the single-lattice solver for comparison.
+
+No counterpart in the reference (its solver handles one lattice per run);
+each instance is the reference's cavity (cases/coarse) with the geometry
+generalised.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import (MRT, PREC_F32, INIT_REST, RESID_VELOCITY, TUNE_NO_GRAPH, Case, Solver,
+               make_options)
+
+
+def random_cavities(count: int = 256, seed: int = 0, side=(65, 513), aspect=(0.5, 2.0),
+                    re=(100.0, 3200.0), lid: float = 0.1, max_rects: int = 3,
+                    rect_frac=(0.05, 0.25)) -> list[dict]:
+    """Seeded cavity specifications: NX uniform in `side` (integers), NY =
+    NX / aspect (aspect uniform in `aspect`, NY clipped to `side`), Re
+    log-uniform in `re` on the lid length NX (nu = lid NX / Re), 0..max_rects
+    solid rectangles with sides rect_frac x the cavity side, kept two nodes
+    clear of the walls."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx = int(rng.integers(side[0], side[1] + 1))
+        a = float(rng.uniform(*aspect))
+        ny = int(min(max(round(nx / a), side[0]), side[1]))
+        Re = float(math.exp(rng.uniform(math.log(re[0]), math.log(re[1]))))
+        rects = []
+        for _ in range(int(rng.integers(0, max_rects + 1))):
+            w = max(1, int(round(rng.uniform(*rect_frac) * nx)))
+            h = max(1, int(round(rng.uniform(*rect_frac) * ny)))
+            i0 = int(rng.integers(3, max(4, nx - w - 2)))
+            j0 = int(rng.integers(3, max(4, ny - h - 2)))
+            rects.append((i0, j0, min(i0 + w - 1, nx - 2), min(j0 + h - 1, ny - 3)))
+        out.append(dict(NX=nx, NY=ny, rects=rects, Re=Re, nu=lid * nx / Re, lid=lid))
+    return out
+
+
+def sweep_options(precision=PREC_F32, crit=1e-6, resid_every=100, max_steps=10**6,
+                  rho0=1.0, **kw):
+    """MRT (Lallemand-Luo rates with 1/tau on the stress moments, textbook
+    equilibria), rest start, per-instance velocity residual every
+    `resid_every` steps, stop at `crit`."""
+    o = make_options(collision=MRT, uavg=0.1, rho_ini=rho0, viscosity=0.01,
+                     precision=precision, init=INIT_REST, resid_kind=RESID_VELOCITY,
+                     resid_every=resid_every, crit_v=crit, use_stop_rule=1,
+                     max_iterations=max_steps, mrt_textbook_meq=1,
+                     mrt_rates=mrt_rates(0.01), **kw)
+    return o
+
+
+def mrt_rates(nu: float) -> tuple:
+    """Lallemand-Luo rates (0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1/tau, 1/tau); a
+    batched solver replaces the last two by each instance's 1/tau."""
+    om = 1.0 / (3.0 * float(np.float32(nu)) + 0.5)
+    return (0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, om, om)
+
+
+def fluid_nodes(case: Case) -> int:
+    return int(case.cells()["fluid"].sum())
+
+
+def build_batch(specs):
+    cases = [Case.cavity_ex(s["NX"], s["NY"], s["rects"]) for s in specs]
+    fl = [fluid_nodes(c) for c in cases]
+    packed = Case.pack(cases)
+    return packed, fl
+
+
+def run_batched(specs, opts=None, chunk: int = 2000, progress=None) -> dict:
+    """Advance all instances in one batched solver until each has converged
+    (or max_iterations).  Returns per-instance steps and latency and the
+    aggregate MLUPS of the useful work (fluid nodes x own steps / wall)."""
+    opts = opts or sweep_options()
+    packed, fl = build_batch(specs)
+    s = Solver(packed, opts, instances=[(sp["nu"], sp["lid"]) for sp in specs])
+    s.sync()
+    t0 = time.perf_counter()
+    while True:
+        done = s.step(chunk)
+        if progress:
+            it, _, _ = s.batch_status()
+            progress(s.iter, int((it != -1).sum()), len(specs))
+        if done < chunk or s.iter >= opts.max_iterations:
+            break
+    s.sync()
+    wall = time.perf_counter() - t0
+    it, sec, res = s.batch_status()
+    steps = np.where(it > 0, it, s.iter)
+    work = float(np.dot(np.asarray(fl, np.float64), steps))
+    return dict(wall_s=wall, iters=int(s.iter), conv_iter=it, conv_s=sec, resid=res,
+                fluid=np.asarray(fl), lattice=(packed.n, packed.m),
+                mlups=work / wall / 1e6, solver=s, case=packed)
+
+
+def run_single(spec, opts=None, chunk: int = 5000) -> dict:
+    """One cavity alone with the single-lattice solver, to the same criterion."""
+    opts = opts or sweep_options()
+    o = make_options(collision=MRT, uavg=spec["lid"], rho_ini=opts.rho_ini,
+                     viscosity=spec["nu"], precision=opts.precision, init=INIT_REST,
+                     resid_kind=RESID_VELOCITY, resid_every=opts.resid_every,
+                     crit_v=opts.crit_v, use_stop_rule=1, max_iterations=opts.max_iterations,
+                     mrt_textbook_meq=opts.mrt_textbook_meq, mrt_rates=mrt_rates(spec["nu"]))
+    case = Case.cavity_ex(spec["NX"], spec["NY"], spec["rects"])
+    s = Solver(case, o)
+    s.sync()
+    t0 = time.perf_counter()
+    while True:
+        done = s.step(chunk)
+        if done < chunk or s.iter >= o.max_iterations:
+            break
+    s.sync()
+    wall = time.perf_counter() - t0
+    fl = fluid_nodes(case)
+    return dict(wall_s=wall, iters=int(s.iter), fluid=fl, mlups=fl * s.iter / wall / 1e6,
+                resid=float(s.residuals()[0]))
+
+
+def autotune(case: Case, opts, instances=None, steps: int = 200,
+             blocks=(256, 512), groups=(1, 2, 4, 8)) -> tuple[dict, list]:
+    """Time `steps` plain steps for each block size / XCD tile-group choice
+    on this lattice and return the fastest setting (and every timing).
+    Block 256 runs the fused step kernel; larger blocks split bulk and
+    boundary launches."""
+    results = []
+    for bs in blocks:
+        for g in (groups if bs == 256 else (1,)):
+            o = type(opts).from_buffer_copy(opts)
+            o.block_size, o.xcd_group = bs, g
+            o.use_stop_rule = 0
+            o.resid_kind = 3
+            o.crit_v = -1.0  # no instance stops while timing
+            s = Solver(case, o, instances=instances) if instances else Solver(case, o)
+            s.step(20)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            dt = (time.perf_counter() - t0) / steps
+            results.append(dict(block_size=bs, xcd_group=g, ms_per_step=dt * 1e3))
+            s.close()
+    best = min(results, key=lambda r: r["ms_per_step"])
+    return best, results
+
+
+__all__ = ["random_cavities", "sweep_options", "build_batch", "run_batched", "run_single",
+           "autotune", "fluid_nodes", "TUNE_NO_GRAPH"]
