@@ -285,6 +285,37 @@ int lbm_iteration(const char* NodeDataFile, const char* BCconnectorDataFile, flo
                   int CalculateDragLift, float ConvergenceCritVeloc, float ConvergenceCritRho,
                   int ReadFormerData);
 
+/* ---------------------------------------------------------------- warm start */
+/* The warm-start network: a U-Net of 3x3 convolutions (BF16 weights and
+ * activations, FP32 accumulation on the matrix cores, hand-written backward
+ * and Adam) mapping (fluid mask, normalised log Re, normalised lid speed) on
+ * an H x W grid to the steady (u, v, rho) there.  `levels` resolutions with
+ * base << l channels (l = 0 .. levels-1); base a multiple of 32.  Arrays are
+ * [sample][H][W][3] FP32, channel 0 of the input is the fluid mask (1/0) and
+ * weighs the loss.  No counterpart in the reference. */
+typedef struct lbm_net lbm_net;
+int lbm_net_create(int H, int W, int batch, int base, int levels, unsigned seed, lbm_net** out);
+void lbm_net_destroy(lbm_net* n);
+long lbm_net_num_params(const lbm_net* n);
+/* FP32 master parameters in canonical order: per conv (encoder a/b per
+ * level incl. the bottleneck, then decoder a/b from the coarsest level)
+ * W[Cout][9][Cin] then bias[Cout] (Cin of the first conv padded to 32);
+ * then the 1x1 head W[3][base], bias[3].  set_params resets Adam. */
+int lbm_net_get_params(lbm_net* n, float* params);
+int lbm_net_set_params(lbm_net* n, const float* params);
+/* Prediction for nb samples (any count; run in batches). */
+int lbm_net_predict(lbm_net* n, const float* in, int nb, float* out);
+/* Masked mean-square loss and its gradient w.r.t. every parameter
+ * (canonical order) for nb <= batch samples; grads may be NULL. */
+int lbm_net_gradients(lbm_net* n, const float* in, const float* target, int nb, double* loss,
+                      float* grads);
+/* Device-resident training set (count samples), then one Adam step
+ * (b1 0.9, b2 0.999, eps 1e-8) on the samples idx[0..nb). */
+int lbm_net_set_data(lbm_net* n, const float* in, const float* target, int count);
+int lbm_net_train(lbm_net* n, const int* idx, int nb, float lr, double* loss);
+int lbm_net_save(lbm_net* n, const char* path);
+int lbm_net_load(lbm_net* n, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

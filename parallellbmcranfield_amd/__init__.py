@@ -44,6 +44,9 @@ EXPORTS = [
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
     "lbm_solver_fused", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs", "lbm_run",
     "lbm_run_checkpointed", "lbm_iteration",
+    "lbm_net_create", "lbm_net_destroy", "lbm_net_num_params", "lbm_net_get_params",
+    "lbm_net_set_params", "lbm_net_predict", "lbm_net_gradients", "lbm_net_set_data",
+    "lbm_net_train", "lbm_net_save", "lbm_net_load",
 ]
 
 
@@ -152,6 +155,17 @@ def lib() -> C.CDLL:
         "lbm_run_checkpointed": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                      C.POINTER(Options), C.c_char_p, C.c_char_p, I]),
         "lbm_run": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Options), I]),
+        "lbm_net_create": (I, [I, I, I, I, I, C.c_uint, C.POINTER(P)]),
+        "lbm_net_destroy": (None, [P]),
+        "lbm_net_num_params": (C.c_long, [P]),
+        "lbm_net_get_params": (I, [P, P]),
+        "lbm_net_set_params": (I, [P, P]),
+        "lbm_net_predict": (I, [P, P, I, P]),
+        "lbm_net_gradients": (I, [P, P, P, I, C.POINTER(D), P]),
+        "lbm_net_set_data": (I, [P, P, P, I]),
+        "lbm_net_train": (I, [P, P, I, C.c_float, C.POINTER(D)]),
+        "lbm_net_save": (I, [P, C.c_char_p]),
+        "lbm_net_load": (I, [P, C.c_char_p]),
         "lbm_iteration": (I, [C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
                               C.c_float, I, I, I, I, I, I, I, I, I, C.c_float, C.c_float, I]),
     }
@@ -413,6 +427,76 @@ class Solver:
         if h and _lib is not None:
             _lib.lbm_solver_destroy(h)
         self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Net:
+    """The warm-start network (lbm_net_*): U-Net of 3x3 convolutions on an
+    H x W grid, BF16 on the matrix cores, trained with Adam on the device.
+    Arrays are [sample, H, W, 3] float32; input channel 0 is the fluid mask."""
+
+    def __init__(self, H: int, W: int, batch: int = 16, base: int = 32, levels: int = 4,
+                 seed: int = 0):
+        h = C.c_void_p()
+        _check(lib().lbm_net_create(H, W, batch, base, levels, seed, C.byref(h)))
+        self.h = h
+        self.H, self.W, self.batch, self.base, self.levels = H, W, batch, base, levels
+
+    @property
+    def num_params(self) -> int:
+        return lib().lbm_net_num_params(self.h)
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        _check(lib().lbm_net_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().lbm_net_set_params(self.h, _ptr(p)))
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape[1:] == (self.H, self.W, 3), a.shape
+        return a
+
+    def predict(self, inp: np.ndarray) -> np.ndarray:
+        inp = self._arr(inp)
+        out = np.zeros_like(inp)
+        _check(lib().lbm_net_predict(self.h, _ptr(inp), inp.shape[0], _ptr(out)))
+        return out
+
+    def gradients(self, inp: np.ndarray, target: np.ndarray):
+        inp, target = self._arr(inp), self._arr(target)
+        loss = C.c_double()
+        g = np.zeros(self.num_params, np.float32)
+        _check(lib().lbm_net_gradients(self.h, _ptr(inp), _ptr(target), inp.shape[0],
+                                       C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def set_data(self, inp: np.ndarray, target: np.ndarray) -> None:
+        inp, target = self._arr(inp), self._arr(target)
+        _check(lib().lbm_net_set_data(self.h, _ptr(inp), _ptr(target), inp.shape[0]))
+
+    def train(self, idx, lr: float) -> float:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_double()
+        _check(lib().lbm_net_train(self.h, _ptr(idx), idx.size, lr, C.byref(loss)))
+        return loss.value
+
+    def save(self, path: str) -> None:
+        _check(lib().lbm_net_save(self.h, path.encode()))
+
+    def load(self, path: str) -> None:
+        _check(lib().lbm_net_load(self.h, path.encode()))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.lbm_net_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         self.close()

@@ -36,9 +36,11 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math
 UNITS = [
     ("kernels_f64.hip", "hip", ["-ffp-contract=off"]),
     ("kernels_lp.hip", "hip", []),
+    ("nn_kernels.hip", "hip", []),
     ("case_io.cpp", "host", []),
     ("solver.cpp", "host", []),
     ("run.cpp", "host", []),
+    ("nn.cpp", "host", []),
 ]
 
 

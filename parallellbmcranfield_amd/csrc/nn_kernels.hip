@@ -1,0 +1,555 @@
+// nn_kernels.hip — kernels of the warm-start network (a 3x3-conv U-Net on a
+// square grid): BF16 weights and activations, FP32 accumulation on the
+// gfx950 matrix cores (v_mfma_f32_16x16x32_bf16), hand-written backward
+// passes and Adam.
+//
+// Tensors are NHWC (channels contiguous); every conv channel count is a
+// multiple of 32, so one 32-deep K slice of the implicit GEMM is one tap of
+// 32 consecutive channels: a 16-byte load per lane.  A decoder conv reads a
+// virtual concatenation [skip (C0 channels, full size), nearest-upsampled
+// coarse tensor (C1 channels, size ceil(H/2) x ceil(W/2))] without
+// materialising it.
+//
+// MFMA 16x16x32 (bf16) operand layout, lane l = 16*kb + r (r = 0..15):
+//   A (16 x 32, M x K): lane holds A[r][8kb .. 8kb+7]
+//   B (32 x 16, K x N): lane holds B[8kb .. 8kb+7][r]
+//   D (16 x 16):        lane holds D[4kb + i][r], i = 0..3
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nn_launch.hpp"
+
+namespace lbmnn {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct Src {  // virtual concat input
+  const bf16* in0;
+  int C0;
+  const bf16* in1;  // coarse, nearest-upsampled (may be null)
+  int C1;
+  int B, H, W;      // full size
+};
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int e = 0; e < 8; e++) z[e] = (__bf16)0.0f;
+  return z;
+}
+
+// 8 channels [ci, ci+8) of the virtual input at (b, y, x); zeros outside.
+__device__ inline bf16x8 load8(const Src& s, int b, int y, int x, int ci) {
+  if (y < 0 || y >= s.H || x < 0 || x >= s.W) return zero8();
+  if (ci < s.C0) {
+    const size_t off = (((size_t)b * s.H + y) * s.W + x) * s.C0 + ci;
+    return *reinterpret_cast<const bf16x8*>(s.in0 + off);
+  }
+  const int Hc = (s.H + 1) >> 1, Wc = (s.W + 1) >> 1;
+  const size_t off = (((size_t)b * Hc + (y >> 1)) * Wc + (x >> 1)) * s.C1 + (ci - s.C0);
+  return *reinterpret_cast<const bf16x8*>(s.in1 + off);
+}
+
+__device__ inline float load1(const Src& s, int b, int y, int x, int ci) {
+  if (y < 0 || y >= s.H || x < 0 || x >= s.W) return 0.0f;
+  if (ci < s.C0) return (float)s.in0[(((size_t)b * s.H + y) * s.W + x) * s.C0 + ci];
+  const int Hc = (s.H + 1) >> 1, Wc = (s.W + 1) >> 1;
+  return (float)s.in1[(((size_t)b * Hc + (y >> 1)) * Wc + (x >> 1)) * s.C1 + (ci - s.C0)];
+}
+
+// ------------------------------------------------------------ conv forward
+// out[p][co] = act(bias[co] + sum_{t, ci} in(nbr(p, t))[ci] * w[co][t][ci]).
+// Workgroup: 4 waves, each 32 pixels x 32 output channels (2 x 2 MFMA
+// tiles); COW waves along the output channels, 4/COW along the pixels.
+template <int COW, bool RELU>
+__global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__ w,
+                                                  const float* __restrict__ bias,
+                                                  bf16* __restrict__ out, int Cout) {
+  constexpr int PXW = 4 / COW;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kb = lane >> 4;
+  const int P = s.B * s.H * s.W;
+  const int Cin = s.C0 + s.C1;
+  const int pb = blockIdx.x * (32 * PXW) + (wave % PXW) * 32;
+  const int cb = blockIdx.y * (32 * COW) + (wave / PXW) * 32;
+  if (cb >= Cout || pb >= P) return;  // no LDS, no barrier: whole waves may leave
+  int pb_[2], py[2], px[2];
+  bool pv[2];
+#pragma unroll
+  for (int g = 0; g < 2; g++) {
+    const int p = pb + g * 16 + r;
+    pv[g] = p < P;
+    const int pp = pv[g] ? p : 0;
+    px[g] = pp % s.W;
+    const int q = pp / s.W;
+    py[g] = q % s.H;
+    pb_[g] = q / s.H;
+  }
+  bool cv[2];
+  const bf16* wrow[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int co = cb + h * 16 + r;
+    cv[h] = co < Cout;
+    wrow[h] = w + (size_t)(cv[h] ? co : 0) * 9 * Cin + kb * 8;
+  }
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int g = 0; g < 2; g++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 9; t++) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+      bf16x8 a[2], bm[2];
+#pragma unroll
+      for (int g = 0; g < 2; g++)
+        a[g] = pv[g] ? load8(s, pb_[g], py[g] + dy, px[g] + dx, ci0 + kb * 8) : zero8();
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        bm[h] = cv[h] ? *reinterpret_cast<const bf16x8*>(wrow[h] + (size_t)t * Cin + ci0)
+                      : zero8();
+#pragma unroll
+      for (int g = 0; g < 2; g++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int co = cb + h * 16 + r;
+    if (co >= Cout) continue;
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int p = pb + g * 16 + kb * 4 + i;
+        if (p >= P) continue;
+        float v = acc[g][h][i] + bv;
+        if (RELU) v = v > 0.0f ? v : 0.0f;
+        out[(size_t)p * Cout + co] = (__bf16)v;
+      }
+  }
+}
+
+// ------------------------------------------------------------ weight gradient
+// part[chunk][co][t][ci] = sum over the chunk's row segments of
+// dout[p][co] * in(nbr(p, t))[ci].  K runs over 32-pixel row segments
+// (pixels (b, y, xs .. xs+31)); the 4 waves of a workgroup take every 4th
+// segment of the chunk and are summed through LDS in a fixed order.
+__global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ dout, int Cout,
+                                               float* __restrict__ part, int seg_per_chunk) {
+  __shared__ float red[4][32][33];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kb = lane >> 4;
+  const int Cin = s.C0 + s.C1;
+  const int cob = blockIdx.y * 32;
+  const int nci = Cin / 32;
+  const int t = blockIdx.z / nci, cib = (blockIdx.z % nci) * 32;
+  const int dy = t / 3 - 1, dx = t % 3 - 1;
+  const int segs_row = (s.W + 31) / 32;
+  const int nseg = s.B * s.H * segs_row;
+  const int s0 = blockIdx.x * seg_per_chunk;
+  const int s1 = min(nseg, s0 + seg_per_chunk);
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int g = 0; g < 2; g++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int sg = s0 + wave; sg < s1; sg += 4) {
+    const int xs = (sg % segs_row) * 32;
+    const int q = sg / segs_row;
+    const int y = q % s.H, b = q / s.H;
+    const size_t rowp = ((size_t)b * s.H + y) * s.W;
+    bf16x8 a[2], bm[2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {  // A: dout^T, rows = co, K = pixels
+      const int co = cob + g * 16 + r;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int x = xs + kb * 8 + e;
+        a[g][e] = (co < Cout && x < s.W) ? dout[(rowp + x) * Cout + co] : (__bf16)0.0f;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {  // B: input, K = pixels, cols = ci
+      const int ci = cib + h * 16 + r;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int x = xs + kb * 8 + e;
+        bm[h][e] = x < s.W ? (__bf16)load1(s, b, y + dy, x + dx, ci) : (__bf16)0.0f;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
+  }
+#pragma unroll
+  for (int g = 0; g < 2; g++)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) red[wave][g * 16 + kb * 4 + i][h * 16 + r] = acc[g][h][i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int row = e / 32, col = e % 32;
+    const int co = cob + row, ci = cib + col;
+    if (co >= Cout) continue;
+    const float v = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
+    part[(((size_t)blockIdx.x * Cout + co) * 9 + t) * Cin + ci] = v;
+  }
+}
+
+// part[chunk][c] = sum over the chunk's pixels of g[p][c] (bias gradient).
+__global__ __launch_bounds__(256) void k_colsum_part(const bf16* __restrict__ g, int P, int C,
+                                                     int px_per_chunk, float* __restrict__ part) {
+  const int p0 = blockIdx.x * px_per_chunk, p1 = min(P, p0 + px_per_chunk);
+  for (int c = threadIdx.x; c < C; c += 256) {
+    float s = 0.0f;
+    for (int p = p0; p < p1; p++) s += (float)g[(size_t)p * C + c];
+    part[(size_t)blockIdx.x * C + c] = s;
+  }
+}
+
+// out[i] = sum_c part[c][i], chunks in order (deterministic).
+__global__ void k_reduce_chunks(const float* __restrict__ part, int nchunk, long n,
+                                float* __restrict__ out) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int c = 0; c < nchunk; c++) s += part[(size_t)c * n + i];
+  out[i] = s;
+}
+
+// ------------------------------------------------------------ elementwise
+// w'[ci][t][co] = w[co][8 - t][ci]: the data-gradient conv's weights.
+__global__ void k_flip_w(const bf16* __restrict__ w, bf16* __restrict__ wt, int Cout, int Cin) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)Cout * 9 * Cin;
+  if (i >= n) return;
+  const int ci = (int)(i % Cin);
+  const int t = (int)((i / Cin) % 9);
+  const int co = (int)(i / (9L * Cin));
+  wt[((size_t)ci * 9 + (8 - t)) * Cout + co] = w[i];
+}
+
+__global__ void k_to_bf16(const float* __restrict__ a, bf16* __restrict__ b, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = (__bf16)a[i];
+}
+
+// g *= (y > 0) (ReLU backward; y is the stored post-activation output).
+__global__ void k_relu_bwd(bf16* __restrict__ g, const bf16* __restrict__ y, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && !((float)y[i] > 0.0f)) g[i] = (__bf16)0.0f;
+}
+
+// 2x2 average pooling, ceil size: out (ceil(H/2), ceil(W/2)).
+__global__ void k_pool(const bf16* __restrict__ in, bf16* __restrict__ out, int B, int H, int W,
+                       int C) {
+  const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+  const long n = (long)B * Ho * Wo * C;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  long q = i / C;
+  const int xo = (int)(q % Wo);
+  q /= Wo;
+  const int yo = (int)(q % Ho);
+  const int b = (int)(q / Ho);
+  float s = 0.0f;
+  int cnt = 0;
+  for (int dy = 0; dy < 2; dy++)
+    for (int dx = 0; dx < 2; dx++) {
+      const int y = 2 * yo + dy, x = 2 * xo + dx;
+      if (y < H && x < W) {
+        s += (float)in[(((size_t)b * H + y) * W + x) * C + c];
+        cnt++;
+      }
+    }
+  out[i] = (__bf16)(s / (float)cnt);
+}
+
+// Gradient of a skip tensor s (full size, C channels): the skip half of the
+// decoder's concat gradient plus the pooling's backward of the gradient of
+// pool(s).  Then ReLU backward against s.
+__global__ void k_skip_grad(const bf16* __restrict__ dcat, int Ccat, const bf16* __restrict__ dpool,
+                            const bf16* __restrict__ s, bf16* __restrict__ ds, int B, int H, int W,
+                            int C) {
+  const long n = (long)B * H * W * C;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  long q = i / C;
+  const int x = (int)(q % W);
+  q /= W;
+  const int y = (int)(q % H);
+  const int b = (int)(q / H);
+  const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+  const int yo = y >> 1, xo = x >> 1;
+  const int cnt = (2 * yo + 1 < H ? 2 : 1) * (2 * xo + 1 < W ? 2 : 1);
+  float v = (float)dcat[(((size_t)b * H + y) * W + x) * Ccat + c] +
+            (float)dpool[(((size_t)b * Ho + yo) * Wo + xo) * C + c] / (float)cnt;
+  if (!((float)s[i] > 0.0f)) v = 0.0f;
+  ds[i] = (__bf16)v;
+}
+
+// Gradient of the coarse (upsampled) input of a decoder conv: sum of the
+// concat gradient's coarse channels over the up-to-4 fine children; then
+// ReLU backward against the coarse tensor y.
+__global__ void k_up_grad(const bf16* __restrict__ dcat, int Ccat, int C0,
+                          const bf16* __restrict__ y, bf16* __restrict__ dy_, int B, int H, int W,
+                          int C1) {
+  const int Hc = (H + 1) / 2, Wc = (W + 1) / 2;
+  const long n = (long)B * Hc * Wc * C1;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % C1);
+  long q = i / C1;
+  const int xc = (int)(q % Wc);
+  q /= Wc;
+  const int yc = (int)(q % Hc);
+  const int b = (int)(q / Hc);
+  float s = 0.0f;
+  for (int dy = 0; dy < 2; dy++)
+    for (int dx = 0; dx < 2; dx++) {
+      const int yy = 2 * yc + dy, xx = 2 * xc + dx;
+      if (yy < H && xx < W) s += (float)dcat[(((size_t)b * H + yy) * W + xx) * Ccat + C0 + c];
+    }
+  if (!((float)y[i] > 0.0f)) s = 0.0f;
+  dy_[i] = (__bf16)s;
+}
+
+// ------------------------------------------------------------ head, loss
+// out[p][k] = b[k] + sum_ci x[p][ci] w[k][ci]   (1x1 conv, FP32 output)
+__global__ void k_head_fwd(const bf16* __restrict__ x, const float* __restrict__ w,
+                           const float* __restrict__ b, float* __restrict__ out, int P, int C,
+                           int K) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  for (int k = 0; k < K; k++) {
+    float s = b[k];
+    for (int c = 0; c < C; c++) s += (float)x[(size_t)p * C + c] * w[k * C + c];
+    out[(size_t)p * K + k] = s;
+  }
+}
+
+// Masked mean-square error: per-workgroup partial sums of mask*(y-t)^2 and
+// dy = 2 mask (y - t) * inv_norm.
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t,
+                                             const float* __restrict__ mask, float* __restrict__ dy,
+                                             float* __restrict__ part, int P, int K, float inv_norm) {
+  __shared__ float red[256];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  float s = 0.0f;
+  if (p < P) {
+    const float m = mask[p];
+    for (int k = 0; k < K; k++) {
+      const float d = m * (y[(size_t)p * K + k] - t[(size_t)p * K + k]);
+      s += d * d;
+      dy[(size_t)p * K + k] = 2.0f * d * inv_norm;
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// dx[p][c] = relu'(x) * sum_k dy[p][k] w[k][c]
+__global__ void k_head_bwd(const float* __restrict__ dy, const float* __restrict__ w,
+                           const bf16* __restrict__ x, bf16* __restrict__ dx, int P, int C, int K) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)P * C) return;
+  const int c = (int)(i % C);
+  const long p = i / C;
+  float s = 0.0f;
+  for (int k = 0; k < K; k++) s += dy[p * K + k] * w[k * C + c];
+  if (!((float)x[i] > 0.0f)) s = 0.0f;
+  dx[i] = (__bf16)s;
+}
+
+// part[chunk][k*C + c] (weights) and part[chunk][K*C + k] (bias) of the head.
+__global__ void k_head_wgrad_part(const float* __restrict__ dy, const bf16* __restrict__ x,
+                                  float* __restrict__ part, int P, int C, int K, int px_per_chunk) {
+  const int p0 = blockIdx.x * px_per_chunk, p1 = min(P, p0 + px_per_chunk);
+  const int nw = K * C + K;
+  for (int e = threadIdx.x; e < nw; e += blockDim.x) {
+    float s = 0.0f;
+    if (e < K * C) {
+      const int k = e / C, c = e % C;
+      for (int p = p0; p < p1; p++) s += dy[(size_t)p * K + k] * (float)x[(size_t)p * C + c];
+    } else {
+      const int k = e - K * C;
+      for (int p = p0; p < p1; p++) s += dy[(size_t)p * K + k];
+    }
+    part[(size_t)blockIdx.x * nw + e] = s;
+  }
+}
+
+// ------------------------------------------------------------ optimiser
+// Adam on the FP32 master copy; bf16 copy of the same element refreshed.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, bf16* __restrict__ pb, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  const float np = p[i] - upd;
+  p[i] = np;
+  pb[i] = (__bf16)np;
+}
+
+// ------------------------------------------------------------ data
+// x[b][y][x][0..Cp) bf16 from samples idx[b] of an FP32 [N][H][W][3]
+// input set (channels >= 3 zero); t[b] and mask[b] from the target set and
+// input channel 0.
+__global__ void k_gather(const float* __restrict__ inputs, const float* __restrict__ targets,
+                         const int* __restrict__ idx, bf16* __restrict__ x, float* __restrict__ t,
+                         float* __restrict__ mask, int B, int HW, int Cp) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)B * HW) return;
+  const int b = (int)(i / HW);
+  const long q = i % HW;
+  const size_t src = ((size_t)idx[b] * HW + q) * 3;
+  for (int c = 0; c < Cp; c++) x[i * Cp + c] = (__bf16)(c < 3 ? inputs[src + c] : 0.0f);
+  if (t) {
+    for (int c = 0; c < 3; c++) t[i * 3 + c] = targets[src + c];
+    mask[i] = inputs[src];
+  }
+}
+
+// ------------------------------------------------------------ launchers
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+static inline const bf16* cb(const u16* p) { return reinterpret_cast<const bf16*>(p); }
+static inline bf16* mb(u16* p) { return reinterpret_cast<bf16*>(p); }
+static inline Src src_of(const ConvSrc& c) { return Src{cb(c.in0), c.C0, cb(c.in1), c.C1, c.B, c.H, c.W}; }
+
+hipError_t conv3x3(const ConvSrc& cs, const u16* w, const float* bias, u16* out, int Cout,
+                   bool relu, hipStream_t st) {
+  const Src s = src_of(cs);
+  const long P = (long)cs.B * cs.H * cs.W;
+  if (Cout >= 64) {
+    dim3 g((unsigned)((P + 63) / 64), (unsigned)((Cout + 63) / 64));
+    if (relu) k_conv3x3<2, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+    else k_conv3x3<2, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+  } else {
+    dim3 g((unsigned)((P + 127) / 128), (unsigned)((Cout + 31) / 32));
+    if (relu) k_conv3x3<1, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+    else k_conv3x3<1, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+  }
+  return hipGetLastError();
+}
+
+int wgrad_chunks(const ConvSrc& cs, int seg_per_chunk) {
+  const long nseg = (long)cs.B * cs.H * ((cs.W + 31) / 32);
+  return (int)((nseg + seg_per_chunk - 1) / seg_per_chunk);
+}
+
+hipError_t wgrad(const ConvSrc& cs, const u16* dout, int Cout, float* part, int seg_per_chunk,
+                 hipStream_t st) {
+  const int Cin = cs.C0 + cs.C1;
+  dim3 g((unsigned)wgrad_chunks(cs, seg_per_chunk), (unsigned)((Cout + 31) / 32),
+         (unsigned)(9 * (Cin / 32)));
+  k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, seg_per_chunk);
+  return hipGetLastError();
+}
+
+hipError_t colsum_part(const u16* g, int P, int C, int px_per_chunk, float* part, hipStream_t st) {
+  k_colsum_part<<<(unsigned)((P + px_per_chunk - 1) / px_per_chunk), 256, 0, st>>>(
+      cb(g), P, C, px_per_chunk, part);
+  return hipGetLastError();
+}
+
+hipError_t reduce_chunks(const float* part, int nchunk, long n, float* out, hipStream_t st) {
+  k_reduce_chunks<<<nblk(n), 256, 0, st>>>(part, nchunk, n, out);
+  return hipGetLastError();
+}
+
+hipError_t flip_w(const u16* w, u16* wt, int Cout, int Cin, hipStream_t st) {
+  k_flip_w<<<nblk((long)Cout * 9 * Cin), 256, 0, st>>>(cb(w), mb(wt), Cout, Cin);
+  return hipGetLastError();
+}
+
+hipError_t to_bf16(const float* a, u16* b, long n, hipStream_t st) {
+  k_to_bf16<<<nblk(n), 256, 0, st>>>(a, mb(b), n);
+  return hipGetLastError();
+}
+
+hipError_t relu_bwd(u16* g, const u16* y, long n, hipStream_t st) {
+  k_relu_bwd<<<nblk(n), 256, 0, st>>>(mb(g), cb(y), n);
+  return hipGetLastError();
+}
+
+hipError_t pool(const u16* in, u16* out, int B, int H, int W, int C, hipStream_t st) {
+  k_pool<<<nblk((long)B * ((H + 1) / 2) * ((W + 1) / 2) * C), 256, 0, st>>>(cb(in), mb(out), B, H,
+                                                                            W, C);
+  return hipGetLastError();
+}
+
+hipError_t skip_grad(const u16* dcat, int Ccat, const u16* dpool, const u16* s, u16* ds, int B,
+                     int H, int W, int C, hipStream_t st) {
+  k_skip_grad<<<nblk((long)B * H * W * C), 256, 0, st>>>(cb(dcat), Ccat, cb(dpool), cb(s), mb(ds),
+                                                         B, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t up_grad(const u16* dcat, int Ccat, int C0, const u16* y, u16* dy, int B, int H, int W,
+                   int C1, hipStream_t st) {
+  k_up_grad<<<nblk((long)B * ((H + 1) / 2) * ((W + 1) / 2) * C1), 256, 0, st>>>(
+      cb(dcat), Ccat, C0, cb(y), mb(dy), B, H, W, C1);
+  return hipGetLastError();
+}
+
+hipError_t head_fwd(const u16* x, const float* w, const float* b, float* out, int P, int C, int K,
+                    hipStream_t st) {
+  k_head_fwd<<<nblk(P), 256, 0, st>>>(cb(x), w, b, out, P, C, K);
+  return hipGetLastError();
+}
+
+hipError_t mse(const float* y, const float* t, const float* mask, float* dy, float* part, int P,
+               int K, float inv_norm, hipStream_t st) {
+  k_mse<<<nblk(P), 256, 0, st>>>(y, t, mask, dy, part, P, K, inv_norm);
+  return hipGetLastError();
+}
+
+hipError_t head_bwd(const float* dy, const float* w, const u16* x, u16* dx, int P, int C, int K,
+                    hipStream_t st) {
+  k_head_bwd<<<nblk((long)P * C), 256, 0, st>>>(dy, w, cb(x), mb(dx), P, C, K);
+  return hipGetLastError();
+}
+
+hipError_t head_wgrad_part(const float* dy, const u16* x, float* part, int P, int C, int K,
+                           int px_per_chunk, hipStream_t st) {
+  k_head_wgrad_part<<<(unsigned)((P + px_per_chunk - 1) / px_per_chunk), 128, 0, st>>>(
+      dy, cb(x), part, P, C, K, px_per_chunk);
+  return hipGetLastError();
+}
+
+hipError_t adam(float* p, const float* g, float* m, float* v, u16* pb, long n, float lr, float b1,
+                float b2, float eps, float c1, float c2, hipStream_t st) {
+  k_adam<<<nblk(n), 256, 0, st>>>(p, g, m, v, mb(pb), n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t gather(const float* inputs, const float* targets, const int* idx, u16* x, float* t,
+                  float* mask, int B, int HW, int Cp, hipStream_t st) {
+  k_gather<<<nblk((long)B * HW), 256, 0, st>>>(inputs, targets, idx, mb(x), t, mask, B, HW, Cp);
+  return hipGetLastError();
+}
+
+}  // namespace lbmnn

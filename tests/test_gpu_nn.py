@@ -1,0 +1,97 @@
+"""The warm-start network's HIP kernels against the CPU oracle, through the
+C ABI (lbm_net_*).
+
+Bar: BF16 weights/activations with FP32 accumulation against the float32
+oracle on the same FP32 master parameters — relative L2 error of the output
+within 3e-2 and of each layer's gradient within 1e-1 (cosine > 0.99); Adam
+training drives the loss down; save/load round-trips the parameters.
+"""
+import numpy as np
+import pytest
+
+import parallellbmcranfield_amd as lbm
+from oracle import nn_oracle as NN
+
+pytestmark = pytest.mark.gpu
+
+
+def _sample(N, H, W, seed):
+    rng = np.random.default_rng(seed)
+    inp = np.zeros((N, H, W, 3), np.float32)
+    inp[..., 0] = rng.random((N, H, W)) > 0.2
+    inp[..., 1] = rng.random((N, 1, 1))
+    inp[..., 2] = rng.random((N, 1, 1))
+    yy, xx = np.meshgrid(np.linspace(0, 1, H), np.linspace(0, 1, W), indexing="ij")
+    tgt = np.stack([np.sin(3 * xx) * yy, np.cos(2 * yy) * xx, xx * yy], -1)[None].repeat(N, 0)
+    tgt = (tgt * (1 + inp[..., 1:2])).astype(np.float32)
+    return inp, tgt
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("H,W,levels", [(37, 37, 4), (20, 27, 3)])
+def test_forward_and_gradients_match_oracle(H, W, levels):
+    print("net vs oracle", H, W, levels, flush=True)
+    net = lbm.Net(H, W, batch=2, base=32, levels=levels, seed=1)
+    p = net.params()
+    assert p.size == NN.num_params(levels, 32)
+    inp, tgt = _sample(2, H, W, 2)
+    y = net.predict(inp)
+    y_ref = NN.forward(p.astype(np.float32), inp, levels, 32)
+    e = _rel(y, y_ref)
+    print("forward rel L2", e, flush=True)
+    assert e < 3e-2
+    loss, g = net.gradients(inp, tgt)
+    loss_ref, g_ref = NN.loss_and_grad(p.astype(np.float32), inp, tgt, levels, 32)
+    assert abs(loss - loss_ref) <= 3e-2 * loss_ref
+    convs, head = NN.unpack(g, levels, 32)
+    convs_r, head_r = NN.unpack(g_ref, levels, 32)
+    for k, ((w, b), (wr, br)) in enumerate(zip(convs, convs_r)):
+        ew, eb = _rel(w, wr), _rel(b, br)
+        cos = float((w * wr).sum() / (np.linalg.norm(w) * np.linalg.norm(wr) + 1e-30))
+        print(f"conv {k}: dW rel {ew:.3e} cos {cos:.5f}  db rel {eb:.3e}", flush=True)
+        assert ew < 1e-1 and cos > 0.99 and eb < 1e-1, k
+    assert _rel(head[0], head_r[0]) < 5e-2 and _rel(head[1], head_r[1]) < 5e-2
+    net.close()
+
+
+def test_training_reduces_loss_and_save_load(tmp_path):
+    print("net training", flush=True)
+    H = W = 33
+    net = lbm.Net(H, W, batch=4, base=32, levels=4, seed=3)
+    inp, tgt = _sample(8, H, W, 4)
+    net.set_data(inp, tgt)
+    rng = np.random.default_rng(0)
+    losses = [net.train(rng.choice(8, 4, replace=False), 2e-3) for _ in range(120)]
+    print("loss first/last", losses[0], losses[-1], flush=True)
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-10:]) < 0.2 * np.mean(losses[:3])
+    # the reported training loss is the loss of the model before its step
+    l_now, _ = net.gradients(inp[:4], tgt[:4])
+    y = net.predict(inp[:4])
+    m = inp[:4, ..., :1]
+    direct = float(((m * (y - tgt[:4])) ** 2).sum() / (3 * m.sum()))
+    assert abs(l_now - direct) <= 1e-3 * direct
+    net.save(str(tmp_path / "w.bin"))
+    p0 = net.params()
+    other = lbm.Net(H, W, batch=4, base=32, levels=4, seed=9)
+    assert not np.array_equal(other.params(), p0)
+    other.load(str(tmp_path / "w.bin"))
+    assert np.array_equal(other.params(), p0)
+    assert np.array_equal(other.predict(inp[:2]), net.predict(inp[:2]))
+    bad = lbm.Net(H, W, batch=4, base=32, levels=3)
+    with pytest.raises(lbm.LbmError):
+        bad.load(str(tmp_path / "w.bin"))
+
+
+def test_full_size_grid_runs():
+    print("net 257x257", flush=True)
+    net = lbm.Net(257, 257, batch=4, base=32, levels=4, seed=0)
+    assert 1.9e6 < net.num_params < 2.1e6
+    inp, tgt = _sample(4, 257, 257, 5)
+    net.set_data(inp, tgt)
+    l0 = net.train(np.arange(4), 1e-3)
+    y = net.predict(inp)
+    assert y.shape == inp.shape and np.isfinite(y).all() and np.isfinite(l0)
