@@ -203,6 +203,11 @@ typedef struct lbm_instance {
  * counterpart in the reference (its solver runs one lattice per process). */
 int lbm_solver_create_batched(const lbm_case* c, const lbm_options* o, const lbm_instance* inst,
                               int count, lbm_solver** out);
+/* Same, every instance starting from f = feq(rho, u, v) of a caller field
+ * (n*m host arrays over the packed lattice, as lbm_solver_create_field). */
+int lbm_solver_create_batched_field(const lbm_case* c, const lbm_options* o,
+                                    const lbm_instance* inst, int count, const double* rho,
+                                    const double* u, const double* v, lbm_solver** out);
 /* Per-instance state of a batched solver: conv_iter (iteration at which the
  * instance converged, -1 still running, -2 diverged), conv_seconds (wall time
  * from the first step to the detection), resid (last residual).  Any pointer

@@ -35,7 +35,7 @@ EXPORTS = [
     "lbm_last_error", "lbm_abi_version", "lbm_read_ini", "lbm_options_from_ini",
     "lbm_case_load", "lbm_case_from_arrays", "lbm_case_cavity", "lbm_case_channel",
     "lbm_case_cavity_ex", "lbm_case_pack", "lbm_case_instances",
-    "lbm_solver_create_batched", "lbm_batch_status",
+    "lbm_solver_create_batched", "lbm_solver_create_batched_field", "lbm_batch_status",
     "lbm_case_write", "lbm_case_info_get", "lbm_case_cells", "lbm_case_free",
     "lbm_solver_create", "lbm_solver_create_field", "lbm_solver_destroy", "lbm_solver_step",
     "lbm_solver_step_timed", "lbm_solver_sync", "lbm_solver_iter", "lbm_solver_residuals",
@@ -129,6 +129,8 @@ def lib() -> C.CDLL:
         "lbm_case_instances": (I, [P]),
         "lbm_solver_create_batched": (I, [P, C.POINTER(Options), P, I, C.POINTER(P)]),
         "lbm_batch_status": (I, [P, I, P, P, P]),
+        "lbm_solver_create_batched_field": (I, [P, C.POINTER(Options), P, I, P, P, P,
+                                                 C.POINTER(P)]),
         "lbm_case_write": (I, [P, C.c_char_p, C.c_char_p]),
         "lbm_case_info_get": (I, [P, C.POINTER(CaseInfo)]),
         "lbm_case_cells": (I, [P] * 11),
@@ -316,8 +318,15 @@ class Solver:
         if instances is not None:
             arr = (Instance * len(instances))(*[Instance(float(nu), float(u))
                                                 for nu, u in instances])
-            _check(lib().lbm_solver_create_batched(case.h, C.byref(opts), C.cast(arr, C.c_void_p),
-                                                   len(instances), C.byref(h)))
+            if field is None:
+                _check(lib().lbm_solver_create_batched(case.h, C.byref(opts),
+                                                       C.cast(arr, C.c_void_p), len(instances),
+                                                       C.byref(h)))
+            else:
+                rho, u, v = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in field)
+                _check(lib().lbm_solver_create_batched_field(
+                    case.h, C.byref(opts), C.cast(arr, C.c_void_p), len(instances), _ptr(rho),
+                    _ptr(u), _ptr(v), C.byref(h)))
         elif field is None:
             _check(lib().lbm_solver_create(case.h, C.byref(opts), C.byref(h)))
         else:

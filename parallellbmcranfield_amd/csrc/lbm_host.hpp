@@ -117,6 +117,11 @@ struct lbm_solver {
   uint8_t* d_inst_active = nullptr;
   int* d_inst_rect = nullptr;        // storage rectangles [ninst][4]
   double* d_inst_res = nullptr;      // [ninst][4]
+  uint32_t* d_tile_list = nullptr;   // bulk tiles of running instances (padded)
+  int nb_list = 0;
+  std::vector<uint32_t> tile_first;  // tile t's instances: tile_ids[tile_first[t] .. [t+1])
+  std::vector<uint16_t> tile_ids;
+  int list_align = 8;
   std::vector<uint8_t> inst_active;
   std::vector<int> inst_conv_iter;
   std::vector<double> inst_conv_s, inst_res;

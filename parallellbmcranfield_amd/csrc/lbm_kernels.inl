@@ -271,7 +271,8 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
   unsigned b = blockIdx.x;
   if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x);
   else if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
-  const int tf = tflag[b];  // uniform per workgroup
+  if (p.tile_list) b = p.tile_list[b];
+  const int tf = b == 0xffffffffu ? 0 : tflag[b];  // uniform per workgroup
   if constexpr (MODE != 1) {
     if (tf == 0) return;
   }
@@ -357,7 +358,8 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
     unsigned b = blockIdx.x - nbb;
     if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x - nbb);
     else if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
-    const int tf = tflag[b];
+    if (p.tile_list) b = p.tile_list[b];
+    const int tf = b == 0xffffffffu ? 0 : tflag[b];
     if constexpr (MODE != 1) {
       if (tf == 0) return;
     }

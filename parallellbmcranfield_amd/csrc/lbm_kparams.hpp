@@ -45,6 +45,8 @@ struct KParams {
   const uint16_t* tile_inst;    // instance of each all-interior bulk tile
   const uint16_t* node_inst;    // instance of each padded cell (0xffff: none)
   const uint8_t* inst_active;   // 0: converged, frozen
+  const uint32_t* tile_list;    // bulk tiles to run, in launch order (null: all);
+                                // 0xffffffff pads the list to the XCD grouping
 };
 
 // Boundary-node record (host builds, kernel reads).  One per fluid node that

@@ -81,6 +81,10 @@ Launch make_launch(lbm_solver* s, int src_buf) {
   L.nb_bulk = s->nb_bulk;
   L.block = s->opt.block_size > 0 ? s->opt.block_size : 256;
   L.kp = s->kp;
+  if (s->d_tile_list) {  // batched: only the tiles of running instances
+    L.nb_bulk = s->nb_list;
+    L.kp.tile_list = s->d_tile_list;
+  }
   L.out.rho = s->d_rho;
   L.out.u = s->d_u;
   L.out.v = s->d_v;
@@ -170,6 +174,8 @@ int observe(lbm_solver* s, bool with_f) {
   Launch L = make_launch(s, s->cur ^ 1);  // previous buffer holds METAF_t
   L.dst = nullptr;
   L.kp.inst_active = nullptr;  // observe converged instances too
+  L.kp.tile_list = nullptr;
+  L.nb_bulk = s->nb_bulk;
   if (with_f) {
     L.out.f = s->d_f;
     L.out.metaf = s->d_metaf;
@@ -190,6 +196,10 @@ void depad(const lbm_solver* s, const T* padded, T* out, long plane_off) {
 
 }  // namespace
 
+extern "C" {
+static int rebuild_tile_list(lbm_solver* s);
+}
+
 static int solver_create(const lbm_case* c, const lbm_options* o, const double* frho,
                          const double* fu, const double* fv, const lbm_instance* inst,
                          int ninst, lbm_solver** out) {
@@ -203,6 +213,10 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   }
   if (ninst > 0 && o->precision == LBM_PREC_F64) {
     set_error("batched instances use FP32 or FP16 storage");
+    return LBM_EINVAL;
+  }
+  if (ninst > 0 && o->force_id != 0) {
+    set_error("force sampling is not offered on batched solvers");
     return LBM_EINVAL;
   }
   if (o->collision < 1 || o->collision > 3) {
@@ -450,6 +464,23 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
       srect[4 * k + 2] = c->inst_rect[4 * k + 2];
       srect[4 * k + 3] = c->inst_rect[4 * k + 3];
     }
+    // instances present in each tile (class-1 cells), for the tile list
+    s->tile_first.assign((size_t)s->nb_bulk + 1, 0);
+    s->tile_ids.clear();
+    for (int t = 0; t < s->nb_bulk; t++) {
+      s->tile_first[t] = (uint32_t)s->tile_ids.size();
+      const size_t e0 = s->tile_ids.size();
+      for (int e = 0; e < BS; e++) {
+        const long idx = (long)kp.bulk_begin + (long)t * BS + e;
+        if (idx >= kp.bulk_end || cls[idx] != 1) continue;
+        const uint16_t k = node_inst[idx];
+        bool seen = false;
+        for (size_t q = e0; q < s->tile_ids.size(); q++) seen = seen || s->tile_ids[q] == k;
+        if (!seen) s->tile_ids.push_back(k);
+      }
+    }
+    s->tile_first[s->nb_bulk] = (uint32_t)s->tile_ids.size();
+    s->list_align = align;
     s->inst_active.assign((size_t)ninst, 1);
     s->inst_conv_iter.assign((size_t)ninst, -1);
     s->inst_conv_s.assign((size_t)ninst, -1.0);
@@ -488,6 +519,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     if (!rc) rc = dev_alloc(s, (void**)&s->d_inst_active, (size_t)ninst);
     if (!rc) rc = dev_alloc(s, (void**)&s->d_inst_rect, sizeof(int) * srect.size());
     if (!rc) rc = dev_alloc(s, (void**)&s->d_inst_res, sizeof(double) * 4 * (size_t)ninst);
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_tile_list, sizeof(uint32_t) * ((size_t)s->nb_bulk + align));
     if (!rc) rc = check_hip(hipMemcpy(s->d_inst_par, inst_par.data(), sizeof(float) * inst_par.size(), hipMemcpyHostToDevice), "upload");
     if (!rc) rc = check_hip(hipMemcpy(s->d_node_inst, node_inst.data(), sizeof(uint16_t) * node_inst.size(), hipMemcpyHostToDevice), "upload");
     if (!rc) rc = check_hip(hipMemcpy(s->d_tile_inst, tile_inst.data(), sizeof(uint16_t) * tile_inst.size(), hipMemcpyHostToDevice), "upload");
@@ -563,6 +595,13 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     return rc;
   }
   s->kp = kp;
+  if (ninst > 0) {
+    rc = rebuild_tile_list(s);
+    if (rc) {
+      lbm_solver_destroy(s);
+      return rc;
+    }
+  }
   // ---- fill both buffers with the solid value, then the first collision
   Launch L = make_launch(s, 1);  // dst = buf[0]
   hipError_t e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[0]) : launch_fill_solid_lp(s->prec, L, s->buf[0]);
@@ -636,10 +675,19 @@ int lbm_solver_create_batched(const lbm_case* c, const lbm_options* o, const lbm
                               int count, lbm_solver** out) {
   if (!c || !o || !inst || count <= 0) return LBM_EINVAL;
   if (o->init == LBM_INIT_FIELD) {
-    set_error("batched solvers start from the reference or rest state");
+    set_error("LBM_INIT_FIELD needs rho/u/v arrays (lbm_solver_create_batched_field)");
     return LBM_EINVAL;
   }
   return solver_create(c, o, nullptr, nullptr, nullptr, inst, count, out);
+}
+
+int lbm_solver_create_batched_field(const lbm_case* c, const lbm_options* o,
+                                    const lbm_instance* inst, int count, const double* rho,
+                                    const double* u, const double* v, lbm_solver** out) {
+  if (!c || !o || !inst || count <= 0 || !rho || !u || !v) return LBM_EINVAL;
+  lbm_options oo = *o;
+  oo.init = LBM_INIT_FIELD;
+  return solver_create(c, &oo, rho, u, v, inst, count, out);
 }
 
 void lbm_solver_destroy(lbm_solver* s) {
@@ -648,7 +696,7 @@ void lbm_solver_destroy(lbm_solver* s) {
   void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_tflag, s->d_bn, s->d_on, s->d_orow, s->d_side,
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
                   s->d_part, s->d_hist, s->d_inst_par, s->d_tile_inst, s->d_node_inst,
-                  s->d_inst_active, s->d_inst_rect, s->d_inst_res};
+                  s->d_inst_active, s->d_inst_rect, s->d_inst_res, s->d_tile_list};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -660,6 +708,34 @@ void lbm_solver_destroy(lbm_solver* s) {
 
 static double wall_seconds() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Tile list of a batched solver: the bulk tiles that hold a node of a running
+// instance, padded with 0xffffffff to a multiple of the XCD grouping so the
+// grouped launch order stays a permutation.  Captured step graphs are dropped
+// (their grid size changes).
+static int rebuild_tile_list(lbm_solver* s) {
+  std::vector<uint32_t> list;
+  list.reserve((size_t)s->nb_bulk);
+  for (int t = 0; t < s->nb_bulk; t++) {
+    bool run = false;
+    for (uint32_t e = s->tile_first[t]; e < s->tile_first[t + 1] && !run; e++)
+      run = s->inst_active[s->tile_ids[e]] != 0;
+    if (run) list.push_back((uint32_t)t);
+  }
+  const size_t a = (size_t)s->list_align;
+  while (list.size() % a) list.push_back(0xffffffffu);
+  s->nb_list = (int)list.size();
+  if (!list.empty())
+    HC(hipMemcpyAsync(s->d_tile_list, list.data(), list.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice, s->stream));
+  HC(hipStreamSynchronize(s->stream));
+  for (hipGraphExec_t& g : s->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  return LBM_OK;
 }
 
 // Per-instance velocity residuals of a batched solver after a residual step:
@@ -713,6 +789,8 @@ static int batch_eval(lbm_solver* s) {
     HC(hipMemcpyAsync(s->d_inst_active, s->inst_active.data(), (size_t)ni, hipMemcpyHostToDevice,
                       s->stream));
     HC(hipStreamSynchronize(s->stream));
+    int rc = rebuild_tile_list(s);
+    if (rc) return rc;
   }
   s->resid[0] = s->resid[1] = worst;
   s->resid[2] = s->resid[3] = 0.0;

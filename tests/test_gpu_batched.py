@@ -147,3 +147,35 @@ def test_batched_rejects_bad_combinations():
     s = lbm.Solver(packed, batch_opts(lbm.PREC_F32), instances=[(0.1, 0.1)] * 2)
     with pytest.raises(lbm.LbmError):
         s.save("/tmp/never.ckpt")
+
+
+def test_batched_field_start_equals_single_field_starts():
+    print("batched field start", flush=True)
+    cases = [lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in SET]
+    packed = lbm.Case.pack(cases)
+    rng = np.random.default_rng(7)
+    fields = []
+    full = [np.ones(packed.n * packed.m), np.zeros(packed.n * packed.m),
+            np.zeros(packed.n * packed.m)]
+    for k, c in enumerate(cases):
+        r = 1.0 + 0.01 * rng.standard_normal(c.n * c.m)
+        u = 0.02 * rng.standard_normal(c.n * c.m)
+        v = 0.02 * rng.standard_normal(c.n * c.m)
+        fields.append((r, u, v))
+        x0, y0, w, h = packed.rects[k]
+        for F, f in zip(full, (r, u, v)):
+            F.reshape(packed.m, packed.n)[y0:y0 + h, x0:x0 + w] = f.reshape(h, w)
+    s = lbm.Solver(packed, batch_opts(lbm.PREC_F32, resid_every=25), field=full,
+                   instances=[(nu, u) for *_, nu, u in SET])
+    s.step(120)
+    rho, u, v = s.macro()
+    for k, (nx, ny, rr, nu, ul) in enumerate(SET):
+        o = lbm.make_options(collision=3, uavg=ul, rho_ini=1.0, viscosity=nu,
+                             precision=lbm.PREC_F32, resid_kind=lbm.RESID_VELOCITY,
+                             resid_every=25)
+        s1 = lbm.Solver(cases[k], o, field=fields[k])
+        s1.step(120)
+        r1, u1, v1 = s1.macro()
+        assert np.array_equal(crop(rho, packed, k), r1), k
+        assert np.array_equal(crop(u, packed, k), u1), k
+        assert np.array_equal(crop(v, packed, k), v1), k
