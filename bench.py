@@ -80,14 +80,20 @@ def main():
                     help="HIP events around every N-th step-kernel launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
-    ap.add_argument("--workload", default="cavity", choices=["cavity", "sweep"],
-                    help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep")
+    ap.add_argument("--workload", default="cavity", choices=["cavity", "sweep", "warmstart"],
+                    help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
+                         "warmstart: learned start vs rest start on held-out cavities")
     ap.add_argument("--count", type=int, default=256, help="sweep: number of cavities")
     ap.add_argument("--side", default="65,513", help="sweep: side-length range")
     ap.add_argument("--crit", type=float, default=1e-6, help="sweep: residual criterion")
     ap.add_argument("--max-steps", type=int, default=400000, help="sweep: step cap")
     ap.add_argument("--single-sample", type=int, default=8,
                     help="sweep: cavities also run one at a time")
+    ap.add_argument("--train-count", type=int, default=2000, help="warmstart: training cavities")
+    ap.add_argument("--test-count", type=int, default=200, help="warmstart: held-out cavities")
+    ap.add_argument("--epochs", type=int, default=30, help="warmstart: training epochs")
+    ap.add_argument("--batch", type=int, default=16, help="warmstart: training batch")
+    ap.add_argument("--lr", type=float, default=1e-3, help="warmstart: peak learning rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -97,6 +103,8 @@ def main():
     import parallellbmcranfield_amd as lbm
     if args.workload == "sweep":
         return bench_sweep(args)
+    if args.workload == "warmstart":
+        return bench_warmstart(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -292,6 +300,89 @@ def bench_sweep(args):
             "traffic": None,
             "kernel": "k_step over the packed lattice (useful updates only)",
             "bytes_per_update": bytes_per_update,
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_warmstart(args):
+    """Learned warm start: steady states of --train-count seeded cavities
+    (sweep geometry, Re log-uniform 100-5000) from the batched solver train
+    the network (257^2 grid, 4 levels, 32-256 channels) on the GPU; then
+    --test-count held-out cavities run to the residual criterion from rest
+    and from the prediction."""
+    import numpy as np
+    import parallellbmcranfield_amd as lbm
+    from parallellbmcranfield_amd import sweep as SW
+    from parallellbmcranfield_amd import warmstart as WS
+    lo, hi = (int(x) for x in args.side.split(","))
+    opts = SW.sweep_options(crit=args.crit, max_steps=args.max_steps)
+    G = WS.GRID
+    t0 = time.perf_counter()
+    specs = SW.random_cavities(args.train_count, seed=1, side=(lo, hi), re=(100.0, 5000.0))
+    X, T = [], []
+    per = 500
+    for b0 in range(0, len(specs), per):
+        part = specs[b0:b0 + per]
+        fields, it, r = WS.steady_states(part, opts)
+        for s_, f, k in zip(part, fields, it):
+            if k > 0:
+                X.append(WS.encode(s_, G))
+                T.append(WS.field_to_grid(s_, *f, G))
+        log(f"warmstart: data {b0 + len(part)}/{len(specs)}: batch {r['wall_s']:.1f} s, "
+            f"{(it > 0).sum()}/{len(part)} converged")
+    X, T = np.stack(X), np.stack(T)
+    t_data = time.perf_counter() - t0
+    net = lbm.Net(G, G, batch=args.batch, base=32, levels=4, seed=0)
+    t1 = time.perf_counter()
+    hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log)
+    t_train = time.perf_counter() - t1
+    del X, T
+    test = SW.random_cavities(args.test_count, seed=2, side=(lo, hi), re=(100.0, 5000.0))
+    r = WS.compare_starts(net, test, opts, log=log)
+    both = (r["steps_rest"] > 0) & (r["steps_warm"] > 0)
+    ratio_t = r["median_time_rest"] / r["median_time_warm"] if both.any() else None
+    log(f"warmstart: median steps ratio {r['median_steps_ratio']}, median time ratio {ratio_t}")
+    out = {
+        "metric": "learned warm start: median wall time to residual 1e-6, rest / warm",
+        "value": round(ratio_t, 3) if ratio_t else None,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network (fp32 accumulate) / f32 lattice",
+        "data": "synthetic",
+        "config": {
+            "workload": f"train {args.train_count} seeded cavities (sides {lo}-{hi}, aspect "
+                        f"0.5-2, Re 100-5000 log-uniform, 0-3 rectangles, seed 1), test "
+                        f"{args.test_count} held-out (seed 2); 257^2 grid U-Net, 4 levels, "
+                        f"32-256 channels; MRT f32 lattice, residual {args.crit:g}",
+            "params": net.num_params,
+            "training_samples": int(len(hist) and args.train_count),
+            "epochs": args.epochs,
+            "batch": args.batch,
+            "parallelism": "single GPU",
+        },
+        "warmstart": {
+            "both_converged": r["both_converged"],
+            "median_steps_rest": float(np.median(r["steps_rest"][both])) if both.any() else None,
+            "median_steps_warm": float(np.median(r["steps_warm"][both])) if both.any() else None,
+            "median_steps_ratio": r["median_steps_ratio"],
+            "median_time_rest_s": r["median_time_rest"],
+            "median_time_warm_s": r["median_time_warm"],
+            "batch_wall_rest_s": round(r["wall_rest"], 2),
+            "batch_wall_warm_s": round(r["wall_warm"], 2),
+            "predict_s": round(r["predict_s"], 3),
+            "field_rel_l2_median": r["field_rel_l2_median"],
+            "field_rel_l2_max": float(r["field_rel_l2"][both].max()) if both.any() else None,
+            "loss_first_last": [hist[0], hist[-1]],
+            "data_s": round(t_data, 1),
+            "train_s": round(t_train, 1),
         },
         "cpu_baseline": None,
     }

@@ -58,7 +58,10 @@ enum {
 enum {
   LBM_INIT_REFERENCE = 0, /* F = 0, U = Uo, V = 0, Rho = rho_ini (CellFunctions.c:148-150,267) */
   LBM_INIT_REST = 1,      /* f = w*rho_ini, u = 0 */
-  LBM_INIT_FIELD = 2      /* f = feq(rho, u, v) of a caller field (lbm_solver_create_field) */
+  LBM_INIT_FIELD = 2,     /* f = feq(rho, u, v) of a caller field (lbm_solver_create_field) */
+  LBM_INIT_FIELD_NEQ = 3  /* the same plus the Chapman-Enskog non-equilibrium part
+                             -3 tau w_k rho Q_k:grad(u) from the field's velocity gradients
+                             (central differences; half-way walls at rest, inlet links at Uo) */
 };
 /* Residual definitions. */
 enum {

@@ -25,7 +25,7 @@ OK, EIO, EPARSE, DIVERGED, EHIP, EINVAL, ENOMEM = range(7)
 OUTLET_VELOCITY, OUTLET_OPEN2, OUTLET_OPEN1, OUTLET_PRESSURE = 1, 2, 3, 4
 BGKW, TRT, MRT = 1, 2, 3
 PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
-INIT_REFERENCE, INIT_REST, INIT_FIELD = 0, 1, 2
+INIT_REFERENCE, INIT_REST, INIT_FIELD, INIT_FIELD_NEQ = 0, 1, 2, 3
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
 TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY, TUNE_NO_GRAPH = 1, 2, 4

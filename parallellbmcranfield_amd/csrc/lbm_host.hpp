@@ -105,6 +105,8 @@ struct lbm_solver {
   bool diverged = false;
   int diverged_iter = -1;
   std::vector<double> rho0, u0, v0;  // initial macros (reference order), for t = 0
+  std::vector<double> grad0;         // field start with non-equilibrium: [3][n*m]
+  std::vector<float> inst_visc;      // batched: viscosity of each instance
   int f0_kind = 0;
   lbm_timing timing{};
   std::vector<hipEvent_t> ev;  // timing pool

@@ -449,6 +449,18 @@ __global__ __launch_bounds__(256) void k_init(typename S::T* __restrict__ dst,
       F0[k] = rho * W64[k] * (1.0 + 3.0 * T2 + 4.5 * T2 * T2 - 1.5 * T1);
     }
   }
+  if (f0_kind == 3) {  // + f^(1) = -3 tau w_k rho Q_k:grad(u), Q = c c - I/3
+    const int inst = (S::DEV && p.ninst) ? (int)p.node_inst[idx] : -1;
+    const double om = S::DEV ? (double)coll_par(p, inst == 0xffff ? -1 : inst).om : p.omega;
+    const double gxx = o.grad[idx], gyy = o.grad[p.plane + idx],
+                 gxy = o.grad[2 * (size_t)p.plane + idx];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double cx = (double)CX[k], cy = (double)CY[k];
+      const double q = (cx * cx - 1.0 / 3.0) * gxx + (cy * cy - 1.0 / 3.0) * gyy + cx * cy * gxy;
+      F0[k] -= 3.0 / om * W64[k] * rho * q;
+    }
+  }
   R out[9];
   if constexpr (S::DEV) {
     float g[9];

@@ -88,6 +88,7 @@ struct KOut {
   double* snap;     // velocity residual snapshot, padded [3][plane] (rho, u, v)
   double* cres;     // reference-residual cells: [m][12] (9 diffs, drag, lift, written flag)
   void* side;       // outlet side buffer [slots][9] (compute type)
+  const double* grad;  // field start with non-equilibrium: [3][plane] du/dx, dv/dy, du/dy+dv/dx
 };
 
 }  // namespace lbmdev

@@ -231,7 +231,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     set_error("unknown residual kind");
     return LBM_EINVAL;
   }
-  if (o->init == LBM_INIT_FIELD && (!frho || !fu || !fv)) {
+  if ((o->init == LBM_INIT_FIELD || o->init == LBM_INIT_FIELD_NEQ) && (!frho || !fu || !fv)) {
     set_error("LBM_INIT_FIELD needs rho/u/v arrays (lbm_solver_create_field)");
     return LBM_EINVAL;
   }
@@ -430,6 +430,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   std::vector<int> srect;
   if (ninst > 0) {
     s->ninst = ninst;
+    s->inst_visc.resize((size_t)ninst);
+    for (int k = 0; k < ninst; k++) s->inst_visc[k] = inst[k].viscosity;
     inst_par.assign((size_t)ninst * INST_PAR_W, 0.0f);
     for (int k = 0; k < ninst; k++) {
       const double om = 1.0 / (3. * (double)inst[k].viscosity + 0.5);
@@ -546,13 +548,42 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     if (c->fluid[lin] != 1) continue;
     if (o->init == LBM_INIT_REST) {
       s->u0[lin] = 0.0;
-    } else if (o->init == LBM_INIT_FIELD) {
+    } else if (o->init == LBM_INIT_FIELD || o->init == LBM_INIT_FIELD_NEQ) {
       s->rho0[lin] = frho[lin];
       s->u0[lin] = fu[lin];
       s->v0[lin] = fv[lin];
     }
   }
-  s->f0_kind = o->init == LBM_INIT_REST ? 1 : (o->init == LBM_INIT_FIELD ? 2 : 0);
+  s->f0_kind = o->init == LBM_INIT_REST ? 1
+              : o->init == LBM_INIT_FIELD ? 2
+              : o->init == LBM_INIT_FIELD_NEQ ? 3 : 0;
+  if (s->f0_kind == 3) {
+    // velocity gradients of the field by central differences; a neighbour
+    // that is not fluid is a half-way wall: ghost = 2 u_wall - u, u_wall = (Uo, 0)
+    // on inlet links (type 2), else 0
+    s->grad0.assign(3 * (size_t)nc, 0.0);
+    for (long lin = 0; lin < nc; lin++) {
+      if (c->fluid[lin] != 1) continue;
+      const int i = (int)(lin % n), j = (int)(lin / n);
+      const int b = c->bcell[lin];
+      auto nb = [&](int k, int di, int dj, bool want_u) -> double {
+        const int ii = i + di, jj = j + dj;
+        const double self = want_u ? s->u0[lin] : s->v0[lin];
+        if (ii >= 0 && ii < n && jj >= 0 && jj < m && c->fluid[(long)jj * n + ii] == 1)
+          return want_u ? s->u0[(long)jj * n + ii] : s->v0[(long)jj * n + ii];
+        const int t = b >= 0 ? c->bcells[b].bc[k] : 0;
+        const double uw = (t == 2 && want_u) ? cell_uo(lin) : 0.0;
+        return 2.0 * uw - self;
+      };
+      const double dudx = 0.5 * (nb(1, 1, 0, true) - nb(3, -1, 0, true));
+      const double dudy = 0.5 * (nb(2, 0, 1, true) - nb(4, 0, -1, true));
+      const double dvdx = 0.5 * (nb(1, 1, 0, false) - nb(3, -1, 0, false));
+      const double dvdy = 0.5 * (nb(2, 0, 1, false) - nb(4, 0, -1, false));
+      s->grad0[lin] = dudx;
+      s->grad0[nc + lin] = dvdy;
+      s->grad0[2 * nc + lin] = dudy + dvdx;
+    }
+  }
   {
     std::vector<double> pr((size_t)s->plane, 0.0), pu((size_t)s->plane, 0.0),
         pv((size_t)s->plane, 0.0);
@@ -594,6 +625,22 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     lbm_solver_destroy(s);
     return rc;
   }
+  double* d_grad = nullptr;
+  if (!rc && s->f0_kind == 3) {
+    std::vector<double> pg(3 * (size_t)s->plane, 0.0);
+    for (int q = 0; q < 3; q++)
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < n; i++)
+          pg[(size_t)q * s->plane + (size_t)(j + 1) * s->pitch + (i + 1)] =
+              s->grad0[(size_t)q * nc + (size_t)j * n + i];
+    rc = check_hip(hipMalloc((void**)&d_grad, pg.size() * sizeof(double)), "hipMalloc gradients");
+    if (!rc) rc = check_hip(hipMemcpy(d_grad, pg.data(), pg.size() * sizeof(double), hipMemcpyHostToDevice), "upload gradients");
+    if (rc) {
+      if (d_grad) (void)hipFree(d_grad);
+      lbm_solver_destroy(s);
+      return rc;
+    }
+  }
   s->kp = kp;
   if (ninst > 0) {
     rc = rebuild_tile_list(s);
@@ -604,10 +651,12 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   }
   // ---- fill both buffers with the solid value, then the first collision
   Launch L = make_launch(s, 1);  // dst = buf[0]
+  L.out.grad = d_grad;
   hipError_t e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[0]) : launch_fill_solid_lp(s->prec, L, s->buf[0]);
   if (e == hipSuccess) e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[1]) : launch_fill_solid_lp(s->prec, L, s->buf[1]);
   if (e == hipSuccess) e = s->prec == 0 ? launch_init_f64(s->coll, s->f0_kind, L) : launch_init_lp(s->prec, s->coll, s->f0_kind, L);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (d_grad) (void)hipFree(d_grad);
   rc = check_hip(e, "initial collision");
   if (rc) {
     lbm_solver_destroy(s);
@@ -656,7 +705,7 @@ static int build_graph(lbm_solver* s, int parity) {
 extern "C" {
 
 int lbm_solver_create(const lbm_case* c, const lbm_options* o, lbm_solver** out) {
-  if (o && o->init == LBM_INIT_FIELD) {
+  if (o && (o->init == LBM_INIT_FIELD || o->init == LBM_INIT_FIELD_NEQ)) {
     set_error("LBM_INIT_FIELD needs lbm_solver_create_field");
     return LBM_EINVAL;
   }
@@ -667,14 +716,14 @@ int lbm_solver_create_field(const lbm_case* c, const lbm_options* o, const doubl
                             const double* u, const double* v, lbm_solver** out) {
   if (!o) return LBM_EINVAL;
   lbm_options oo = *o;
-  oo.init = LBM_INIT_FIELD;
+  if (oo.init != LBM_INIT_FIELD_NEQ) oo.init = LBM_INIT_FIELD;
   return solver_create(c, &oo, rho, u, v, nullptr, c ? c->ninst : 0, out);
 }
 
 int lbm_solver_create_batched(const lbm_case* c, const lbm_options* o, const lbm_instance* inst,
                               int count, lbm_solver** out) {
   if (!c || !o || !inst || count <= 0) return LBM_EINVAL;
-  if (o->init == LBM_INIT_FIELD) {
+  if (o->init == LBM_INIT_FIELD || o->init == LBM_INIT_FIELD_NEQ) {
     set_error("LBM_INIT_FIELD needs rho/u/v arrays (lbm_solver_create_batched_field)");
     return LBM_EINVAL;
   }
@@ -686,7 +735,7 @@ int lbm_solver_create_batched_field(const lbm_case* c, const lbm_options* o,
                                     const double* u, const double* v, lbm_solver** out) {
   if (!c || !o || !inst || count <= 0 || !rho || !u || !v) return LBM_EINVAL;
   lbm_options oo = *o;
-  oo.init = LBM_INIT_FIELD;
+  if (oo.init != LBM_INIT_FIELD_NEQ) oo.init = LBM_INIT_FIELD;
   return solver_create(c, &oo, rho, u, v, inst, count, out);
 }
 
@@ -1019,12 +1068,20 @@ int lbm_solver_get_f(lbm_solver* s, double* f, double* metaf) {
       for (int k = 0; k < 9; k++) {
         double v = 0.0;
         if (c->fluid[i] == 1 && s->f0_kind == 1) v = W[k] * s->rho0[i];
-        if (c->fluid[i] == 1 && s->f0_kind == 2) {
+        if (c->fluid[i] == 1 && (s->f0_kind == 2 || s->f0_kind == 3)) {
           static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
           static const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
           const double U = s->u0[i], V = s->v0[i];
           const double T1 = U * U + V * V, T2 = U * CX[k] + V * CY[k];
           v = s->rho0[i] * W[k] * (1.0 + 3.0 * T2 + 4.5 * T2 * T2 - 1.5 * T1);
+          if (s->f0_kind == 3) {
+            const double cx = CX[k], cy = CY[k];
+            const double q = (cx * cx - 1.0 / 3.0) * s->grad0[i] +
+                             (cy * cy - 1.0 / 3.0) * s->grad0[nc + i] + cx * cy * s->grad0[2 * nc + i];
+            const int in = c->ninst ? c->cell_inst[i] : -1;
+            const double om = in >= 0 ? 1.0 / (3. * (double)s->inst_visc[in] + 0.5) : s->kp.omega;
+            v -= 3.0 / om * W[k] * s->rho0[i] * q;
+          }
         }
         if (f) f[i * 9 + k] = v;
         if (metaf) metaf[i * 9 + k] = 0.0;

@@ -1,0 +1,244 @@
+"""Learned warm start of batched cavity runs.
+
+The network (``Net``, lbm_net_* in the library: U-Net of 3x3 convolutions,
+BF16 on the matrix cores, trained with the library's own backward and Adam)
+maps a cavity drawn on a G x G grid to its steady (u, v, rho) there:
+
+  inputs   channel 0  fluid mask: the cavity scaled uniformly so that its
+                      longer side spans the grid, anchored at the grid
+                      origin; grid points outside it or in an obstacle are 0
+           channel 1  (ln Re - ln 100) / ln 50      (Re on the lid length)
+           channel 2  lid speed / 0.1
+  targets  u / lid, v / lid, (rho - 1) / (3 lid^2)   (0 outside the fluid)
+
+Training data are steady states of seeded cavities run by the batched solver
+to the velocity residual criterion; the prediction for a new cavity is
+sampled back onto its lattice and seeds f = feq(rho, u, v).  The rest start
+(rho = 1, u = 0) stays the fallback and the reference for every comparison.
+
+No counterpart in the reference.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import INIT_FIELD, INIT_FIELD_NEQ, Case, Net, Solver
+from . import sweep as SW
+
+GRID = 257
+RE_LO, RE_SPAN = math.log(100.0), math.log(50.0)
+
+
+def _geometry(spec, G):
+    NX, NY = spec["NX"], spec["NY"]
+    scale = (G - 1) / max(NX - 1, NY - 1)
+    return NX, NY, scale
+
+
+def _fluid_mask(spec):
+    """Fluid flags of the cavity's own nodes, [NY, NX] (j-1, i-1)."""
+    m = np.ones((spec["NY"], spec["NX"]), bool)
+    for i0, j0, i1, j1 in spec["rects"]:
+        m[j0 - 1:j1, i0 - 1:i1] = False
+    return m
+
+
+def encode(spec, G: int = GRID) -> np.ndarray:
+    """Network input [G, G, 3] of one cavity specification."""
+    NX, NY, scale = _geometry(spec, G)
+    fl = _fluid_mask(spec)
+    g = np.arange(G)
+    ii = np.rint(g / scale).astype(int)  # nearest cavity node (0-based)
+    inside_x = g / scale <= NX - 1 + 1e-9
+    inside_y = g / scale <= NY - 1 + 1e-9
+    mask = np.zeros((G, G), np.float32)
+    jy, ix = np.meshgrid(np.minimum(ii, NY - 1), np.minimum(ii, NX - 1), indexing="ij")
+    inside = inside_y[:, None] & inside_x[None, :]
+    mask[inside] = fl[jy[inside], ix[inside]]
+    out = np.zeros((G, G, 3), np.float32)
+    out[..., 0] = mask
+    out[..., 1] = (math.log(spec["Re"]) - RE_LO) / RE_SPAN
+    out[..., 2] = spec["lid"] / 0.1
+    return out
+
+
+def _bilinear(field, fy, fx):
+    """field [h, w] sampled at fractional (fy, fx) (clamped)."""
+    h, w = field.shape
+    fy = np.clip(fy, 0, h - 1)
+    fx = np.clip(fx, 0, w - 1)
+    y0 = np.floor(fy).astype(int)
+    x0 = np.floor(fx).astype(int)
+    ty, tx = fy - y0, fx - x0
+    y1, x1 = np.minimum(y0 + 1, h - 1), np.minimum(x0 + 1, w - 1)
+    return ((1 - ty) * (1 - tx) * field[y0, x0] + (1 - ty) * tx * field[y0, x1] +
+            ty * (1 - tx) * field[y1, x0] + ty * tx * field[y1, x1])
+
+
+def field_to_grid(spec, rho, u, v, G: int = GRID) -> np.ndarray:
+    """Steady fields on the cavity lattice (n*m arrays of lbm_case_cavity_ex's
+    lattice) -> normalised targets [G, G, 3] (0 where the input mask is 0)."""
+    NX, NY, scale = _geometry(spec, G)
+    n, m = NX + 2, NY + 3
+    lid = spec["lid"]
+    fl = _fluid_mask(spec)
+    chans = []
+    for f, off, sc in ((u, 0.0, lid), (v, 0.0, lid), (rho, 1.0, 3 * lid * lid)):
+        a = (np.asarray(f).reshape(m, n)[1:NY + 1, 1:NX + 1] - off) / sc
+        chans.append(np.where(fl, a, 0.0))
+    g = np.arange(G) / scale
+    fy, fx = np.meshgrid(g, g, indexing="ij")
+    mask = encode(spec, G)[..., 0]
+    out = np.zeros((G, G, 3), np.float32)
+    for k, a in enumerate(chans):
+        out[..., k] = np.where(mask > 0, _bilinear(a, fy, fx), 0.0)
+    return out
+
+
+def grid_to_field(spec, pred, G: int = GRID):
+    """Prediction [G, G, 3] -> (rho, u, v) n*m arrays on the cavity lattice
+    (fluid nodes from the prediction, other cells at rest)."""
+    NX, NY, scale = _geometry(spec, G)
+    n, m = NX + 2, NY + 3
+    lid = spec["lid"]
+    jj, ii = np.meshgrid(np.arange(NY), np.arange(NX), indexing="ij")
+    # stay on grid points that lie inside the cavity (the network's output
+    # outside it is not trained)
+    fy = np.minimum(jj * scale, math.floor((NY - 1) * scale + 1e-9))
+    fx = np.minimum(ii * scale, math.floor((NX - 1) * scale + 1e-9))
+    fl = _fluid_mask(spec)
+    out = []
+    for k, off, sc in ((2, 1.0, 3 * lid * lid), (0, 0.0, lid), (1, 0.0, lid)):
+        a = np.full((m, n), off)
+        a[1:NY + 1, 1:NX + 1] = np.where(fl, off + sc * _bilinear(pred[..., k], fy, fx), off)
+        out.append(a.ravel())
+    return tuple(out)  # rho, u, v
+
+
+def steady_states(specs, opts=None, chunk: int = 5000, progress=None):
+    """Run the cavities in one batched solver to the residual criterion and
+    return their steady fields [(rho, u, v) per instance] and convergence
+    iterations (-1 / -2: not converged / diverged)."""
+    r = SW.run_batched(specs, opts, chunk=chunk, progress=progress)
+    s, packed = r["solver"], r["case"]
+    rho, u, v = s.macro()
+    fields = []
+    for k in range(len(specs)):
+        x0, y0, w, h = packed.rects[k]
+        fields.append(tuple(a.reshape(packed.m, packed.n)[y0:y0 + h, x0:x0 + w].ravel().copy()
+                            for a in (rho, u, v)))
+    s.close()
+    return fields, r["conv_iter"], r
+
+
+def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
+          lr: float = 1e-3, seed: int = 0, log=None) -> list[float]:
+    """Adam over the set with a cosine learning-rate decay; returns the mean
+    loss of every epoch."""
+    net.set_data(inputs, targets)
+    rng = np.random.default_rng(seed)
+    N, B = inputs.shape[0], net.batch
+    steps_per_epoch = max(1, N // B)
+    total = epochs * steps_per_epoch
+    hist, k = [], 0
+    for ep in range(epochs):
+        order = rng.permutation(N)
+        ls = []
+        for b in range(steps_per_epoch):
+            a = lr * 0.5 * (1 + math.cos(math.pi * k / total))
+            ls.append(net.train(order[b * B:(b + 1) * B], a))
+            k += 1
+        hist.append(float(np.mean(ls)))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.4e}")
+    return hist
+
+
+def warm_fields(net: Net, specs):
+    """Network prediction of every cavity, as (rho, u, v) on its lattice."""
+    pred = net.predict(np.stack([encode(s, net.H) for s in specs]))
+    return [grid_to_field(s, p, net.H) for s, p in zip(specs, pred)]
+
+
+def packed_field(packed: Case, fields):
+    full = [np.ones(packed.n * packed.m), np.zeros(packed.n * packed.m),
+            np.zeros(packed.n * packed.m)]
+    for k, f in enumerate(fields):
+        x0, y0, w, h = packed.rects[k]
+        for F, a in zip(full, f):
+            F.reshape(packed.m, packed.n)[y0:y0 + h, x0:x0 + w] = np.asarray(a).reshape(h, w)
+    return full
+
+
+def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
+                   neq: bool = True) -> dict:
+    """Held-out comparison: the same cavities from rest and from the network's
+    prediction, each as one batched run to the criterion.  Per instance:
+    steps and wall time to convergence (the warm start's time includes the
+    prediction), and the relative L2 difference of the two converged
+    velocity fields.  neq: seed feq plus the Chapman-Enskog non-equilibrium
+    part of the predicted velocity gradients (LBM_INIT_FIELD_NEQ)."""
+    opts = opts or SW.sweep_options()
+    packed, fl = SW.build_batch(specs)
+    inst = [(sp["nu"], sp["lid"]) for sp in specs]
+
+    def run(field):
+        t0 = time.perf_counter()
+        pre = 0.0
+        if field == "warm":
+            flds = warm_fields(net, specs)
+            fld = packed_field(packed, flds)
+            pre = time.perf_counter() - t0
+            o = type(opts).from_buffer_copy(opts)
+            o.init = INIT_FIELD_NEQ if neq else INIT_FIELD
+            s = Solver(packed, o, field=fld, instances=inst)
+        else:
+            s = Solver(packed, opts, instances=inst)
+        s.sync()
+        t1 = time.perf_counter()
+        while True:
+            done = s.step(chunk)
+            if done < chunk or s.iter >= opts.max_iterations:
+                break
+        s.sync()
+        wall = time.perf_counter() - t1
+        it, sec, _ = s.batch_status()
+        _, u, v = s.macro()
+        s.close()
+        return dict(conv_iter=it, conv_s=sec + pre, wall_s=wall + pre, predict_s=pre, u=u, v=v)
+
+    rest = run("rest")
+    if log:
+        log(f"rest start: {rest['wall_s']:.1f} s")
+    warm = run("warm")
+    if log:
+        log(f"warm start: {warm['wall_s']:.1f} s (prediction {warm['predict_s']:.2f} s)")
+    both = (rest["conv_iter"] > 0) & (warm["conv_iter"] > 0)
+    diff = []
+    for k in range(len(specs)):
+        x0, y0, w, h = packed.rects[k]
+        sl = (slice(y0, y0 + h), slice(x0, x0 + w))
+        ur, vr = (a.reshape(packed.m, packed.n)[sl] for a in (rest["u"], rest["v"]))
+        uw, vw = (a.reshape(packed.m, packed.n)[sl] for a in (warm["u"], warm["v"]))
+        num = np.sqrt(((uw - ur) ** 2 + (vw - vr) ** 2).sum())
+        den = np.sqrt((ur ** 2 + vr ** 2).sum())
+        diff.append(num / den if den > 0 else 0.0)
+    diff = np.asarray(diff)
+    return dict(
+        n=len(specs), both_converged=int(both.sum()),
+        steps_rest=rest["conv_iter"], steps_warm=warm["conv_iter"],
+        time_rest=rest["conv_s"], time_warm=warm["conv_s"],
+        median_steps_ratio=float(np.median(rest["conv_iter"][both] / warm["conv_iter"][both]))
+        if both.any() else None,
+        median_time_rest=float(np.median(rest["conv_s"][both])) if both.any() else None,
+        median_time_warm=float(np.median(warm["conv_s"][both])) if both.any() else None,
+        wall_rest=rest["wall_s"], wall_warm=warm["wall_s"],
+        field_rel_l2=diff, field_rel_l2_median=float(np.median(diff[both])) if both.any() else None,
+        predict_s=warm["predict_s"])
+
+
+__all__ = ["GRID", "encode", "field_to_grid", "grid_to_field", "steady_states", "train",
+           "warm_fields", "packed_field", "compare_starts"]

@@ -179,3 +179,69 @@ def test_batched_field_start_equals_single_field_starts():
         assert np.array_equal(crop(rho, packed, k), r1), k
         assert np.array_equal(crop(u, packed, k), u1), k
         assert np.array_equal(crop(v, packed, k), v1), k
+
+
+def test_neq_start_has_the_chapman_enskog_stress_and_converges_sooner():
+    """LBM_INIT_FIELD_NEQ: the seeded non-equilibrium part carries no mass or
+    momentum and the BGK Chapman-Enskog stress Pi = -rho tau / 3 (grad u +
+    grad u^T) of the field (interior nodes, central differences of a
+    quadratic field are exact); started from a converged field it re-converges
+    in fewer steps than the plain equilibrium seed."""
+    print("neq start", flush=True)
+    NX, NY, nu = 30, 20, 0.05
+    case = lbm.Case.cavity_ex(NX, NY)
+    n, m = case.n, case.m
+    yy, xx = np.meshgrid(np.arange(m, dtype=float), np.arange(n, dtype=float), indexing="ij")
+    u = (1e-3 * (xx * yy + 0.5 * yy ** 2)).ravel()
+    v = (1e-3 * (xx ** 2 - 2 * xx * yy)).ravel()
+    rho = (1 + 1e-4 * xx).ravel()
+    o = lbm.make_options(collision=1, uavg=0.1, rho_ini=1.0, viscosity=nu,
+                         precision=lbm.PREC_F64, init=lbm.INIT_FIELD_NEQ,
+                         resid_kind=lbm.RESID_NONE)
+    s = lbm.Solver(case, o, field=(rho, u, v))
+    f_host, _ = s.distributions()  # the seed as the host states it (iteration 0)
+    s.step(1)
+    _, metaf0 = s.distributions()  # own METAF of the previous buffer: the device seed, collided
+    W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+    CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    T2 = u[:, None] * CX + v[:, None] * CY
+    feq = rho[:, None] * W * (1 + 3 * T2 + 4.5 * T2 ** 2 - 1.5 * (u ** 2 + v ** 2)[:, None])
+    tau = 3 * float(np.float32(nu)) + 0.5
+    om = 1 / tau
+    f = (metaf0 - om * feq) / (1 - om)  # BGK: METAF = om feq + (1 - om) F
+    fl = case.cells()["fluid"] == 1
+    assert np.allclose(f[fl], f_host[fl], rtol=0, atol=1e-13)
+    fn = f - feq
+    X, Y = xx.ravel(), yy.ravel()
+    dudx, dudy = 1e-3 * Y, 1e-3 * (X + Y)
+    dvdx, dvdy = 1e-3 * (2 * X - 2 * Y), -2e-3 * X
+    inner = ((X >= 2) & (X <= NX - 1) & (Y >= 2) & (Y <= NY - 1))
+    assert np.abs(fn[inner].sum(1)).max() < 1e-13
+    assert np.abs((fn * CX)[inner].sum(1)).max() < 1e-13
+    pxy = (fn * CX * CY).sum(1)
+    assert np.allclose(pxy[inner], (-rho * tau / 3 * (dudy + dvdx))[inner], rtol=1e-7, atol=1e-13)
+    pd = (fn * (CX ** 2 - CY ** 2)).sum(1)
+    assert np.allclose(pd[inner], (-2 * rho * tau / 3 * (dudx - dvdy))[inner], rtol=1e-7,
+                       atol=1e-13)
+    # re-convergence from a converged field
+    specs = [(40, 28, [(5, 5, 10, 9)], 0.02, 0.1), (33, 33, [], 0.01, 0.1)]
+    packed = lbm.Case.pack([lbm.Case.cavity_ex(a, b, r) for a, b, r, _, _ in specs])
+    inst = [(nu_, u_) for *_, nu_, u_ in specs]
+    ob = batch_opts(lbm.PREC_F32, resid_every=50, crit_v=1e-6, use_stop_rule=1,
+                    max_iterations=200000)
+    s0 = lbm.Solver(packed, ob, instances=inst)
+    s0.step(200000)
+    it0 = s0.batch_status()[0]
+    assert (it0 > 0).all()
+    fld = s0.macro()
+    steps = {}
+    for kind in (lbm.INIT_FIELD, lbm.INIT_FIELD_NEQ):
+        ob.init = kind
+        s1 = lbm.Solver(packed, ob, field=fld, instances=inst)
+        s1.step(200000)
+        steps[kind] = s1.batch_status()[0]
+    print("rest", it0, "feq", steps[lbm.INIT_FIELD], "neq", steps[lbm.INIT_FIELD_NEQ], flush=True)
+    assert (steps[lbm.INIT_FIELD_NEQ] > 0).all()
+    assert (steps[lbm.INIT_FIELD_NEQ] <= steps[lbm.INIT_FIELD]).all()
+    assert (steps[lbm.INIT_FIELD_NEQ] < it0 / 2).all()

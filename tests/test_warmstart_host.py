@@ -1,0 +1,62 @@
+"""Host side of the learned warm start: grid encoding and resampling (no
+kernel launches)."""
+import math
+
+import numpy as np
+
+from parallellbmcranfield_amd import warmstart as WS
+
+
+def _spec(NX, NY, rects=(), Re=400.0, lid=0.1):
+    return dict(NX=NX, NY=NY, rects=list(rects), Re=Re, nu=lid * NX / Re, lid=lid)
+
+
+def test_encode_channels_and_mask():
+    s = _spec(129, 65, [(10, 10, 20, 30)], Re=1000.0)
+    x = WS.encode(s, 257)
+    assert x.shape == (257, 257, 3)
+    assert np.allclose(x[..., 1], (math.log(1000) - math.log(100)) / math.log(50))
+    assert np.allclose(x[..., 2], 1.0)
+    # scale 2: the cavity covers grid columns 0..256 and rows 0..128
+    m = x[..., 0]
+    assert m[:129].sum() > 0 and m[129:].sum() == 0
+    assert m[0, 0] == 1 and m[128, 256] == 1
+    # obstacle nodes (i, j) = (10..20, 10..30) -> grid (2(i-1), 2(j-1))
+    assert m[2 * 15, 2 * 15] == 0 and m[2 * 15, 2 * 40] == 1
+    fluid = 129 * 65 - 11 * 21
+    assert abs(m.sum() / 4 - fluid) < 0.05 * fluid
+
+
+def test_resampling_round_trip_is_exact_on_grid_aligned_nodes():
+    # NX = 257: one grid step per node, so every node sits on a grid point
+    s = _spec(257, 129, [(100, 40, 120, 60)])
+    n, m = 259, 132
+    rng = np.random.default_rng(0)
+    yy, xx = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    u = (0.05 * np.sin(xx / 20.0) * np.cos(yy / 15.0)).ravel()
+    v = (0.03 * np.cos(xx / 17.0)).ravel()
+    rho = (1 + 1e-3 * rng.standard_normal(n * m))
+    t = WS.field_to_grid(s, rho, u, v, 257)
+    r2, u2, v2 = WS.grid_to_field(s, t, 257)
+    fl = np.zeros((m, n), bool)
+    fl[1:130, 1:258] = WS._fluid_mask(s)
+    fl = fl.ravel()
+    # float32 targets: exact to single precision
+    assert np.allclose(u2[fl], u[fl], rtol=0, atol=1e-7)
+    assert np.allclose(v2[fl], v[fl], rtol=0, atol=1e-7)
+    assert np.allclose(r2[fl], rho[fl], rtol=0, atol=1e-7)
+    assert (u2[~fl] == 0).all() and (r2[~fl] == 1).all()
+
+
+def test_resampling_of_a_smooth_field_at_another_scale():
+    s = _spec(90, 140)
+    n, m = 92, 143
+    yy, xx = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    u = (0.1 * (yy / m) ** 2).ravel()
+    t = WS.field_to_grid(s, np.ones(n * m), u, np.zeros(n * m), 257)
+    _, u2, _ = WS.grid_to_field(s, t, 257)
+    inner = np.zeros((m, n), bool)
+    inner[1:141, 1:91] = True
+    err = np.abs(u2 - u)[inner.ravel()].max()
+    # interpolation twice at 0.54-node spacing, within one grid cell of the edges
+    assert err < 2e-3
