@@ -80,9 +80,11 @@ def main():
                     help="HIP events around every N-th step-kernel launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
-    ap.add_argument("--workload", default="cavity", choices=["cavity", "sweep", "warmstart"],
+    ap.add_argument("--workload", default="cavity",
+                    choices=["cavity", "sweep", "warmstart", "netstep"],
                     help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
-                         "warmstart: learned start vs rest start on held-out cavities")
+                         "warmstart: learned start vs rest start on held-out cavities; "
+                         "netstep: training steps of the warm-start network")
     ap.add_argument("--count", type=int, default=256, help="sweep: number of cavities")
     ap.add_argument("--side", default="65,513", help="sweep: side-length range")
     ap.add_argument("--crit", type=float, default=1e-6, help="sweep: residual criterion")
@@ -105,6 +107,8 @@ def main():
         return bench_sweep(args)
     if args.workload == "warmstart":
         return bench_warmstart(args)
+    if args.workload == "netstep":
+        return bench_netstep(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -301,6 +305,68 @@ def bench_sweep(args):
             "kernel": "k_step over the packed lattice (useful updates only)",
             "bytes_per_update": bytes_per_update,
         },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def net_flops(H, W, batch, base=32, levels=4):
+    """FLOP of one training step (forward, weight and data gradients; the
+    first conv needs no data gradient) of the warm-start network."""
+    import sys as _s
+    _s.path.insert(0, os.path.join(ROOT))
+    Hl, Wl = [H], [W]
+    for _ in range(levels - 1):
+        Hl.append((Hl[-1] + 1) // 2)
+        Wl.append((Wl[-1] + 1) // 2)
+    C = [base << l for l in range(levels)]
+    convs = []
+    for l in range(levels):
+        convs.append((32 if l == 0 else C[l - 1], C[l], l))
+        convs.append((C[l], C[l], l))
+    for l in range(levels - 2, -1, -1):
+        convs.append((C[l] + C[l + 1], C[l], l))
+        convs.append((C[l], C[l], l))
+    fwd = sum(2 * 9 * ci * co * batch * Hl[l] * Wl[l] for ci, co, l in convs)
+    dgrad = fwd - 2 * 9 * 32 * C[0] * batch * H * W
+    return fwd, fwd + fwd + dgrad
+
+
+def bench_netstep(args):
+    """Adam training steps of the warm-start network on a synthetic batch
+    (257^2 grid, 4 levels, 32-256 channels, batch --batch), timed over
+    --steps after --warmup; FLOP/s of the convolutions against the MI355X
+    dense BF16 matrix peak."""
+    import numpy as np
+    import parallellbmcranfield_amd as lbm
+    G, B = 257, args.batch
+    net = lbm.Net(G, G, batch=B, base=32, levels=4, seed=0)
+    rng = np.random.default_rng(0)
+    inp = np.zeros((B, G, G, 3), np.float32)
+    inp[..., 0] = 1.0
+    inp[..., 1:] = rng.random((B, 1, 1, 2))
+    tgt = rng.standard_normal((B, G, G, 3)).astype(np.float32)
+    net.set_data(inp, tgt)
+    idx = np.arange(B)
+    for _ in range(args.warmup):
+        net.train(idx, 1e-4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.train(idx, 1e-4)
+    dt = (time.perf_counter() - t0) / args.steps
+    fwd, tot = net_flops(G, G, B)
+    tflops = tot / dt / 1e12
+    out = {
+        "metric": "warm-start network training step: convolution TFLOP/s",
+        "value": round(tflops, 2), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": f"U-Net 257x257, 4 levels, 32-256 channels, batch {B}, "
+                               "forward + backward + Adam", "params": net.num_params,
+                   "flop_per_step": tot, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4), "traffic": None},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
