@@ -93,9 +93,11 @@ def main():
                     help="sweep: cavities also run one at a time")
     ap.add_argument("--train-count", type=int, default=2000, help="warmstart: training cavities")
     ap.add_argument("--test-count", type=int, default=200, help="warmstart: held-out cavities")
-    ap.add_argument("--epochs", type=int, default=30, help="warmstart: training epochs")
+    ap.add_argument("--epochs", type=int, default=60, help="warmstart: training epochs")
     ap.add_argument("--batch", type=int, default=16, help="warmstart: training batch")
     ap.add_argument("--lr", type=float, default=1e-3, help="warmstart: peak learning rate")
+    ap.add_argument("--data-max-steps", type=int, default=200000,
+                    help="warmstart: step cap of the training-data runs (unconverged dropped)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -389,9 +391,10 @@ def bench_warmstart(args):
     specs = SW.random_cavities(args.train_count, seed=1, side=(lo, hi), re=(100.0, 5000.0))
     X, T = [], []
     per = 500
+    dopts = SW.sweep_options(crit=args.crit, max_steps=args.data_max_steps)
     for b0 in range(0, len(specs), per):
         part = specs[b0:b0 + per]
-        fields, it, r = WS.steady_states(part, opts)
+        fields, it, r = WS.steady_states(part, dopts)
         for s_, f, k in zip(part, fields, it):
             if k > 0:
                 X.append(WS.encode(s_, G))
@@ -400,7 +403,7 @@ def bench_warmstart(args):
             f"{(it > 0).sum()}/{len(part)} converged")
     X, T = np.stack(X), np.stack(T)
     t_data = time.perf_counter() - t0
-    net = lbm.Net(G, G, batch=args.batch, base=32, levels=4, seed=0)
+    net = lbm.Net(G, G, batch=args.batch, base=32, levels=4, seed=0, in_channels=WS.IN_CH)
     t1 = time.perf_counter()
     hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log)
     t_train = time.perf_counter() - t1

@@ -296,18 +296,22 @@ int lbm_iteration(const char* NodeDataFile, const char* BCconnectorDataFile, flo
 /* ---------------------------------------------------------------- warm start */
 /* The warm-start network: a U-Net of 3x3 convolutions (BF16 weights and
  * activations, FP32 accumulation on the matrix cores, hand-written backward
- * and Adam) mapping (fluid mask, normalised log Re, normalised lid speed) on
- * an H x W grid to the steady (u, v, rho) there.  `levels` resolutions with
- * base << l channels (l = 0 .. levels-1); base a multiple of 32.  Arrays are
- * [sample][H][W][3] FP32, channel 0 of the input is the fluid mask (1/0) and
- * weighs the loss.  No counterpart in the reference. */
+ * and Adam) mapping per-pixel cavity descriptors (fluid mask, normalised
+ * log Re, lid speed, wall distances, ...) on an H x W grid to the steady
+ * (u, v, rho) there.  `levels` resolutions with base << l channels
+ * (l = 0 .. levels-1); base a multiple of 32.  Inputs are
+ * [sample][H][W][in_channels] FP32 (in_channels <= 32), channel 0 the fluid
+ * mask (1/0) that weighs the loss; targets and predictions [sample][H][W][3].
+ * No counterpart in the reference. */
 typedef struct lbm_net lbm_net;
-int lbm_net_create(int H, int W, int batch, int base, int levels, unsigned seed, lbm_net** out);
+int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channels,
+                   unsigned seed, lbm_net** out);
 void lbm_net_destroy(lbm_net* n);
 long lbm_net_num_params(const lbm_net* n);
 /* FP32 master parameters in canonical order: per conv (encoder a/b per
  * level incl. the bottleneck, then decoder a/b from the coarsest level)
- * W[Cout][9][Cin] then bias[Cout] (Cin of the first conv padded to 32);
+ * W[Cout][9][Cin] then bias[Cout] (Cin of the first conv padded to 32,
+ * weights of the padding channels stay 0 in training);
  * then the 1x1 head W[3][base], bias[3].  set_params resets Adam. */
 int lbm_net_get_params(lbm_net* n, float* params);
 int lbm_net_set_params(lbm_net* n, const float* params);

@@ -128,11 +128,11 @@ def relu(x):
 
 
 def forward(params, inp, levels, base, keep=False):
-    """inp [N,H,W,3] -> y [N,H,W,3] (and the cache for backward)."""
+    """inp [N,H,W,Ci] (Ci <= 32) -> y [N,H,W,3] (and the cache for backward)."""
     convs, (wh, bh) = unpack(params, levels, base)
     N, H, W, _ = inp.shape
     x = np.zeros((N, H, W, CIN0), params.dtype)
-    x[..., :3] = inp
+    x[..., :inp.shape[-1]] = inp
     cache = {"x0": x}
     k = 0
     a, s, p_a, p_s, xp, pc = {}, {}, {}, {}, {}, {}
@@ -208,12 +208,12 @@ def loss_and_grad(params, inp, target, levels, base):
     return loss, grads
 
 
-def init_params(levels, base, seed=0, dtype=np.float64):
+def init_params(levels, base, seed=0, dtype=np.float64, in_channels=3):
     rng = np.random.default_rng(seed)
     parts = []
     for i, (c0, c1, co, _) in enumerate(layer_shapes(levels, base)):
         ci = c0 + c1
-        real = 3 if i == 0 else ci
+        real = in_channels if i == 0 else ci
         w = rng.normal(0, np.sqrt(2.0 / (9 * real)), (co, 9, ci))
         w[:, :, real:] = 0
         parts += [w.ravel(), rng.normal(0, 0.05, co)]

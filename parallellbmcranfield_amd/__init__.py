@@ -157,7 +157,7 @@ def lib() -> C.CDLL:
         "lbm_run_checkpointed": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                      C.POINTER(Options), C.c_char_p, C.c_char_p, I]),
         "lbm_run": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Options), I]),
-        "lbm_net_create": (I, [I, I, I, I, I, C.c_uint, C.POINTER(P)]),
+        "lbm_net_create": (I, [I, I, I, I, I, I, C.c_uint, C.POINTER(P)]),
         "lbm_net_destroy": (None, [P]),
         "lbm_net_num_params": (C.c_long, [P]),
         "lbm_net_get_params": (I, [P, P]),
@@ -444,14 +444,16 @@ class Solver:
 class Net:
     """The warm-start network (lbm_net_*): U-Net of 3x3 convolutions on an
     H x W grid, BF16 on the matrix cores, trained with Adam on the device.
-    Arrays are [sample, H, W, 3] float32; input channel 0 is the fluid mask."""
+    Inputs are [sample, H, W, in_channels] float32 (channel 0: fluid mask),
+    targets and predictions [sample, H, W, 3]."""
 
     def __init__(self, H: int, W: int, batch: int = 16, base: int = 32, levels: int = 4,
-                 seed: int = 0):
+                 seed: int = 0, in_channels: int = 3):
         h = C.c_void_p()
-        _check(lib().lbm_net_create(H, W, batch, base, levels, seed, C.byref(h)))
+        _check(lib().lbm_net_create(H, W, batch, base, levels, in_channels, seed, C.byref(h)))
         self.h = h
         self.H, self.W, self.batch, self.base, self.levels = H, W, batch, base, levels
+        self.in_channels = in_channels
 
     @property
     def num_params(self) -> int:
@@ -467,19 +469,19 @@ class Net:
         assert p.size == self.num_params
         _check(lib().lbm_net_set_params(self.h, _ptr(p)))
 
-    def _arr(self, a):
+    def _arr(self, a, ch=3):
         a = np.ascontiguousarray(a, dtype=np.float32)
-        assert a.shape[1:] == (self.H, self.W, 3), a.shape
+        assert a.shape[1:] == (self.H, self.W, ch), a.shape
         return a
 
     def predict(self, inp: np.ndarray) -> np.ndarray:
-        inp = self._arr(inp)
-        out = np.zeros_like(inp)
+        inp = self._arr(inp, self.in_channels)
+        out = np.zeros(inp.shape[:3] + (3,), np.float32)
         _check(lib().lbm_net_predict(self.h, _ptr(inp), inp.shape[0], _ptr(out)))
         return out
 
     def gradients(self, inp: np.ndarray, target: np.ndarray):
-        inp, target = self._arr(inp), self._arr(target)
+        inp, target = self._arr(inp, self.in_channels), self._arr(target)
         loss = C.c_double()
         g = np.zeros(self.num_params, np.float32)
         _check(lib().lbm_net_gradients(self.h, _ptr(inp), _ptr(target), inp.shape[0],
@@ -487,7 +489,7 @@ class Net:
         return loss.value, g
 
     def set_data(self, inp: np.ndarray, target: np.ndarray) -> None:
-        inp, target = self._arr(inp), self._arr(target)
+        inp, target = self._arr(inp, self.in_channels), self._arr(target)
         _check(lib().lbm_net_set_data(self.h, _ptr(inp), _ptr(target), inp.shape[0]))
 
     def train(self, idx, lr: float) -> float:

@@ -3,6 +3,8 @@
 // (u, v, rho) there.  Device-resident BF16 weights and activations with an
 // FP32 master copy, hand-written forward/backward (nn_kernels.hip) and Adam.
 //
+// Inputs carry `cin` channels (<= 32, zero-padded to 32 for the first conv);
+// channel 0 is the fluid mask that weighs the loss.
 // Architecture for `levels` L and base width c (C_l = c 2^l):
 //   encoder l < L-1:  a_l = relu(conv(x_l)), s_l = relu(conv(a_l)), x_{l+1} = avgpool2(s_l)
 //   bottleneck:       a_{L-1} = relu(conv(x_{L-1})), s_{L-1} = relu(conv(a_{L-1}))
@@ -44,7 +46,7 @@ struct Conv {
   } while (0)
 
 struct lbm_net {
-  int H = 0, W = 0, Bmax = 0, base = 32, L = 4;
+  int H = 0, W = 0, Bmax = 0, base = 32, L = 4, cin = 3;  // cin: input channels
   hipStream_t st = nullptr;
   std::vector<Conv> conv;
   size_t nparam = 0, head_w = 0, head_b = 0;
@@ -113,8 +115,10 @@ static double urand(uint64_t& st) {
   return ((st >> 11) + 1) * (1.0 / 9007199254740992.0);
 }
 
-int lbm_net_create(int H, int W, int batch, int base, int levels, unsigned seed, lbm_net** out) {
-  if (!out || H < 2 || W < 2 || batch < 1 || base < 32 || base % 32 || levels < 2 || levels > 6)
+int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channels, unsigned seed,
+                   lbm_net** out) {
+  if (!out || H < 2 || W < 2 || batch < 1 || base < 32 || base % 32 || levels < 2 || levels > 6 ||
+      in_channels < 1 || in_channels > CIN0)
     return LBM_EINVAL;
   *out = nullptr;
   lbm_net* n = new lbm_net();
@@ -123,6 +127,7 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, unsigned seed,
   n->Bmax = batch;
   n->base = base;
   n->L = levels;
+  n->cin = in_channels;
   const int L = levels;
   n->Hl.resize(L);
   n->Wl.resize(L);
@@ -199,7 +204,7 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, unsigned seed,
   if (!rc) rc = nalloc_t(n, &n->mask, P0);
   if (!rc) rc = nalloc_t(n, &n->lpart, (P0 + 255) / 256);
   if (!rc) rc = nalloc_t(n, &n->idx, (size_t)B);
-  if (!rc) rc = nalloc_t(n, &n->stage_in, P0 * 3);
+  if (!rc) rc = nalloc_t(n, &n->stage_in, P0 * in_channels);
   if (!rc) rc = nalloc_t(n, &n->stage_t, P0 * 3);
   // partial-sum scratch: the largest conv weight-gradient partials, the
   // bias and head partials
@@ -224,7 +229,7 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, unsigned seed,
   for (size_t ci = 0; ci < n->conv.size(); ci++) {
     const Conv& c = n->conv[ci];
     const int Cin = c.C0 + c.C1;
-    const int real = ci == 0 ? 3 : Cin;
+    const int real = ci == 0 ? n->cin : Cin;
     const double sd = std::sqrt(2.0 / (9.0 * real));
     for (int co = 0; co < c.Cout; co++)
       for (int tp = 0; tp < 9; tp++)
@@ -413,16 +418,17 @@ static int upload_batch(lbm_net* n, const float* in, const float* tg, int nb, do
   const size_t HW = (size_t)n->H * n->W;
   std::vector<int> id(nb);
   for (int b = 0; b < nb; b++) id[b] = b;
-  NC(hipMemcpyAsync(n->stage_in, in, sizeof(float) * HW * 3 * nb, hipMemcpyHostToDevice, n->st));
+  NC(hipMemcpyAsync(n->stage_in, in, sizeof(float) * HW * n->cin * nb, hipMemcpyHostToDevice,
+                    n->st));
   if (tg)
     NC(hipMemcpyAsync(n->stage_t, tg, sizeof(float) * HW * 3 * nb, hipMemcpyHostToDevice, n->st));
   NC(hipMemcpyAsync(n->idx, id.data(), sizeof(int) * nb, hipMemcpyHostToDevice, n->st));
   NC(gather(n->stage_in, n->stage_t, n->idx, n->x0, tg ? n->t : nullptr, tg ? n->mask : nullptr,
-            nb, (int)HW, CIN0, n->st));
+            nb, (int)HW, n->cin, CIN0, n->st));
   NC(hipStreamSynchronize(n->st));  // the host arrays may go after return
   if (fluid) {
     double f = 0.0;
-    for (size_t q = 0; q < HW * nb; q++) f += in[q * 3];
+    for (size_t q = 0; q < HW * nb; q++) f += in[q * n->cin];
     *fluid = f;
   }
   return LBM_OK;
@@ -433,7 +439,7 @@ int lbm_net_predict(lbm_net* n, const float* in, int nb, float* out) {
   const size_t HW = (size_t)n->H * n->W;
   for (int b0 = 0; b0 < nb; b0 += n->Bmax) {
     const int B = std::min(n->Bmax, nb - b0);
-    int rc = upload_batch(n, in + HW * 3 * b0, nullptr, B, nullptr);
+    int rc = upload_batch(n, in + HW * n->cin * b0, nullptr, B, nullptr);
     if (rc) return rc;
     rc = forward(n, B);
     if (rc) return rc;
@@ -469,17 +475,17 @@ int lbm_net_set_data(lbm_net* n, const float* in, const float* target, int count
       if (q == n->set_in || q == n->set_t) q = nullptr;
     n->set_in = n->set_t = nullptr;
   }
-  NC(hipMalloc((void**)&n->set_in, sizeof(float) * HW * 3 * count));
+  NC(hipMalloc((void**)&n->set_in, sizeof(float) * HW * n->cin * count));
   NC(hipMalloc((void**)&n->set_t, sizeof(float) * HW * 3 * count));
   n->allocs.push_back(n->set_in);
   n->allocs.push_back(n->set_t);
-  NC(hipMemcpy(n->set_in, in, sizeof(float) * HW * 3 * count, hipMemcpyHostToDevice));
+  NC(hipMemcpy(n->set_in, in, sizeof(float) * HW * n->cin * count, hipMemcpyHostToDevice));
   NC(hipMemcpy(n->set_t, target, sizeof(float) * HW * 3 * count, hipMemcpyHostToDevice));
   n->nset = count;
   n->set_fluid.assign(count, 0.0);
   for (int k = 0; k < count; k++) {
     double f = 0.0;
-    for (size_t q = 0; q < HW; q++) f += in[((size_t)k * HW + q) * 3];
+    for (size_t q = 0; q < HW; q++) f += in[((size_t)k * HW + q) * n->cin];
     n->set_fluid[k] = f;
   }
   return LBM_OK;
@@ -496,7 +502,8 @@ int lbm_net_train(lbm_net* n, const int* idx, int nb, float lr, double* loss) {
     fluid += n->set_fluid[idx[b]];
   }
   NC(hipMemcpyAsync(n->idx, idx, sizeof(int) * nb, hipMemcpyHostToDevice, n->st));
-  NC(gather(n->set_in, n->set_t, n->idx, n->x0, n->t, n->mask, nb, n->H * n->W, CIN0, n->st));
+  NC(gather(n->set_in, n->set_t, n->idx, n->x0, n->t, n->mask, nb, n->H * n->W, n->cin, CIN0,
+            n->st));
   int rc = forward(n, nb);
   if (!rc) rc = backward(n, nb, fluid, loss);
   if (rc) return rc;
@@ -519,7 +526,7 @@ int lbm_net_save(lbm_net* n, const char* path) {
     set_error(std::string("cannot write ") + path);
     return LBM_EIO;
   }
-  const int hdr[6] = {0x4c4e4554 /* "LNET" */, n->H, n->W, n->base, n->L, (int)n->nparam};
+  const int hdr[7] = {0x4c4e4554 /* "LNET" */, n->H, n->W, n->base, n->L, n->cin, (int)n->nparam};
   bool ok = std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
             std::fwrite(hp.data(), sizeof(float), hp.size(), f) == hp.size();
   ok = (std::fclose(f) == 0) && ok;
@@ -537,10 +544,11 @@ int lbm_net_load(lbm_net* n, const char* path) {
     set_error(std::string("cannot open ") + path);
     return LBM_EIO;
   }
-  int hdr[6];
+  int hdr[7];
   std::vector<float> hp(n->nparam);
   bool ok = std::fread(hdr, sizeof hdr, 1, f) == 1 && hdr[0] == 0x4c4e4554 && hdr[1] == n->H &&
-            hdr[2] == n->W && hdr[3] == n->base && hdr[4] == n->L && hdr[5] == (int)n->nparam &&
+            hdr[2] == n->W && hdr[3] == n->base && hdr[4] == n->L && hdr[5] == n->cin &&
+            hdr[6] == (int)n->nparam &&
             std::fread(hp.data(), sizeof(float), hp.size(), f) == hp.size();
   std::fclose(f);
   if (!ok) {

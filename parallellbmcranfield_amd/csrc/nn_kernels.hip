@@ -415,20 +415,21 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // ------------------------------------------------------------ data
-// x[b][y][x][0..Cp) bf16 from samples idx[b] of an FP32 [N][H][W][3]
-// input set (channels >= 3 zero); t[b] and mask[b] from the target set and
-// input channel 0.
+// x[b][y][x][0..Cp) bf16 from samples idx[b] of an FP32 [N][H][W][Ci]
+// input set (channels >= Ci zero); t[b] from the [N][H][W][3] target set and
+// mask[b] from input channel 0.
 __global__ void k_gather(const float* __restrict__ inputs, const float* __restrict__ targets,
                          const int* __restrict__ idx, bf16* __restrict__ x, float* __restrict__ t,
-                         float* __restrict__ mask, int B, int HW, int Cp) {
+                         float* __restrict__ mask, int B, int HW, int Ci, int Cp) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)B * HW) return;
   const int b = (int)(i / HW);
   const long q = i % HW;
-  const size_t src = ((size_t)idx[b] * HW + q) * 3;
-  for (int c = 0; c < Cp; c++) x[i * Cp + c] = (__bf16)(c < 3 ? inputs[src + c] : 0.0f);
+  const size_t src = ((size_t)idx[b] * HW + q) * Ci;
+  for (int c = 0; c < Cp; c++) x[i * Cp + c] = (__bf16)(c < Ci ? inputs[src + c] : 0.0f);
   if (t) {
-    for (int c = 0; c < 3; c++) t[i * 3 + c] = targets[src + c];
+    const size_t ts = ((size_t)idx[b] * HW + q) * 3;
+    for (int c = 0; c < 3; c++) t[i * 3 + c] = targets[ts + c];
     mask[i] = inputs[src];
   }
 }
@@ -547,8 +548,9 @@ hipError_t adam(float* p, const float* g, float* m, float* v, u16* pb, long n, f
 }
 
 hipError_t gather(const float* inputs, const float* targets, const int* idx, u16* x, float* t,
-                  float* mask, int B, int HW, int Cp, hipStream_t st) {
-  k_gather<<<nblk((long)B * HW), 256, 0, st>>>(inputs, targets, idx, mb(x), t, mask, B, HW, Cp);
+                  float* mask, int B, int HW, int Ci, int Cp, hipStream_t st) {
+  k_gather<<<nblk((long)B * HW), 256, 0, st>>>(inputs, targets, idx, mb(x), t, mask, B, HW, Ci,
+                                              Cp);
   return hipGetLastError();
 }
 

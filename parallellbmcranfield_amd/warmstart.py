@@ -9,6 +9,10 @@ maps a cavity drawn on a G x G grid to its steady (u, v, rho) there:
                       origin; grid points outside it or in an obstacle are 0
            channel 1  (ln Re - ln 100) / ln 50      (Re on the lid length)
            channel 2  lid speed / 0.1
+           channels 3-6  distances to the west, east, south and north (lid)
+                      walls over the longer side (0 outside the cavity):
+                      a 3x3-conv U-Net sees about 100 grid points, less than
+                      the cavity, so every point is told where it lies
   targets  u / lid, v / lid, (rho - 1) / (3 lid^2)   (0 outside the fluid)
 
 Training data are steady states of seeded cavities run by the batched solver
@@ -29,6 +33,7 @@ from . import INIT_FIELD, INIT_FIELD_NEQ, Case, Net, Solver
 from . import sweep as SW
 
 GRID = 257
+IN_CH = 7
 RE_LO, RE_SPAN = math.log(100.0), math.log(50.0)
 
 
@@ -47,7 +52,7 @@ def _fluid_mask(spec):
 
 
 def encode(spec, G: int = GRID) -> np.ndarray:
-    """Network input [G, G, 3] of one cavity specification."""
+    """Network input [G, G, IN_CH] of one cavity specification."""
     NX, NY, scale = _geometry(spec, G)
     fl = _fluid_mask(spec)
     g = np.arange(G)
@@ -58,10 +63,16 @@ def encode(spec, G: int = GRID) -> np.ndarray:
     jy, ix = np.meshgrid(np.minimum(ii, NY - 1), np.minimum(ii, NX - 1), indexing="ij")
     inside = inside_y[:, None] & inside_x[None, :]
     mask[inside] = fl[jy[inside], ix[inside]]
-    out = np.zeros((G, G, 3), np.float32)
+    out = np.zeros((G, G, IN_CH), np.float32)
     out[..., 0] = mask
     out[..., 1] = (math.log(spec["Re"]) - RE_LO) / RE_SPAN
     out[..., 2] = spec["lid"] / 0.1
+    Lmax = max(NX, NY) - 1
+    xs, ys = np.meshgrid(g / scale, g / scale)  # node coordinates (0-based) of grid points
+    out[..., 3] = np.where(inside, xs / Lmax, 0.0)
+    out[..., 4] = np.where(inside, (NX - 1 - xs) / Lmax, 0.0)
+    out[..., 5] = np.where(inside, ys / Lmax, 0.0)
+    out[..., 6] = np.where(inside, (NY - 1 - ys) / Lmax, 0.0)
     return out
 
 
@@ -240,5 +251,5 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         predict_s=warm["predict_s"])
 
 
-__all__ = ["GRID", "encode", "field_to_grid", "grid_to_field", "steady_states", "train",
+__all__ = ["GRID", "IN_CH", "encode", "field_to_grid", "grid_to_field", "steady_states", "train",
            "warm_fields", "packed_field", "compare_starts"]

@@ -15,12 +15,14 @@ from oracle import nn_oracle as NN
 pytestmark = pytest.mark.gpu
 
 
-def _sample(N, H, W, seed):
+def _sample(N, H, W, seed, ch=3):
     rng = np.random.default_rng(seed)
-    inp = np.zeros((N, H, W, 3), np.float32)
+    inp = np.zeros((N, H, W, ch), np.float32)
     inp[..., 0] = rng.random((N, H, W)) > 0.2
     inp[..., 1] = rng.random((N, 1, 1))
     inp[..., 2] = rng.random((N, 1, 1))
+    if ch > 3:
+        inp[..., 3:] = rng.random((N, H, W, ch - 3))
     yy, xx = np.meshgrid(np.linspace(0, 1, H), np.linspace(0, 1, W), indexing="ij")
     tgt = np.stack([np.sin(3 * xx) * yy, np.cos(2 * yy) * xx, xx * yy], -1)[None].repeat(N, 0)
     tgt = (tgt * (1 + inp[..., 1:2])).astype(np.float32)
@@ -31,13 +33,15 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("H,W,levels", [(37, 37, 4), (20, 27, 3)])
-def test_forward_and_gradients_match_oracle(H, W, levels):
-    print("net vs oracle", H, W, levels, flush=True)
-    net = lbm.Net(H, W, batch=2, base=32, levels=levels, seed=1)
+@pytest.mark.parametrize("H,W,levels,ch", [(37, 37, 4, 3), (20, 27, 3, 7)])
+def test_forward_and_gradients_match_oracle(H, W, levels, ch):
+    print("net vs oracle", H, W, levels, ch, flush=True)
+    net = lbm.Net(H, W, batch=2, base=32, levels=levels, seed=1, in_channels=ch)
     p = net.params()
     assert p.size == NN.num_params(levels, 32)
-    inp, tgt = _sample(2, H, W, 2)
+    w0 = NN.unpack(p, levels, 32)[0][0][0]
+    assert not w0[:, :, ch:].any() and w0[:, :, :ch].any()  # padding channels start at 0
+    inp, tgt = _sample(2, H, W, 2, ch)
     y = net.predict(inp)
     y_ref = NN.forward(p.astype(np.float32), inp, levels, 32)
     e = _rel(y, y_ref)

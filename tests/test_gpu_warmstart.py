@@ -26,7 +26,7 @@ def test_trained_warm_start_beats_rest_start_on_held_out_cavities():
     X = np.stack([WS.encode(s, G) for s, k in zip(train_specs, ok) if k])
     T = np.stack([WS.field_to_grid(s, *f, G) for s, f, k in zip(train_specs, fields, ok) if k])
     print("warm start: training on", len(X), flush=True)
-    net = lbm.Net(G, G, batch=8, base=32, levels=4, seed=0)
+    net = lbm.Net(G, G, batch=8, base=32, levels=4, seed=0, in_channels=WS.IN_CH)
     hist = WS.train(net, X, T, epochs=160, lr=3e-3)
     print("loss", hist[0], hist[-1], flush=True)
     assert hist[-1] < 0.3 * hist[0]

@@ -14,7 +14,7 @@ def _spec(NX, NY, rects=(), Re=400.0, lid=0.1):
 def test_encode_channels_and_mask():
     s = _spec(129, 65, [(10, 10, 20, 30)], Re=1000.0)
     x = WS.encode(s, 257)
-    assert x.shape == (257, 257, 3)
+    assert x.shape == (257, 257, WS.IN_CH)
     assert np.allclose(x[..., 1], (math.log(1000) - math.log(100)) / math.log(50))
     assert np.allclose(x[..., 2], 1.0)
     # scale 2: the cavity covers grid columns 0..256 and rows 0..128
@@ -25,6 +25,13 @@ def test_encode_channels_and_mask():
     assert m[2 * 15, 2 * 15] == 0 and m[2 * 15, 2 * 40] == 1
     fluid = 129 * 65 - 11 * 21
     assert abs(m.sum() / 4 - fluid) < 0.05 * fluid
+    # wall distances over the longer side (128 node spacings): west + east = 1,
+    # south + north = 64 / 128 inside the cavity, 0 outside
+    assert np.allclose(x[:129, :, 3] + x[:129, :, 4], 1.0)
+    assert np.allclose(x[:129, :, 5] + x[:129, :, 6], 0.5)
+    assert x[0, 0, 3] == 0 and x[128, 256, 4] == 0 and x[128, 100, 6] == 0
+    assert np.isclose(x[100, 50, 3], 25 / 128) and np.isclose(x[100, 50, 5], 50 / 128)
+    assert not x[129:, :, 3:].any()
 
 
 def test_resampling_round_trip_is_exact_on_grid_aligned_nodes():
