@@ -403,7 +403,8 @@ def bench_warmstart(args):
             f"{(it > 0).sum()}/{len(part)} converged")
     X, T = np.stack(X), np.stack(T)
     t_data = time.perf_counter() - t0
-    net = lbm.Net(G, G, batch=args.batch, base=32, levels=4, seed=0, in_channels=WS.IN_CH)
+    net = lbm.Net(G, G, batch=args.batch, base=32, levels=4, seed=0, in_channels=WS.IN_CH,
+                  dilation=WS.DILATION)
     t1 = time.perf_counter()
     hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log)
     t_train = time.perf_counter() - t1
@@ -432,6 +433,7 @@ def bench_warmstart(args):
                         f"{args.test_count} held-out (seed 2); 257^2 grid U-Net, 4 levels, "
                         f"32-256 channels; MRT f32 lattice, residual {args.crit:g}",
             "params": net.num_params,
+            "bottleneck_dilation": WS.DILATION,
             "training_samples": int(len(hist) and args.train_count),
             "epochs": args.epochs,
             "batch": args.batch,
@@ -448,6 +450,7 @@ def bench_warmstart(args):
             "batch_wall_warm_s": round(r["wall_warm"], 2),
             "predict_s": round(r["predict_s"], 3),
             "field_rel_l2_median": r["field_rel_l2_median"],
+            "seed_rel_l2_median": r["seed_rel_l2_median"],
             "field_rel_l2_max": float(r["field_rel_l2"][both].max()) if both.any() else None,
             "loss_first_last": [hist[0], hist[-1]],
             "data_s": round(t_data, 1),

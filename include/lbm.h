@@ -304,8 +304,10 @@ int lbm_iteration(const char* NodeDataFile, const char* BCconnectorDataFile, flo
  * mask (1/0) that weighs the loss; targets and predictions [sample][H][W][3].
  * No counterpart in the reference. */
 typedef struct lbm_net lbm_net;
+/* dilation d >= 1: the two bottleneck convolutions use 3x3 stencils dilated
+ * by d and 2d (1: plain), widening the receptive field without parameters. */
 int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channels,
-                   unsigned seed, lbm_net** out);
+                   int dilation, unsigned seed, lbm_net** out);
 void lbm_net_destroy(lbm_net* n);
 long lbm_net_num_params(const lbm_net* n);
 /* FP32 master parameters in canonical order: per conv (encoder a/b per

@@ -10,7 +10,7 @@ documented with ``lbm_net_get_params`` in include/lbm.h.
 
 Layers for `levels` L and base c (C_l = c 2^l):
   encoder l < L-1:  a_l = relu(conv(x_l)), s_l = relu(conv(a_l)), x_{l+1} = avgpool2(s_l)
-  bottleneck:       a_{L-1}, s_{L-1} likewise
+  bottleneck:       a_{L-1}, s_{L-1} likewise, 3x3 stencils dilated by dil and 2 dil
   decoder l = L-2..0: da_l = relu(conv([s_l, up2(prev)])), dd_l = relu(conv(da_l))
   head:             y = dd_0 . Wh^T + bh  (3 channels)
 Loss = sum(mask (y - t)^2) / (3 sum(mask)), mask = input channel 0.
@@ -57,41 +57,42 @@ def unpack(params: np.ndarray, levels: int, base: int):
     return out, (wh, bh)
 
 
-def _patches(x):
-    """[N,H,W,C] -> [N,H,W,9,C]; tap t = 3 (dy+1) + (dx+1) reads x[y+dy, x+dx]."""
+def _patches(x, d=1):
+    """[N,H,W,C] -> [N,H,W,9,C]; tap t = 3 (dy+1) + (dx+1) reads x[y+d dy, x+d dx]
+    (zero outside)."""
     N, H, W, C = x.shape
-    xp = np.zeros((N, H + 2, W + 2, C), x.dtype)
-    xp[:, 1:-1, 1:-1] = x
-    return np.stack([xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+    xp = np.zeros((N, H + 2 * d, W + 2 * d, C), x.dtype)
+    xp[:, d:d + H, d:d + W] = x
+    return np.stack([xp[:, d + d * dy:d + d * dy + H, d + d * dx:d + d * dx + W]
                      for dy in (-1, 0, 1) for dx in (-1, 0, 1)], axis=3)
 
 
-def _col2im(dp):
+def _col2im(dp, d=1):
     """Adjoint of _patches: [N,H,W,9,C] -> [N,H,W,C]."""
     N, H, W, _, C = dp.shape
-    dx_ = np.zeros((N, H + 2, W + 2, C), dp.dtype)
+    dx_ = np.zeros((N, H + 2 * d, W + 2 * d, C), dp.dtype)
     t = 0
     for dy in (-1, 0, 1):
         for dx in (-1, 0, 1):
-            dx_[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W] += dp[:, :, :, t]
+            dx_[:, d + d * dy:d + d * dy + H, d + d * dx:d + d * dx + W] += dp[:, :, :, t]
             t += 1
-    return dx_[:, 1:-1, 1:-1]
+    return dx_[:, d:d + H, d:d + W]
 
 
-def conv(x, w, b):
-    p = _patches(x)
+def conv(x, w, b, d=1):
+    p = _patches(x, d)
     N, H, W_, _, C = p.shape
     y = p.reshape(-1, 9 * C) @ w.reshape(w.shape[0], -1).T + b
     return y.reshape(N, H, W_, -1), p
 
 
-def conv_bwd(g, p, w):
+def conv_bwd(g, p, w, d=1):
     N, H, W_, _, C = p.shape
     g2 = g.reshape(-1, g.shape[-1])
     dw = (g2.T @ p.reshape(-1, 9 * C)).reshape(w.shape)
     db = g2.sum(0)
     dp = (g2 @ w.reshape(w.shape[0], -1)).reshape(N, H, W_, 9, C)
-    return _col2im(dp), dw, db
+    return _col2im(dp, d), dw, db
 
 
 def pool(x):
@@ -127,7 +128,7 @@ def relu(x):
     return np.maximum(x, 0)
 
 
-def forward(params, inp, levels, base, keep=False):
+def forward(params, inp, levels, base, keep=False, dil=1):
     """inp [N,H,W,Ci] (Ci <= 32) -> y [N,H,W,3] (and the cache for backward)."""
     convs, (wh, bh) = unpack(params, levels, base)
     N, H, W, _ = inp.shape
@@ -139,9 +140,10 @@ def forward(params, inp, levels, base, keep=False):
     for l in range(levels):
         (wa, ba), (wb, bb) = convs[k], convs[k + 1]
         k += 2
-        za, p_a[l] = conv(x, wa, ba)
+        da_, db_ = (dil, 2 * dil) if l == levels - 1 else (1, 1)  # dilated bottleneck
+        za, p_a[l] = conv(x, wa, ba, da_)
         a[l] = relu(za)
-        zs, p_s[l] = conv(a[l], wb, bb)
+        zs, p_s[l] = conv(a[l], wb, bb, db_)
         s[l] = relu(zs)
         if l < levels - 1:
             xp[l], pc[l] = pool(s[l])
@@ -166,9 +168,9 @@ def forward(params, inp, levels, base, keep=False):
     return y
 
 
-def loss_and_grad(params, inp, target, levels, base):
+def loss_and_grad(params, inp, target, levels, base, dil=1):
     """Masked MSE and d loss / d params (canonical layout)."""
-    y, c = forward(params, inp, levels, base, keep=True)
+    y, c = forward(params, inp, levels, base, keep=True, dil=dil)
     convs, (wh, bh) = unpack(params, levels, base)
     grads = np.zeros_like(params)
     gconvs, (gwh, gbh) = unpack(grads, levels, base)
@@ -202,9 +204,11 @@ def loss_and_grad(params, inp, target, levels, base):
             Cl = c["s"][l].shape[-1]
             H, W = c["s"][l].shape[1:3]
             g = (gcat[l][..., :Cl] + pool_bwd(gxp, c["pc"][l], H, W)) * (c["s"][l] > 0)
-        gx, gconvs[2 * l + 1][0][...], gconvs[2 * l + 1][1][...] = conv_bwd(g, c["p_s"][l], wb)
+        da_, db_ = (dil, 2 * dil) if l == L - 1 else (1, 1)
+        gx, gconvs[2 * l + 1][0][...], gconvs[2 * l + 1][1][...] = conv_bwd(g, c["p_s"][l], wb,
+                                                                             db_)
         gx = gx * (c["a"][l] > 0)
-        gxp, gconvs[2 * l][0][...], gconvs[2 * l][1][...] = conv_bwd(gx, c["p_a"][l], wa)
+        gxp, gconvs[2 * l][0][...], gconvs[2 * l][1][...] = conv_bwd(gx, c["p_a"][l], wa, da_)
     return loss, grads
 
 

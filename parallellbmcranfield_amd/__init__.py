@@ -157,7 +157,7 @@ def lib() -> C.CDLL:
         "lbm_run_checkpointed": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                      C.POINTER(Options), C.c_char_p, C.c_char_p, I]),
         "lbm_run": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Options), I]),
-        "lbm_net_create": (I, [I, I, I, I, I, I, C.c_uint, C.POINTER(P)]),
+        "lbm_net_create": (I, [I, I, I, I, I, I, I, C.c_uint, C.POINTER(P)]),
         "lbm_net_destroy": (None, [P]),
         "lbm_net_num_params": (C.c_long, [P]),
         "lbm_net_get_params": (I, [P, P]),
@@ -448,9 +448,11 @@ class Net:
     targets and predictions [sample, H, W, 3]."""
 
     def __init__(self, H: int, W: int, batch: int = 16, base: int = 32, levels: int = 4,
-                 seed: int = 0, in_channels: int = 3):
+                 seed: int = 0, in_channels: int = 3, dilation: int = 1):
         h = C.c_void_p()
-        _check(lib().lbm_net_create(H, W, batch, base, levels, in_channels, seed, C.byref(h)))
+        _check(lib().lbm_net_create(H, W, batch, base, levels, in_channels, dilation, seed,
+                                    C.byref(h)))
+        self.dilation = dilation
         self.h = h
         self.H, self.W, self.batch, self.base, self.levels = H, W, batch, base, levels
         self.in_channels = in_channels

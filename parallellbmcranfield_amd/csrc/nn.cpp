@@ -7,7 +7,8 @@
 // channel 0 is the fluid mask that weighs the loss.
 // Architecture for `levels` L and base width c (C_l = c 2^l):
 //   encoder l < L-1:  a_l = relu(conv(x_l)), s_l = relu(conv(a_l)), x_{l+1} = avgpool2(s_l)
-//   bottleneck:       a_{L-1} = relu(conv(x_{L-1})), s_{L-1} = relu(conv(a_{L-1}))
+//   bottleneck:       a_{L-1} = relu(conv_d(x_{L-1})), s_{L-1} = relu(conv_2d(a_{L-1}))
+//                     (3x3 stencils dilated by d and 2d: the receptive field spans the grid)
 //   decoder l = L-2..0: da_l = relu(conv([s_l, up2(prev)])), dd_l = relu(conv(da_l)), prev = dd_l
 //   head:             y = 1x1 conv(dd_0) -> 3 channels (FP32)
 // Loss: mean over fluid cells (input channel 0) and the 3 channels of (y - t)^2.
@@ -34,7 +35,7 @@ namespace {
 constexpr int CIN0 = 32;  // padded input channels
 constexpr int KOUT = 3;   // u, v, rho
 struct Conv {
-  int C0, C1, Cout, level;
+  int C0, C1, Cout, level, dil;
   size_t w, b;  // offsets of W and bias in the parameter vector
 };
 }  // namespace
@@ -47,6 +48,7 @@ struct Conv {
 
 struct lbm_net {
   int H = 0, W = 0, Bmax = 0, base = 32, L = 4, cin = 3;  // cin: input channels
+  int dil = 1;  // bottleneck dilation: its convs use dil and 2 dil
   hipStream_t st = nullptr;
   std::vector<Conv> conv;
   size_t nparam = 0, head_w = 0, head_b = 0;
@@ -90,13 +92,14 @@ static int nalloc_t(lbm_net* n, T** ptr, size_t count) {
 static int C_of(const lbm_net* n, int l) { return n->base << l; }
 static long P_of(const lbm_net* n, int l, int B) { return (long)B * n->Hl[l] * n->Wl[l]; }
 
-static ConvSrc src1(const lbm_net* n, const u16* x, int C, int l, int B) {
+static ConvSrc src1(const lbm_net* n, const u16* x, int C, int l, int B, int dil = 1) {
   ConvSrc s;
   s.in0 = x;
   s.C0 = C;
   s.B = B;
   s.H = n->Hl[l];
   s.W = n->Wl[l];
+  s.dil = dil;
   return s;
 }
 
@@ -115,10 +118,10 @@ static double urand(uint64_t& st) {
   return ((st >> 11) + 1) * (1.0 / 9007199254740992.0);
 }
 
-int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channels, unsigned seed,
-                   lbm_net** out) {
+int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channels, int dilation,
+                   unsigned seed, lbm_net** out) {
   if (!out || H < 2 || W < 2 || batch < 1 || base < 32 || base % 32 || levels < 2 || levels > 6 ||
-      in_channels < 1 || in_channels > CIN0)
+      in_channels < 1 || in_channels > CIN0 || dilation < 1 || dilation > 64)
     return LBM_EINVAL;
   *out = nullptr;
   lbm_net* n = new lbm_net();
@@ -128,6 +131,7 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channel
   n->base = base;
   n->L = levels;
   n->cin = in_channels;
+  n->dil = dilation;
   const int L = levels;
   n->Hl.resize(L);
   n->Wl.resize(L);
@@ -139,20 +143,21 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channel
   }
   // layers and parameter offsets
   size_t off = 0;
-  auto add = [&](int C0, int C1, int Cout, int l) {
-    Conv c{C0, C1, Cout, l, off, 0};
+  auto add = [&](int C0, int C1, int Cout, int l, int dil) {
+    Conv c{C0, C1, Cout, l, dil, off, 0};
     off += (size_t)Cout * 9 * (C0 + C1);
     c.b = off;
     off += Cout;
     n->conv.push_back(c);
   };
   for (int l = 0; l < L; l++) {
-    add(l == 0 ? CIN0 : C_of(n, l - 1), 0, C_of(n, l), l);
-    add(C_of(n, l), 0, C_of(n, l), l);
+    const bool bot = l == L - 1;
+    add(l == 0 ? CIN0 : C_of(n, l - 1), 0, C_of(n, l), l, bot ? dilation : 1);
+    add(C_of(n, l), 0, C_of(n, l), l, bot ? 2 * dilation : 1);
   }
   for (int l = L - 2; l >= 0; l--) {
-    add(C_of(n, l), C_of(n, l + 1), C_of(n, l), l);
-    add(C_of(n, l), 0, C_of(n, l), l);
+    add(C_of(n, l), C_of(n, l + 1), C_of(n, l), l, 1);
+    add(C_of(n, l), 0, C_of(n, l), l, 1);
   }
   n->head_w = off;
   off += (size_t)KOUT * base;
@@ -297,9 +302,10 @@ static int forward(lbm_net* n, int B) {
   for (int l = 0; l < L; l++) {
     const Conv& ca = n->conv[ci++];
     const Conv& cb = n->conv[ci++];
-    NC(conv3x3(src1(n, x, Cx, l, B), n->pb + ca.w, n->p + ca.b, n->a[l], ca.Cout, true, n->st));
-    NC(conv3x3(src1(n, n->a[l], C_of(n, l), l, B), n->pb + cb.w, n->p + cb.b, n->s[l], cb.Cout,
-               true, n->st));
+    NC(conv3x3(src1(n, x, Cx, l, B, ca.dil), n->pb + ca.w, n->p + ca.b, n->a[l], ca.Cout, true,
+               n->st));
+    NC(conv3x3(src1(n, n->a[l], C_of(n, l), l, B, cb.dil), n->pb + cb.w, n->p + cb.b, n->s[l],
+               cb.Cout, true, n->st));
     if (l < L - 1) {
       NC(pool(n->s[l], n->xp[l], B, n->Hl[l], n->Wl[l], C_of(n, l), n->st));
       x = n->xp[l];
@@ -342,8 +348,8 @@ static int conv_grads(lbm_net* n, const Conv& c, const ConvSrc& s, const u16* go
 
 // data gradient of conv c: gin = conv(gout, flipped W), Cin channels
 static int conv_dgrad(lbm_net* n, const Conv& c, const u16* gout, int l, int B, u16* gin) {
-  return check_hip(conv3x3(src1(n, gout, c.Cout, l, B), n->wt + c.w, nullptr, gin, c.C0 + c.C1,
-                           false, n->st),
+  return check_hip(conv3x3(src1(n, gout, c.Cout, l, B, c.dil), n->wt + c.w, nullptr, gin,
+                           c.C0 + c.C1, false, n->st),
                    "conv dgrad");
 }
 
@@ -387,14 +393,14 @@ static int backward(lbm_net* n, int B, double fluid, double* loss) {
     if (l < L - 1)
       NC(skip_grad(n->dcat[l], C_of(n, l) + C_of(n, l + 1), n->gxp[l], n->s[l], n->gs[l], B,
                    n->Hl[l], n->Wl[l], C_of(n, l), n->st));
-    int rc = conv_grads(n, cb, src1(n, n->a[l], C_of(n, l), l, B), n->gs[l]);
+    int rc = conv_grads(n, cb, src1(n, n->a[l], C_of(n, l), l, B, cb.dil), n->gs[l]);
     if (rc) return rc;
     rc = conv_dgrad(n, cb, n->gs[l], l, B, n->ga[l]);
     if (rc) return rc;
     NC(relu_bwd(n->ga[l], n->a[l], P_of(n, l, B) * C_of(n, l), n->st));
     const u16* xin = l == 0 ? n->x0 : n->xp[l - 1];
     const int Cx = l == 0 ? CIN0 : C_of(n, l - 1);
-    rc = conv_grads(n, ca, src1(n, xin, Cx, l, B), n->ga[l]);
+    rc = conv_grads(n, ca, src1(n, xin, Cx, l, B, ca.dil), n->ga[l]);
     if (rc) return rc;
     if (l > 0) {
       rc = conv_dgrad(n, ca, n->ga[l], l, B, n->gxp[l - 1]);
@@ -526,7 +532,8 @@ int lbm_net_save(lbm_net* n, const char* path) {
     set_error(std::string("cannot write ") + path);
     return LBM_EIO;
   }
-  const int hdr[7] = {0x4c4e4554 /* "LNET" */, n->H, n->W, n->base, n->L, n->cin, (int)n->nparam};
+  const int hdr[8] = {0x4c4e4554 /* "LNET" */, n->H, n->W, n->base, n->L, n->cin, n->dil,
+                      (int)n->nparam};
   bool ok = std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
             std::fwrite(hp.data(), sizeof(float), hp.size(), f) == hp.size();
   ok = (std::fclose(f) == 0) && ok;
@@ -544,11 +551,11 @@ int lbm_net_load(lbm_net* n, const char* path) {
     set_error(std::string("cannot open ") + path);
     return LBM_EIO;
   }
-  int hdr[7];
+  int hdr[8];
   std::vector<float> hp(n->nparam);
   bool ok = std::fread(hdr, sizeof hdr, 1, f) == 1 && hdr[0] == 0x4c4e4554 && hdr[1] == n->H &&
             hdr[2] == n->W && hdr[3] == n->base && hdr[4] == n->L && hdr[5] == n->cin &&
-            hdr[6] == (int)n->nparam &&
+            hdr[6] == n->dil && hdr[7] == (int)n->nparam &&
             std::fread(hp.data(), sizeof(float), hp.size(), f) == hp.size();
   std::fclose(f);
   if (!ok) {

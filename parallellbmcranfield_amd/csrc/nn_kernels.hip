@@ -31,6 +31,7 @@ struct Src {  // virtual concat input
   const bf16* in1;  // coarse, nearest-upsampled (may be null)
   int C1;
   int B, H, W;      // full size
+  int dil;          // tap spacing (dilation) of the 3x3 stencil
 };
 
 __device__ inline bf16x8 zero8() {
@@ -60,7 +61,8 @@ __device__ inline float load1(const Src& s, int b, int y, int x, int ci) {
 }
 
 // ------------------------------------------------------------ conv forward
-// out[p][co] = act(bias[co] + sum_{t, ci} in(nbr(p, t))[ci] * w[co][t][ci]).
+// out[p][co] = act(bias[co] + sum_{t, ci} in(nbr(p, t))[ci] * w[co][t][ci]),
+// nbr(p, t) = p + dil * (t / 3 - 1, t % 3 - 1).
 // Workgroup: 4 waves, each 32 pixels x 32 output channels (2 x 2 MFMA
 // tiles); COW waves along the output channels, 4/COW along the pixels.
 template <int COW, bool RELU>
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int t = 0; t < 9; t++) {
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
     for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
       bf16x8 a[2], bm[2];
 #pragma unroll
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   const int cob = blockIdx.y * 32;
   const int nci = Cin / 32;
   const int t = blockIdx.z / nci, cib = (blockIdx.z % nci) * 32;
-  const int dy = t / 3 - 1, dx = t % 3 - 1;
+  const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
   const int segs_row = (s.W + 31) / 32;
   const int nseg = s.B * s.H * segs_row;
   const int s0 = blockIdx.x * seg_per_chunk;
@@ -438,7 +440,9 @@ __global__ void k_gather(const float* __restrict__ inputs, const float* __restri
 static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 static inline const bf16* cb(const u16* p) { return reinterpret_cast<const bf16*>(p); }
 static inline bf16* mb(u16* p) { return reinterpret_cast<bf16*>(p); }
-static inline Src src_of(const ConvSrc& c) { return Src{cb(c.in0), c.C0, cb(c.in1), c.C1, c.B, c.H, c.W}; }
+static inline Src src_of(const ConvSrc& c) {
+  return Src{cb(c.in0), c.C0, cb(c.in1), c.C1, c.B, c.H, c.W, c.dil};
+}
 
 hipError_t conv3x3(const ConvSrc& cs, const u16* w, const float* bias, u16* out, int Cout,
                    bool relu, hipStream_t st) {

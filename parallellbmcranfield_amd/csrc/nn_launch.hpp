@@ -17,6 +17,7 @@ struct ConvSrc {
   const u16* in1 = nullptr;
   int C1 = 0;
   int B = 0, H = 0, W = 0;
+  int dil = 1;  // dilation of the 3x3 stencil
 };
 
 // out = act(conv3x3(src, w) + bias); w is [Cout][9][C0 + C1], bias may be null.

@@ -34,6 +34,7 @@ from . import sweep as SW
 
 GRID = 257
 IN_CH = 7
+DILATION = 4  # bottleneck stencils dilated by 4 and 8 at 33^2: the field of view spans the grid
 RE_LO, RE_SPAN = math.log(100.0), math.log(50.0)
 
 
@@ -190,11 +191,14 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
     prediction, each as one batched run to the criterion.  Per instance:
     steps and wall time to convergence (the warm start's time includes the
     prediction), and the relative L2 difference of the two converged
-    velocity fields.  neq: seed feq plus the Chapman-Enskog non-equilibrium
+    velocity fields and of the seed against the rest start's converged
+    velocity (the prediction error).  neq: seed feq plus the Chapman-Enskog non-equilibrium
     part of the predicted velocity gradients (LBM_INIT_FIELD_NEQ)."""
     opts = opts or SW.sweep_options()
     packed, fl = SW.build_batch(specs)
     inst = [(sp["nu"], sp["lid"]) for sp in specs]
+
+    seeds = {}
 
     def run(field):
         t0 = time.perf_counter()
@@ -203,6 +207,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
             flds = warm_fields(net, specs)
             fld = packed_field(packed, flds)
             pre = time.perf_counter() - t0
+            seeds["u"], seeds["v"] = fld[1], fld[2]
             o = type(opts).from_buffer_copy(opts)
             o.init = INIT_FIELD_NEQ if neq else INIT_FIELD
             s = Solver(packed, o, field=fld, instances=inst)
@@ -228,16 +233,20 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
     if log:
         log(f"warm start: {warm['wall_s']:.1f} s (prediction {warm['predict_s']:.2f} s)")
     both = (rest["conv_iter"] > 0) & (warm["conv_iter"] > 0)
-    diff = []
+    def rel(a_u, a_v, b_u, b_v, sl):
+        ur, vr = (a.reshape(packed.m, packed.n)[sl] for a in (b_u, b_v))
+        uw, vw = (a.reshape(packed.m, packed.n)[sl] for a in (a_u, a_v))
+        num = np.sqrt(((uw - ur) ** 2 + (vw - vr) ** 2).sum())
+        den = np.sqrt((ur ** 2 + vr ** 2).sum())
+        return num / den if den > 0 else 0.0
+
+    diff, pred = [], []
     for k in range(len(specs)):
         x0, y0, w, h = packed.rects[k]
         sl = (slice(y0, y0 + h), slice(x0, x0 + w))
-        ur, vr = (a.reshape(packed.m, packed.n)[sl] for a in (rest["u"], rest["v"]))
-        uw, vw = (a.reshape(packed.m, packed.n)[sl] for a in (warm["u"], warm["v"]))
-        num = np.sqrt(((uw - ur) ** 2 + (vw - vr) ** 2).sum())
-        den = np.sqrt((ur ** 2 + vr ** 2).sum())
-        diff.append(num / den if den > 0 else 0.0)
-    diff = np.asarray(diff)
+        diff.append(rel(warm["u"], warm["v"], rest["u"], rest["v"], sl))
+        pred.append(rel(seeds["u"], seeds["v"], rest["u"], rest["v"], sl))
+    diff, pred = np.asarray(diff), np.asarray(pred)
     return dict(
         n=len(specs), both_converged=int(both.sum()),
         steps_rest=rest["conv_iter"], steps_warm=warm["conv_iter"],
@@ -248,8 +257,9 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         median_time_warm=float(np.median(warm["conv_s"][both])) if both.any() else None,
         wall_rest=rest["wall_s"], wall_warm=warm["wall_s"],
         field_rel_l2=diff, field_rel_l2_median=float(np.median(diff[both])) if both.any() else None,
+        seed_rel_l2=pred, seed_rel_l2_median=float(np.median(pred[both])) if both.any() else None,
         predict_s=warm["predict_s"])
 
 
-__all__ = ["GRID", "IN_CH", "encode", "field_to_grid", "grid_to_field", "steady_states", "train",
+__all__ = ["GRID", "IN_CH", "DILATION", "encode", "field_to_grid", "grid_to_field", "steady_states", "train",
            "warm_fields", "packed_field", "compare_starts"]

@@ -33,22 +33,23 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("H,W,levels,ch", [(37, 37, 4, 3), (20, 27, 3, 7)])
-def test_forward_and_gradients_match_oracle(H, W, levels, ch):
-    print("net vs oracle", H, W, levels, ch, flush=True)
-    net = lbm.Net(H, W, batch=2, base=32, levels=levels, seed=1, in_channels=ch)
+@pytest.mark.parametrize("H,W,levels,ch,dil", [(37, 37, 4, 3, 1), (20, 27, 3, 7, 1),
+                                                (65, 57, 4, 7, 2)])
+def test_forward_and_gradients_match_oracle(H, W, levels, ch, dil):
+    print("net vs oracle", H, W, levels, ch, dil, flush=True)
+    net = lbm.Net(H, W, batch=2, base=32, levels=levels, seed=1, in_channels=ch, dilation=dil)
     p = net.params()
     assert p.size == NN.num_params(levels, 32)
     w0 = NN.unpack(p, levels, 32)[0][0][0]
     assert not w0[:, :, ch:].any() and w0[:, :, :ch].any()  # padding channels start at 0
     inp, tgt = _sample(2, H, W, 2, ch)
     y = net.predict(inp)
-    y_ref = NN.forward(p.astype(np.float32), inp, levels, 32)
+    y_ref = NN.forward(p.astype(np.float32), inp, levels, 32, dil=dil)
     e = _rel(y, y_ref)
     print("forward rel L2", e, flush=True)
     assert e < 3e-2
     loss, g = net.gradients(inp, tgt)
-    loss_ref, g_ref = NN.loss_and_grad(p.astype(np.float32), inp, tgt, levels, 32)
+    loss_ref, g_ref = NN.loss_and_grad(p.astype(np.float32), inp, tgt, levels, 32, dil)
     assert abs(loss - loss_ref) <= 3e-2 * loss_ref
     convs, head = NN.unpack(g, levels, 32)
     convs_r, head_r = NN.unpack(g_ref, levels, 32)
@@ -85,9 +86,10 @@ def test_training_reduces_loss_and_save_load(tmp_path):
     other.load(str(tmp_path / "w.bin"))
     assert np.array_equal(other.params(), p0)
     assert np.array_equal(other.predict(inp[:2]), net.predict(inp[:2]))
-    bad = lbm.Net(H, W, batch=4, base=32, levels=3)
-    with pytest.raises(lbm.LbmError):
-        bad.load(str(tmp_path / "w.bin"))
+    for bad in (lbm.Net(H, W, batch=4, base=32, levels=3),
+                lbm.Net(H, W, batch=4, base=32, levels=4, dilation=2)):
+        with pytest.raises(lbm.LbmError):
+            bad.load(str(tmp_path / "w.bin"))
 
 
 def test_full_size_grid_runs():

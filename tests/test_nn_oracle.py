@@ -33,9 +33,10 @@ def test_layer_adjoints(shape):
     rng = np.random.default_rng(1)
     H, W, C = shape
     x = rng.normal(size=(2, H, W, C))
-    p = NN._patches(x)
-    g = rng.normal(size=p.shape)
-    assert np.isclose((p * g).sum(), (x * NN._col2im(g)).sum())
+    for d in (1, 2, 4):
+        p = NN._patches(x, d)
+        g = rng.normal(size=p.shape)
+        assert np.isclose((p * g).sum(), (x * NN._col2im(g, d)).sum())
     y, cnt = NN.pool(x)
     g = rng.normal(size=y.shape)
     assert np.isclose((y * g).sum(), (x * NN.pool_bwd(g, cnt, H, W)).sum())
@@ -44,18 +45,19 @@ def test_layer_adjoints(shape):
     assert np.isclose((NN.up2(xc, H, W) * g).sum(), (xc * NN.up2_bwd(g, *xc.shape[1:3])).sum())
 
 
-def test_conv_matches_direct_sum():
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_conv_matches_direct_sum(d):
     rng = np.random.default_rng(2)
-    x = rng.normal(size=(1, 4, 5, 3))
+    x = rng.normal(size=(1, 6, 7, 3))
     w = rng.normal(size=(2, 9, 3))
     b = rng.normal(size=2)
-    y, _ = NN.conv(x, w, b)
-    for yy in range(4):
-        for xx in range(5):
+    y, _ = NN.conv(x, w, b, d)
+    for yy in range(6):
+        for xx in range(7):
             ref = b.copy()
             for t in range(9):
-                dy, dx = t // 3 - 1, t % 3 - 1
-                if 0 <= yy + dy < 4 and 0 <= xx + dx < 5:
+                dy, dx = d * (t // 3 - 1), d * (t % 3 - 1)
+                if 0 <= yy + dy < 6 and 0 <= xx + dx < 7:
                     ref += w[:, t] @ x[0, yy + dy, xx + dx]
             assert np.allclose(y[0, yy, xx], ref)
 
@@ -77,11 +79,12 @@ def test_zero_convs_give_the_head_bias():
     assert np.allclose(y, bh)
 
 
-def test_backward_matches_finite_differences():
+@pytest.mark.parametrize("dil", [1, 2])
+def test_backward_matches_finite_differences(dil):
     L, c = 3, 32
     p = NN.init_params(L, c, seed=4)
-    inp, tgt = _sample(2, 7, 6, 5)
-    loss, g = NN.loss_and_grad(p, inp, tgt, L, c)
+    inp, tgt = _sample(2, 11, 10, 5)
+    loss, g = NN.loss_and_grad(p, inp, tgt, L, c, dil)
     rng = np.random.default_rng(6)
     # coordinates from every layer, the padded input channels excluded
     offs, off = [], 0
@@ -97,9 +100,9 @@ def test_backward_matches_finite_differences():
     for k in offs:
         q = p.copy()
         q[k] += eps
-        lp = NN.loss_and_grad(q, inp, tgt, L, c)[0]
+        lp = NN.loss_and_grad(q, inp, tgt, L, c, dil)[0]
         q[k] -= 2 * eps
-        lm = NN.loss_and_grad(q, inp, tgt, L, c)[0]
+        lm = NN.loss_and_grad(q, inp, tgt, L, c, dil)[0]
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[k]) <= 1e-6 + 1e-4 * abs(fd), (k, fd, g[k])
     # the padded input channels carry no gradient
