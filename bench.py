@@ -93,7 +93,7 @@ def main():
                     help="sweep: cavities also run one at a time")
     ap.add_argument("--train-count", type=int, default=2000, help="warmstart: training cavities")
     ap.add_argument("--test-count", type=int, default=200, help="warmstart: held-out cavities")
-    ap.add_argument("--epochs", type=int, default=60, help="warmstart: training epochs")
+    ap.add_argument("--epochs", type=int, default=120, help="warmstart: training epochs")
     ap.add_argument("--batch", type=int, default=16, help="warmstart: training batch")
     ap.add_argument("--lr", type=float, default=1e-3, help="warmstart: peak learning rate")
     ap.add_argument("--data-max-steps", type=int, default=200000,

@@ -146,11 +146,22 @@ def steady_states(specs, opts=None, chunk: int = 5000, progress=None):
     return fields, r["conv_iter"], r
 
 
+def channel_scales(inputs: np.ndarray, targets: np.ndarray) -> np.ndarray:
+    """RMS of each target channel over the fluid points of a set: the
+    network learns targets / scale, so u, v and rho weigh alike in the loss
+    (the density channel's corner peaks would otherwise dominate it)."""
+    m = inputs[..., 0:1]
+    ms = (m * targets.astype(np.float64) ** 2).sum(axis=(0, 1, 2)) / max(1.0, float(m.sum()))
+    return np.sqrt(np.maximum(ms, 1e-12)).astype(np.float32)
+
+
 def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
           lr: float = 1e-3, seed: int = 0, log=None) -> list[float]:
-    """Adam over the set with a cosine learning-rate decay; returns the mean
-    loss of every epoch."""
-    net.set_data(inputs, targets)
+    """Adam over the set with a cosine learning-rate decay on targets scaled
+    per channel (``channel_scales``, kept as ``net.target_scales``); returns
+    the mean loss of every epoch."""
+    net.target_scales = channel_scales(inputs, targets)
+    net.set_data(inputs, targets / net.target_scales)
     rng = np.random.default_rng(seed)
     N, B = inputs.shape[0], net.batch
     steps_per_epoch = max(1, N // B)
@@ -170,9 +181,22 @@ def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
 
 
 def warm_fields(net: Net, specs):
-    """Network prediction of every cavity, as (rho, u, v) on its lattice."""
-    pred = net.predict(np.stack([encode(s, net.H) for s in specs]))
-    return [grid_to_field(s, p, net.H) for s, p in zip(specs, pred)]
+    """Network prediction of every cavity, as (rho, u, v) on its lattice.
+    The predicted density is shifted so the cavity holds the mass of the rest
+    start (mean rho = 1 over its fluid nodes), as the walls conserve mass."""
+    scales = getattr(net, "target_scales", np.ones(3, np.float32))
+    pred = net.predict(np.stack([encode(s, net.H) for s in specs])) * scales
+    out = []
+    for s, p in zip(specs, pred):
+        rho, u, v = grid_to_field(s, p, net.H)
+        NX, NY = s["NX"], s["NY"]
+        fl = np.zeros((NY + 3, NX + 2), bool)
+        fl[1:NY + 1, 1:NX + 1] = _fluid_mask(s)
+        fl = fl.ravel()
+        rho = rho.copy()
+        rho[fl] -= rho[fl].mean() - 1.0
+        out.append((rho, u, v))
+    return out
 
 
 def packed_field(packed: Case, fields):
@@ -261,5 +285,6 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         predict_s=warm["predict_s"])
 
 
-__all__ = ["GRID", "IN_CH", "DILATION", "encode", "field_to_grid", "grid_to_field", "steady_states", "train",
+__all__ = ["GRID", "IN_CH", "DILATION", "encode", "field_to_grid", "grid_to_field", "steady_states",
+           "channel_scales", "train",
            "warm_fields", "packed_field", "compare_starts"]

@@ -67,3 +67,15 @@ def test_resampling_of_a_smooth_field_at_another_scale():
     err = np.abs(u2 - u)[inner.ravel()].max()
     # interpolation twice at 0.54-node spacing, within one grid cell of the edges
     assert err < 2e-3
+
+
+def test_channel_scales_are_masked_rms():
+    inp = np.zeros((2, 4, 4, WS.IN_CH), np.float32)
+    inp[0, :2, :, 0] = 1  # 8 fluid points in sample 0, none in sample 1
+    t = np.zeros((2, 4, 4, 3), np.float32)
+    t[0, :2, :, 0] = 3.0
+    t[0, :2, :, 1] = np.array([1.0, -1.0, 1.0, -1.0])
+    t[0, :2, :, 2] = 0.5
+    t[1] = 100.0  # outside the fluid: ignored
+    s = WS.channel_scales(inp, t)
+    assert np.allclose(s, [3.0, 1.0, 0.5])
