@@ -34,7 +34,7 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("H,W,levels,ch,dil", [(37, 37, 4, 3, 1), (20, 27, 3, 7, 1),
-                                                (65, 57, 4, 7, 2)])
+                                                (65, 57, 4, 7, 2), (129, 129, 4, 7, 4)])
 def test_forward_and_gradients_match_oracle(H, W, levels, ch, dil):
     print("net vs oracle", H, W, levels, ch, dil, flush=True)
     net = lbm.Net(H, W, batch=2, base=32, levels=levels, seed=1, in_channels=ch, dilation=dil)
