@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=120, help="warmstart: training epochs")
     ap.add_argument("--batch", type=int, default=16, help="warmstart: training batch")
     ap.add_argument("--lr", type=float, default=1e-3, help="warmstart: peak learning rate")
+    ap.add_argument("--save-net", default="", help="warmstart: write the trained weights here")
     ap.add_argument("--data-max-steps", type=int, default=200000,
                     help="warmstart: step cap of the training-data runs (unconverged dropped)")
     args = ap.parse_args()
@@ -408,6 +409,9 @@ def bench_warmstart(args):
     t1 = time.perf_counter()
     hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log)
     t_train = time.perf_counter() - t1
+    if args.save_net:
+        net.save(args.save_net)
+        np.save(args.save_net + ".scales.npy", net.target_scales)
     del X, T
     test = SW.random_cavities(args.test_count, seed=2, side=(lo, hi), re=(100.0, 5000.0))
     r = WS.compare_starts(net, test, opts, log=log)
@@ -451,6 +455,8 @@ def bench_warmstart(args):
             "predict_s": round(r["predict_s"], 3),
             "field_rel_l2_median": r["field_rel_l2_median"],
             "seed_rel_l2_median": r["seed_rel_l2_median"],
+            "heldout_mse_u_v_rho": [round(float(x), 5) for x in r["heldout_mse"]],
+            "target_scales": [round(float(x), 5) for x in net.target_scales],
             "field_rel_l2_max": float(r["field_rel_l2"][both].max()) if both.any() else None,
             "loss_first_last": [hist[0], hist[-1]],
             "data_s": round(t_data, 1),

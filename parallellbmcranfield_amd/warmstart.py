@@ -246,9 +246,10 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         s.sync()
         wall = time.perf_counter() - t1
         it, sec, _ = s.batch_status()
-        _, u, v = s.macro()
+        rho, u, v = s.macro()
         s.close()
-        return dict(conv_iter=it, conv_s=sec + pre, wall_s=wall + pre, predict_s=pre, u=u, v=v)
+        return dict(conv_iter=it, conv_s=sec + pre, wall_s=wall + pre, predict_s=pre, rho=rho, u=u,
+                    v=v)
 
     rest = run("rest")
     if log:
@@ -257,9 +258,12 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
     if log:
         log(f"warm start: {warm['wall_s']:.1f} s (prediction {warm['predict_s']:.2f} s)")
     both = (rest["conv_iter"] > 0) & (warm["conv_iter"] > 0)
-    def rel(a_u, a_v, b_u, b_v, sl):
-        ur, vr = (a.reshape(packed.m, packed.n)[sl] for a in (b_u, b_v))
-        uw, vw = (a.reshape(packed.m, packed.n)[sl] for a in (a_u, a_v))
+    fluid = packed.cells()["fluid"].reshape(packed.m, packed.n) == 1
+
+    def rel(a_u, a_v, b_u, b_v, sl):  # relative L2 over the instance's fluid nodes
+        f = fluid[sl]
+        ur, vr = (a.reshape(packed.m, packed.n)[sl][f] for a in (b_u, b_v))
+        uw, vw = (a.reshape(packed.m, packed.n)[sl][f] for a in (a_u, a_v))
         num = np.sqrt(((uw - ur) ** 2 + (vw - vr) ** 2).sum())
         den = np.sqrt((ur ** 2 + vr ** 2).sum())
         return num / den if den > 0 else 0.0
@@ -271,6 +275,23 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         diff.append(rel(warm["u"], warm["v"], rest["u"], rest["v"], sl))
         pred.append(rel(seeds["u"], seeds["v"], rest["u"], rest["v"], sl))
     diff, pred = np.asarray(diff), np.asarray(pred)
+    # held-out loss per channel, in the network's scaled units, against the
+    # rest start's converged fields
+    scales = getattr(net, "target_scales", np.ones(3, np.float32))
+    X = np.stack([encode(sp, net.H) for sp in specs])
+    P = net.predict(X)
+    mse = np.zeros(3)
+    wsum = 0.0
+    rest_rho = rest["rho"]
+    for k, sp in enumerate(specs):
+        x0, y0, w, h = packed.rects[k]
+        sl = (slice(y0, y0 + h), slice(x0, x0 + w))
+        T = field_to_grid(sp, *(a.reshape(packed.m, packed.n)[sl].ravel()
+                                for a in (rest_rho, rest["u"], rest["v"])), net.H) / scales
+        m = X[k, ..., 0:1]
+        mse += (m * (P[k] - T) ** 2).sum(axis=(0, 1))
+        wsum += float(m.sum())
+    heldout = mse / max(1.0, wsum)
     return dict(
         n=len(specs), both_converged=int(both.sum()),
         steps_rest=rest["conv_iter"], steps_warm=warm["conv_iter"],
@@ -282,6 +303,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         wall_rest=rest["wall_s"], wall_warm=warm["wall_s"],
         field_rel_l2=diff, field_rel_l2_median=float(np.median(diff[both])) if both.any() else None,
         seed_rel_l2=pred, seed_rel_l2_median=float(np.median(pred[both])) if both.any() else None,
+        heldout_mse=heldout,
         predict_s=warm["predict_s"])
 
 
