@@ -81,10 +81,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--workload", default="cavity",
-                    choices=["cavity", "sweep", "warmstart", "netstep"],
+                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix"],
                     help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
                          "warmstart: learned start vs rest start on held-out cavities; "
-                         "netstep: training steps of the warm-start network")
+                         "netstep: training steps of the warm-start network; "
+                         "matrix: every collision x storage at --sizes, error vs FP64")
     ap.add_argument("--count", type=int, default=256, help="sweep: number of cavities")
     ap.add_argument("--side", default="65,513", help="sweep: side-length range")
     ap.add_argument("--crit", type=float, default=1e-6, help="sweep: residual criterion")
@@ -96,6 +97,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=120, help="warmstart: training epochs")
     ap.add_argument("--batch", type=int, default=16, help="warmstart: training batch")
     ap.add_argument("--lr", type=float, default=1e-3, help="warmstart: peak learning rate")
+    ap.add_argument("--sizes", default="1025,4097", help="matrix: cavity sizes")
     ap.add_argument("--save-net", default="", help="warmstart: write the trained weights here")
     ap.add_argument("--data-max-steps", type=int, default=200000,
                     help="warmstart: step cap of the training-data runs (unconverged dropped)")
@@ -112,6 +114,8 @@ def main():
         return bench_warmstart(args)
     if args.workload == "netstep":
         return bench_netstep(args)
+    if args.workload == "matrix":
+        return bench_matrix(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -309,6 +313,74 @@ def bench_sweep(args):
             "bytes_per_update": bytes_per_update,
         },
         "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_matrix(args):
+    """The cavity at each of --sizes (Re = 1000 on L = N - 1, lid 0.1, rest
+    start) for BGK, TRT (magic parameter 1/4) and MRT (Lallemand-Luo rates,
+    1/tau on the stress moments) in FP64, FP32 and FP16 storage: --steps
+    timed steps each, MLUPS, step-kernel HBM fraction, and the velocity
+    relative L2 error of the reduced-precision fields against the FP64 run
+    of the same model after the same steps."""
+    import numpy as np
+    import parallellbmcranfield_amd as lbm
+    rows = []
+    for N in (int(x) for x in args.sizes.split(",")):
+        case = lbm.Case.cavity(N)
+        nf = case.info.num_fluid
+        visc = 0.1 * (N - 1) / 1000.0
+        tau = 3 * float(np.float32(visc)) + 0.5
+        rates = [0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, 1 / tau, 1 / tau]
+        for name, coll in (("bgk", lbm.BGKW), ("trt", lbm.TRT), ("mrt", lbm.MRT)):
+            ref = None
+            for pname, prec, esize in (("f64", lbm.PREC_F64, 8), ("f32", lbm.PREC_F32, 4),
+                                       ("f16", lbm.PREC_F16, 2)):
+                kw = dict(collision=coll, uavg=0.1, rho_ini=1.0, viscosity=visc,
+                          init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE, use_stop_rule=0)
+                if coll == lbm.MRT:
+                    kw.update(mrt_textbook_meq=1, mrt_rates=rates)
+                if coll == lbm.TRT:
+                    kw.update(trt_lambda=0.25, trt_fp64_parts=1)
+                o = lbm.make_options(**kw)
+                o.precision = prec
+                s = lbm.Solver(case, o)
+                s.step(args.warmup)
+                s.sync()
+                t0 = time.perf_counter()
+                _, kern_ms = s.step_timed(args.steps, args.sample_every)
+                s.sync()
+                wall = time.perf_counter() - t0
+                _, u, v = s.macro()
+                s.close()
+                if ref is None:
+                    ref = (u, v)
+                    err = 0.0
+                else:
+                    err = float(np.sqrt(((u - ref[0]) ** 2 + (v - ref[1]) ** 2).sum() /
+                                        (ref[0] ** 2 + ref[1] ** 2).sum()))
+                gbs = 2 * 9 * esize * nf / (kern_ms / 1e3) / 1e9
+                row = dict(N=N, collision=name, storage=pname,
+                           mlups=round(nf * args.steps / wall / 1e6, 1),
+                           kernel_us=round(kern_ms * 1e3, 2), hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
+                           vel_rel_l2_vs_f64=err)
+                rows.append(row)
+                log(f"matrix: {row}")
+            del ref
+        del case
+    head = [r for r in rows if r["N"] == max(r_["N"] for r_ in rows) and r["collision"] == "mrt"
+            and r["storage"] == "f32"]
+    out = {
+        "metric": "MLUPS per collision x storage, velocity error vs FP64",
+        "value": head[0]["mlups"] if head else None, "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64/f32/f16",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavities {args.sizes}, Re=1000, lid 0.1, rest start, "
+                               "BGK / TRT(1/4) / MRT(Lallemand-Luo) x FP64 / FP32 / FP16 storage",
+                   "parallelism": "single GPU"},
+        "rows": rows, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -34,6 +34,7 @@ using lbmhost::set_error;
 namespace {
 constexpr int CIN0 = 32;  // padded input channels
 constexpr int KOUT = 3;   // u, v, rho
+constexpr int RED_CHUNK = 1024;  // pixels per workgroup of the bias / head-gradient partials
 struct Conv {
   int C0, C1, Cout, level, dil;
   size_t w, b;  // offsets of W and bias in the parameter vector
@@ -219,9 +220,9 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channel
     s.C1 = c.C1;
     const int ch = wgrad_chunks(s, seg_per_chunk(n, c, s));
     cap = std::max(cap, (size_t)ch * c.Cout * 9 * (c.C0 + c.C1));
-    cap = std::max(cap, (size_t)((P_of(n, c.level, B) + 4095) / 4096) * c.Cout);
+    cap = std::max(cap, (size_t)((P_of(n, c.level, B) + RED_CHUNK - 1) / RED_CHUNK) * c.Cout);
   }
-  cap = std::max(cap, (size_t)((P0 + 4095) / 4096) * (KOUT * base + KOUT));
+  cap = std::max(cap, (size_t)((P0 + RED_CHUNK - 1) / RED_CHUNK) * (KOUT * base + KOUT));
   n->part_cap = cap;
   if (!rc) rc = nalloc_t(n, &n->part, cap);
   if (rc) {
@@ -341,8 +342,8 @@ static int conv_grads(lbm_net* n, const Conv& c, const ConvSrc& s, const u16* go
   NC(wgrad(s, gout, c.Cout, n->part, spc, n->st));
   NC(reduce_chunks(n->part, ch, nw, n->g + c.w, n->st));
   const int P = s.B * s.H * s.W;
-  NC(colsum_part(gout, P, c.Cout, 4096, n->part, n->st));
-  NC(reduce_chunks(n->part, (P + 4095) / 4096, c.Cout, n->g + c.b, n->st));
+  NC(colsum_part(gout, P, c.Cout, RED_CHUNK, n->part, n->st));
+  NC(reduce_chunks(n->part, (P + RED_CHUNK - 1) / RED_CHUNK, c.Cout, n->g + c.b, n->st));
   return LBM_OK;
 }
 
@@ -361,8 +362,8 @@ static int backward(lbm_net* n, int B, double fluid, double* loss) {
   NC(mse(n->y, n->t, n->mask, n->dy, n->lpart, P0, KOUT, inv_norm, n->st));
   // head
   NC(head_bwd(n->dy, n->p + n->head_w, n->dd[0], n->gdd[0], P0, n->base, KOUT, n->st));
-  const int hch = (P0 + 4095) / 4096;
-  NC(head_wgrad_part(n->dy, n->dd[0], n->part, P0, n->base, KOUT, 4096, n->st));
+  const int hch = (P0 + RED_CHUNK - 1) / RED_CHUNK;
+  NC(head_wgrad_part(n->dy, n->dd[0], n->part, P0, n->base, KOUT, RED_CHUNK, n->st));
   NC(reduce_chunks(n->part, hch, (long)KOUT * n->base + KOUT, n->g + n->head_w, n->st));
   // decoder, fine to coarse
   const size_t ndec0 = 2 * (size_t)L;  // index of the first decoder conv (coarsest level)

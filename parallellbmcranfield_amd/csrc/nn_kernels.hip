@@ -209,13 +209,32 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
 }
 
 // part[chunk][c] = sum over the chunk's pixels of g[p][c] (bias gradient).
+// Threads are (lane, channel) pairs, lane = t / C over 256 / C pixel lanes,
+// so a wave reads whole pixel rows; lanes are summed in a fixed order in LDS.
 __global__ __launch_bounds__(256) void k_colsum_part(const bf16* __restrict__ g, int P, int C,
                                                      int px_per_chunk, float* __restrict__ part) {
+  __shared__ float red[256];
   const int p0 = blockIdx.x * px_per_chunk, p1 = min(P, p0 + px_per_chunk);
-  for (int c = threadIdx.x; c < C; c += 256) {
-    float s = 0.0f;
-    for (int p = p0; p < p1; p++) s += (float)g[(size_t)p * C + c];
-    part[(size_t)blockIdx.x * C + c] = s;
+  const int lanes = C <= 256 ? 256 / C : 1;
+  const int t = threadIdx.x;
+  if (C > 256) {  // wide layers: one thread per channel
+    for (int c = t; c < C; c += 256) {
+      float s = 0.0f;
+      for (int p = p0; p < p1; p++) s += (float)g[(size_t)p * C + c];
+      part[(size_t)blockIdx.x * C + c] = s;
+    }
+    return;
+  }
+  const int c = t % C, lane = t / C;
+  float s = 0.0f;
+  if (lane < lanes)
+    for (int p = p0 + lane; p < p1; p += lanes) s += (float)g[(size_t)p * C + c];
+  red[t] = s;
+  __syncthreads();
+  if (t < C) {
+    float r = 0.0f;
+    for (int l = 0; l < lanes; l++) r += red[l * C + t];
+    part[(size_t)blockIdx.x * C + t] = r;
   }
 }
 
@@ -380,21 +399,44 @@ __global__ void k_head_bwd(const float* __restrict__ dy, const float* __restrict
   dx[i] = (__bf16)s;
 }
 
-// part[chunk][k*C + c] (weights) and part[chunk][K*C + k] (bias) of the head.
-__global__ void k_head_wgrad_part(const float* __restrict__ dy, const bf16* __restrict__ x,
-                                  float* __restrict__ part, int P, int C, int K, int px_per_chunk) {
+// part[chunk][k*C + c] (weights) and part[chunk][K*C + k] (bias) of the head
+// (K = 3 outputs, C <= 256 inputs).  Threads are (lane, channel) pairs as in
+// k_colsum_part; channel-0 threads also sum the bias terms.
+__global__ __launch_bounds__(256) void k_head_wgrad_part(const float* __restrict__ dy,
+                                                         const bf16* __restrict__ x,
+                                                         float* __restrict__ part, int P, int C,
+                                                         int K, int px_per_chunk) {
+  __shared__ float red[256][7];
   const int p0 = blockIdx.x * px_per_chunk, p1 = min(P, p0 + px_per_chunk);
-  const int nw = K * C + K;
-  for (int e = threadIdx.x; e < nw; e += blockDim.x) {
-    float s = 0.0f;
-    if (e < K * C) {
-      const int k = e / C, c = e % C;
-      for (int p = p0; p < p1; p++) s += dy[(size_t)p * K + k] * (float)x[(size_t)p * C + c];
-    } else {
-      const int k = e - K * C;
-      for (int p = p0; p < p1; p++) s += dy[(size_t)p * K + k];
+  const int lanes = 256 / C;
+  const int t = threadIdx.x, c = t % C, lane = t / C;
+  float a[3] = {0.f, 0.f, 0.f}, bsum[3] = {0.f, 0.f, 0.f};
+  if (lane < lanes)
+    for (int p = p0 + lane; p < p1; p += lanes) {
+      const float xv = (float)x[(size_t)p * C + c];
+      for (int k = 0; k < K; k++) {
+        const float d = dy[(size_t)p * K + k];
+        a[k] += d * xv;
+        bsum[k] += d;
+      }
     }
-    part[(size_t)blockIdx.x * nw + e] = s;
+  for (int k = 0; k < 3; k++) {
+    red[t][k] = a[k];
+    red[t][3 + k] = bsum[k];
+  }
+  __syncthreads();
+  const int nw = K * C + K;
+  if (t < C) {
+    for (int k = 0; k < K; k++) {
+      float r = 0.0f;
+      for (int l = 0; l < lanes; l++) r += red[l * C + t][k];
+      part[(size_t)blockIdx.x * nw + k * C + t] = r;
+    }
+  }
+  if (t < K) {  // bias: the channel-0 threads of every lane saw each pixel once
+    float r = 0.0f;
+    for (int l = 0; l < lanes; l++) r += red[l * C][3 + t];
+    part[(size_t)blockIdx.x * nw + K * C + t] = r;
   }
 }
 
@@ -540,7 +582,8 @@ hipError_t head_bwd(const float* dy, const float* w, const u16* x, u16* dx, int 
 
 hipError_t head_wgrad_part(const float* dy, const u16* x, float* part, int P, int C, int K,
                            int px_per_chunk, hipStream_t st) {
-  k_head_wgrad_part<<<(unsigned)((P + px_per_chunk - 1) / px_per_chunk), 128, 0, st>>>(
+  if (C > 256 || 256 % C || K > 3) return hipErrorInvalidValue;
+  k_head_wgrad_part<<<(unsigned)((P + px_per_chunk - 1) / px_per_chunk), 256, 0, st>>>(
       dy, cb(x), part, P, C, K, px_per_chunk);
   return hipGetLastError();
 }
