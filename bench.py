@@ -487,6 +487,7 @@ def bench_warmstart(args):
     del X, T
     test = SW.random_cavities(args.test_count, seed=2, side=(lo, hi), re=(100.0, 5000.0))
     r = WS.compare_starts(net, test, opts, log=log)
+    r1 = WS.compare_starts(net, test, opts, log=log, density=False, run_rest=r["rest_run"])
     both = (r["steps_rest"] > 0) & (r["steps_warm"] > 0)
     ratio_t = r["median_time_rest"] / r["median_time_warm"] if both.any() else None
     log(f"warmstart: median steps ratio {r['median_steps_ratio']}, median time ratio {ratio_t}")
@@ -526,6 +527,8 @@ def bench_warmstart(args):
             "batch_wall_warm_s": round(r["wall_warm"], 2),
             "predict_s": round(r["predict_s"], 3),
             "field_rel_l2_median": r["field_rel_l2_median"],
+            "uniform_density_seed": {"median_steps_ratio": r1["median_steps_ratio"],
+                                     "median_time_warm_s": r1["median_time_warm"]},
             "seed_rel_l2_median": r["seed_rel_l2_median"],
             "heldout_mse_u_v_rho": [round(float(x), 5) for x in r["heldout_mse"]],
             "target_scales": [round(float(x), 5) for x in net.target_scales],

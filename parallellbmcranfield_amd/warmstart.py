@@ -180,10 +180,11 @@ def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
     return hist
 
 
-def warm_fields(net: Net, specs):
+def warm_fields(net: Net, specs, density: bool = True):
     """Network prediction of every cavity, as (rho, u, v) on its lattice.
     The predicted density is shifted so the cavity holds the mass of the rest
-    start (mean rho = 1 over its fluid nodes), as the walls conserve mass."""
+    start (mean rho = 1 over its fluid nodes), as the walls conserve mass;
+    density=False seeds rho = 1 with the predicted velocity."""
     scales = getattr(net, "target_scales", np.ones(3, np.float32))
     pred = net.predict(np.stack([encode(s, net.H) for s in specs])) * scales
     out = []
@@ -195,6 +196,8 @@ def warm_fields(net: Net, specs):
         fl = fl.ravel()
         rho = rho.copy()
         rho[fl] -= rho[fl].mean() - 1.0
+        if not density:
+            rho[:] = 1.0
         out.append((rho, u, v))
     return out
 
@@ -210,14 +213,16 @@ def packed_field(packed: Case, fields):
 
 
 def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
-                   neq: bool = True) -> dict:
+                   neq: bool = True, density: bool = True, run_rest=None) -> dict:
     """Held-out comparison: the same cavities from rest and from the network's
     prediction, each as one batched run to the criterion.  Per instance:
     steps and wall time to convergence (the warm start's time includes the
     prediction), and the relative L2 difference of the two converged
     velocity fields and of the seed against the rest start's converged
     velocity (the prediction error).  neq: seed feq plus the Chapman-Enskog non-equilibrium
-    part of the predicted velocity gradients (LBM_INIT_FIELD_NEQ)."""
+    part of the predicted velocity gradients (LBM_INIT_FIELD_NEQ).  density:
+    seed the predicted density (else rho = 1).  run_rest: the ``rest_run`` of
+    an earlier comparison on the same cavities, reused."""
     opts = opts or SW.sweep_options()
     packed, fl = SW.build_batch(specs)
     inst = [(sp["nu"], sp["lid"]) for sp in specs]
@@ -228,7 +233,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         t0 = time.perf_counter()
         pre = 0.0
         if field == "warm":
-            flds = warm_fields(net, specs)
+            flds = warm_fields(net, specs, density)
             fld = packed_field(packed, flds)
             pre = time.perf_counter() - t0
             seeds["u"], seeds["v"] = fld[1], fld[2]
@@ -251,7 +256,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         return dict(conv_iter=it, conv_s=sec + pre, wall_s=wall + pre, predict_s=pre, rho=rho, u=u,
                     v=v)
 
-    rest = run("rest")
+    rest = run_rest if run_rest is not None else run("rest")
     if log:
         log(f"rest start: {rest['wall_s']:.1f} s")
     warm = run("warm")
@@ -301,6 +306,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         median_time_rest=float(np.median(rest["conv_s"][both])) if both.any() else None,
         median_time_warm=float(np.median(warm["conv_s"][both])) if both.any() else None,
         wall_rest=rest["wall_s"], wall_warm=warm["wall_s"],
+        rest_run=rest,
         field_rel_l2=diff, field_rel_l2_median=float(np.median(diff[both])) if both.any() else None,
         seed_rel_l2=pred, seed_rel_l2_median=float(np.median(pred[both])) if both.any() else None,
         heldout_mse=heldout,
