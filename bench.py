@@ -81,11 +81,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--workload", default="cavity",
-                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix"],
+                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix", "config0"],
                     help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
                          "warmstart: learned start vs rest start on held-out cavities; "
                          "netstep: training steps of the warm-start network; "
-                         "matrix: every collision x storage at --sizes, error vs FP64")
+                         "matrix: every collision x storage at --sizes, error vs FP64; "
+                         "config0: the 129^2 Re=100 cavity beside the reference program")
     ap.add_argument("--count", type=int, default=256, help="sweep: number of cavities")
     ap.add_argument("--side", default="65,513", help="sweep: side-length range")
     ap.add_argument("--crit", type=float, default=1e-6, help="sweep: residual criterion")
@@ -116,6 +117,8 @@ def main():
         return bench_netstep(args)
     if args.workload == "matrix":
         return bench_matrix(args)
+    if args.workload == "config0":
+        return bench_config0(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -314,6 +317,84 @@ def bench_sweep(args):
         },
         "cpu_baseline": None,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_config0(args):
+    """The shipped-convention lid-driven cavity with 129 x 129 fluid nodes,
+    BGK, FP64, lid u = 0.1, Re = 100 (nu = 0.128), rest start, up to 10 000
+    steps or velocity residual 1e-8 (checked every 100 steps) on the GPU;
+    beside it the reference program built from its own sources
+    (oracle/_ref/LBMSolver, 1 host thread) on the same mesh for the same
+    number of steps (its own start and residual; same work per step)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import parallellbmcranfield_amd as lbm
+    N, steps_cap = 129, 10000
+    case = lbm.Case.cavity(N)
+    o = lbm.make_options(collision=lbm.BGKW, uavg=0.1, rho_ini=1.0, viscosity=0.128,
+                         init=lbm.INIT_REST, resid_kind=lbm.RESID_VELOCITY, resid_every=100,
+                         crit_v=1e-8, use_stop_rule=1, max_iterations=steps_cap)
+    o.precision = lbm.PREC_F64
+    s = lbm.Solver(case, o)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps_cap)
+    s.sync()
+    wall = time.perf_counter() - t0
+    steps = s.iter
+    res = float(s.residuals()[0])
+    nf = case.info.num_fluid
+    # step-kernel duration on the same lattice (plain steps, events)
+    o2 = lbm.make_options(collision=lbm.BGKW, uavg=0.1, rho_ini=1.0, viscosity=0.128,
+                          init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE, use_stop_rule=0)
+    o2.precision = lbm.PREC_F64
+    s2 = lbm.Solver(case, o2)
+    s2.step(50)
+    _, kern_ms = s2.step_timed(2000, 8)
+    s2.close()
+    achieved = 2 * 9 * 8 * nf / (kern_ms / 1e3) / 1e9
+    cpu = None
+    ref = os.path.join(ROOT, "oracle", "_ref", "LBMSolver")
+    if not args.no_cpu_baseline and os.path.exists(ref):
+        d = tempfile.mkdtemp(prefix="lbm_config0_")
+        try:
+            os.makedirs(os.path.join(d, "Mesh"))
+            os.makedirs(os.path.join(d, "Results"))
+            case.write(os.path.join(d, "Mesh", "D2node.dat"),
+                       os.path.join(d, "Mesh", "BCconnectors.dat"))
+            with open(os.path.join(d, "SetUpData.ini"), "w") as fh:
+                fh.write(f"0.1\n0.0\n1.0\n0.128\n2\n1\n2\n2\n{steps}\n1000000\n1000000\n1\n0\n"
+                         "-1.0\n-1.0\n0\n")
+            log(f"config0: reference program, {steps} steps on 1 host thread ...")
+            t1 = time.perf_counter()
+            subprocess.run([ref], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           check=True, timeout=900)
+            dt = time.perf_counter() - t1
+            cpu = {"value": round(nf * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
+                   "kind": "reference",
+                   "sample": f"the reference program (serial/, built from its sources into "
+                             f"oracle/_ref) on the same 129^2 mesh, {steps} steps, BGK, "
+                             f"whole run incl. set-up and result files: {dt:.1f} s"}
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    out = {
+        "metric": "config 0: 129^2 cavity BGK FP64 Re=100, MLUPS to 10 000 steps or residual 1e-8",
+        "value": round(nf * steps / wall / 1e6, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": round(wall * 1e3 / max(1, steps), 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "lid-driven cavity 129x129 fluid nodes (reference mesh convention), "
+                               "BGK, FP64, lid u=0.1, nu=0.128 (Re=100), rest start, velocity "
+                               "residual every 100 steps", "fluid_nodes": nf,
+                   "final_velocity_residual": res, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(kern_ms * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    s.close()
     print(json.dumps(out), flush=True)
 
 
