@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the D2Q9 hot path on one MI355X.
 
-Default workload (the headline configuration): lid-driven cavity with
+Default workload: lid-driven cavity with
 4097 x 4097 fluid nodes in the reference mesh convention, Re = 1000 on L = N-1
 (nu = 0.4096, tau = 1.7288), MRT with rates s = (0, 1.4, 1.4, 0, 1.2, 0, 1.2,
 1/tau, 1/tau) and textbook equilibria, FP32 storage of f - w_k rho0, rest start,
@@ -81,12 +81,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--workload", default="cavity",
-                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix", "config0"],
+                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix", "cavity129"],
                     help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
                          "warmstart: learned start vs rest start on held-out cavities; "
                          "netstep: training steps of the warm-start network; "
                          "matrix: every collision x storage at --sizes, error vs FP64; "
-                         "config0: the 129^2 Re=100 cavity beside the reference program")
+                         "cavity129: the 129^2 Re=100 cavity beside the reference program")
     ap.add_argument("--count", type=int, default=256, help="sweep: number of cavities")
     ap.add_argument("--side", default="65,513", help="sweep: side-length range")
     ap.add_argument("--crit", type=float, default=1e-6, help="sweep: residual criterion")
@@ -117,8 +117,8 @@ def main():
         return bench_netstep(args)
     if args.workload == "matrix":
         return bench_matrix(args)
-    if args.workload == "config0":
-        return bench_config0(args)
+    if args.workload == "cavity129":
+        return bench_cavity129(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -320,7 +320,7 @@ def bench_sweep(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_config0(args):
+def bench_cavity129(args):
     """The shipped-convention lid-driven cavity with 129 x 129 fluid nodes,
     BGK, FP64, lid u = 0.1, Re = 100 (nu = 0.128), rest start, up to 10 000
     steps or velocity residual 1e-8 (checked every 100 steps) on the GPU;
@@ -358,7 +358,7 @@ def bench_config0(args):
     cpu = None
     ref = os.path.join(ROOT, "oracle", "_ref", "LBMSolver")
     if not args.no_cpu_baseline and os.path.exists(ref):
-        d = tempfile.mkdtemp(prefix="lbm_config0_")
+        d = tempfile.mkdtemp(prefix="lbm_cavity129_")
         try:
             os.makedirs(os.path.join(d, "Mesh"))
             os.makedirs(os.path.join(d, "Results"))
@@ -367,7 +367,7 @@ def bench_config0(args):
             with open(os.path.join(d, "SetUpData.ini"), "w") as fh:
                 fh.write(f"0.1\n0.0\n1.0\n0.128\n2\n1\n2\n2\n{steps}\n1000000\n1000000\n1\n0\n"
                          "-1.0\n-1.0\n0\n")
-            log(f"config0: reference program, {steps} steps on 1 host thread ...")
+            log(f"cavity129: reference program, {steps} steps on 1 host thread ...")
             t1 = time.perf_counter()
             subprocess.run([ref], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
                            check=True, timeout=900)
@@ -380,7 +380,7 @@ def bench_config0(args):
         finally:
             shutil.rmtree(d, ignore_errors=True)
     out = {
-        "metric": "config 0: 129^2 cavity BGK FP64 Re=100, MLUPS to 10 000 steps or residual 1e-8",
+        "metric": "129^2 cavity BGK FP64 Re=100: MLUPS to 10 000 steps or residual 1e-8",
         "value": round(nf * steps / wall / 1e6, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
         "warmup": 0, "ms_per_step": round(wall * 1e3 / max(1, steps), 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",

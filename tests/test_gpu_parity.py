@@ -73,7 +73,7 @@ def test_fp64_bit_exact_on_coarse(name):
 SWITCH_CASES = {
     # name: (mesh factory, oracle params, steps)
     "rest_bgk": (lambda: meshgen.cavity(20), dict(collision=1, init_rest=1), 60),
-    "rest_mrt_textbook_config_rates": (
+    "rest_mrt_textbook_lallemand_luo_rates": (
         lambda: meshgen.cavity(20),
         dict(collision=3, init_rest=1, mrt_textbook=1, mrt_s=None), 60),
     "trt_lambda_quarter_fp64": (lambda: meshgen.cavity(20),
@@ -155,7 +155,7 @@ def test_fp64_switches_match_oracle(name):
 def test_reduced_precision_velocity_error(coll, prec, tol):
     """FP32 (f - w rho0) and FP16 (scaled) storage against the FP64 oracle:
     relative L2 error of the velocity field after 500 steps on a 64^2 cavity
-    (physics mode: rest start, TRT Lambda = 1/4, MRT with the config rates)."""
+    (physics mode: rest start, TRT Lambda = 1/4, MRT with Lallemand-Luo rates)."""
     print("prec", prec, "coll", coll, flush=True)
     visc = 0.0256
     tau = 3 * np.float64(np.float32(visc)) + 0.5
@@ -246,7 +246,7 @@ def test_divergence_reports_like_reference(tmp_path):
 
 # ------------------------------------------------------- full-size properties
 def test_large_cavity_fp64_matches_oracle_first_steps():
-    """1025^2 (the bandwidth configuration's smaller size): FP64 MRT, 3 steps,
+    """1025^2 cavity: FP64 MRT, 3 steps,
     bit-identical to the oracle on every cell."""
     N = 1025
     print("cavity", N, flush=True)
@@ -267,7 +267,7 @@ def test_large_cavity_fp64_matches_oracle_first_steps():
 
 
 def test_4097_fp32_deterministic_and_close_to_fp64():
-    """4097^2 MRT (the headline configuration): two FP32 runs agree bit for bit,
+    """4097^2 cavity, MRT: two FP32 runs agree bit for bit,
     and after 200 steps FP32 stays within 1e-5 relative L2 (velocity) of FP64."""
     N = 4097
     print("cavity", N, flush=True)
@@ -291,11 +291,11 @@ def test_4097_fp32_deterministic_and_close_to_fp64():
 
 
 @pytest.mark.parametrize("init_rest", [0, 1])
-def test_config0_cavity_129_bgk_fp64_1000_steps(init_rest):
-    """The plain-cavity configuration (129x129 fluid nodes, BGK, FP64, lid 0.1,
+def test_cavity_129_bgk_fp64_1000_steps(init_rest):
+    """The plain 129x129 cavity (BGK, FP64, lid 0.1,
     nu = 0.128): after 1000 steps u, v, rho are bit-identical to the oracle
     (the 1e-12 max-abs bar holds with zero difference)."""
-    print("config0 init_rest", init_rest, flush=True)
+    print("cavity 129 init_rest", init_rest, flush=True)
     p = O.params(collision=1, uavg=0.1, rho_ini=1.0, viscosity=0.128, init_rest=init_rest)
     nodes, conn = meshgen.cavity_arrays_fast(129)
     o = O.Oracle(nodes, conn, p)
@@ -378,7 +378,7 @@ def _cylinder_run(prec, steps, every):
     return its, f, uavg, D
 
 
-def test_config2_cylinder_vortex_shedding():
+def test_cylinder_2049x513_vortex_shedding():
     """Channel 2049 x 513 with a D = 64 cylinder at (512, 260), parabolic inlet
     (u_max = 0.05), rho = 1 pressure outlet, interpolated walls, MRT, Re = 100:
     the lift oscillates with a Strouhal number in the confined-cylinder range,

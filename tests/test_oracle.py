@@ -289,7 +289,7 @@ def test_writers_match_reference_text():
         assert open(os.path.join(t, "f.csv")).read() == open(os.path.join(d, "FinalData.csv")).read()
 
 
-# ------------------------------------------------ config-2 physics (not in the reference)
+# ------------------------------------------------ channel physics (not in the reference)
 def test_pressure_outlet_closed_forms():
     """Zou-He pressure outlet: at every outlet node (BC_ID[1] = 3, not a
     corner) the post-boundary density equals rho_out and j_y = 0."""
