@@ -379,6 +379,104 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
 
+// FP16 storage, plain steps: the bulk workgroups take PAIRS of tiles
+// (2 x 256 consecutive cells) and each thread two neighbouring cells, so the
+// loads and stores of all-interior pairs move 32-bit words (2 cells) instead
+// of 16-bit halves: twice the bytes per memory instruction.  A pull with
+// cx = +-1 starts on an odd cell; its two halves come from the two aligned
+// words around it.  Same arithmetic per cell as k_step (macro_dev and
+// collide_dev on the same floats), so results are identical.  Boundary
+// workgroups and pairs with a mixed or empty tile run k_step's per-cell code.
+// Launch: nb_bnd + nb_bulk / 2 workgroups of 256; XCD grouping by runs of
+// G / 2 pairs (G even, nb_bulk a multiple of 8 G).
+__device__ inline float h2_lo(uint32_t w, float inv) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)) * inv;
+}
+__device__ inline float h2_hi(uint32_t w, float inv) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)) * inv;
+}
+
+template <int COLL>
+__global__ __launch_bounds__(256) void k_step_h2(const _Float16* __restrict__ src,
+                                                 _Float16* __restrict__ dst,
+                                                 const uint8_t* __restrict__ cls,
+                                                 const uint8_t* __restrict__ tflag,
+                                                 const BNode* __restrict__ bn, int nbn, KParams p,
+                                                 KOut o) {
+  using S = StF16;
+  using R = float;
+  const int nbb = p.nb_bnd;
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if ((int)blockIdx.x < nbb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
+      const BNode& nd = bn[i];
+      R F[9];
+      pull_masked<S>(F, nd, src, p);
+      apply_bc<S>(F, nd, src, p, nd.idx);
+      if (nd.flags & BN_FEEDER) {
+        R* side = (R*)o.side + (size_t)nd.side * 9;
+#pragma unroll
+        for (int k = 0; k < 9; k++) side[k] = F[k];
+      }
+      finish_node<S, COLL, 0, true>(F, src, dst, p, o, nd.idx, &nd, acc, -1);
+    }
+    return;
+  }
+  unsigned q = blockIdx.x - nbb;
+  if (p.xcd_group > 1) q = xcd_group_block(q, (unsigned)(p.xcd_group / 2));
+  const unsigned t0 = 2 * q;
+  if (tflag[t0] == 1 && tflag[t0 + 1] == 1) {
+    const int c0 = p.bulk_begin + (int)(t0 * 256 + 2 * threadIdx.x);  // even
+    const float inv = p.inv_scale_f;
+    const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src);
+    R F0[9], F1[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int s0 = k * p.plane + c0 - CX[k] - CY[k] * p.pitch;  // source of cell c0
+      if (CX[k] == 0) {
+        const uint32_t w = ld(s32 + (s0 >> 1));
+        F0[k] = h2_lo(w, inv);
+        F1[k] = h2_hi(w, inv);
+      } else {  // s0 odd: high half of the word before, low half of the word after
+        const uint32_t wa = ld(s32 + ((s0 - 1) >> 1));
+        const uint32_t wb = ld(s32 + ((s0 + 1) >> 1));
+        F0[k] = h2_hi(wa, inv);
+        F1[k] = h2_lo(wb, inv);
+      }
+    }
+    R out0[9], out1[9];
+    {
+      R Rh, U, V, dr;
+      macro_dev(F0, p.rho0_f, dr, Rh, U, V);
+      collide_dev<COLL>(F0, dr, Rh, U, V, coll_par(p, -1), out0);
+      macro_dev(F1, p.rho0_f, dr, Rh, U, V);
+      collide_dev<COLL>(F1, dr, Rh, U, V, coll_par(p, -1), out1);
+    }
+    uint32_t* d32 = reinterpret_cast<uint32_t*>(dst);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const uint16_t lo = __builtin_bit_cast(uint16_t, (_Float16)(out0[k] * p.scale_f));
+      const uint16_t hi = __builtin_bit_cast(uint16_t, (_Float16)(out1[k] * p.scale_f));
+      __builtin_nontemporal_store((uint32_t)lo | ((uint32_t)hi << 16),
+                                  d32 + ((k * p.plane + c0) >> 1));
+    }
+    return;
+  }
+  for (int h = 0; h < 2; h++) {  // per-cell path of k_step for each tile of the pair
+    const unsigned b = t0 + h;
+    const int tf = tflag[b];
+    if (tf == 0) continue;
+    const int idx = p.bulk_begin + (int)(b * 256 + threadIdx.x);
+    const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+    if (!active) continue;
+    R F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+    finish_node<S, COLL, 0, false>(F, src, dst, p, o, idx, nullptr, acc, -1);
+  }
+}
+
 // Open outlet (Iterate.c:533-541, 2nd order; cuda_step3 GpuBoundaries.cu:200-206,
 // 1st order): one thread walks one row's outlet nodes in increasing i, reading
 // the post-boundary F of its (i-1), (i-2) neighbours from the side buffer,

@@ -1,5 +1,6 @@
 // lbm_pass_impl.inl — grid shapes of the passes (one copy per storage unit).
 #pragma once
+#include <type_traits>
 #include "lbm_kernels.inl"
 #include "lbm_launch.hpp"
 
@@ -29,6 +30,15 @@ static hipError_t fused_launch(const Launch& L) {
   using T = typename S::T;
   const T* src = (const T*)L.src;
   T* dst = (T*)L.dst;
+  if constexpr (std::is_same_v<S, StF16> && MODE == 0) {
+    if (L.h2) {  // paired-cell FP16 plain step (nb_bulk even, grouping by G / 2 pairs)
+      const int grid2 = L.kp.nb_bnd + L.nb_bulk / 2;
+      if (grid2 <= 0) return hipSuccess;
+      k_step_h2<COLL><<<grid2, 256, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.bn, L.nbn, L.kp,
+                                                   L.out);
+      return hipGetLastError();
+    }
+  }
   const int grid = L.kp.nb_bnd + L.nb_bulk;
   if (grid <= 0) return hipSuccess;
   k_step<S, COLL, MODE, 256><<<grid, 256, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.bn, L.nbn,

@@ -85,6 +85,11 @@ Launch make_launch(lbm_solver* s, int src_buf) {
     L.nb_bulk = s->nb_list;
     L.kp.tile_list = s->d_tile_list;
   }
+  // FP16, fused, single lattice, even XCD grouping (the pair grouping must
+  // stay a permutation): paired-cell plain steps
+  L.h2 = s->prec == 2 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
+         s->kp.xcd_group % 2 == 0 && s->nb_bulk % (8 * s->kp.xcd_group) == 0 &&
+         !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
   L.out.rho = s->d_rho;
   L.out.u = s->d_u;
   L.out.v = s->d_v;
@@ -495,8 +500,10 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   const size_t sbytes = (size_t)9 * s->plane * s->elem;
   const size_t rsz = s->prec == 0 ? sizeof(double) : sizeof(float);
   rc = LBM_OK;
-  if (!rc) rc = dev_alloc(s, &s->buf[0], sbytes);
-  if (!rc) rc = dev_alloc(s, &s->buf[1], sbytes);
+  // + 256 B: the paired-cell FP16 kernel reads whole 32-bit words around a
+  // pull, which can reach one word past the last plane's last row
+  if (!rc) rc = dev_alloc(s, &s->buf[0], sbytes + 256);
+  if (!rc) rc = dev_alloc(s, &s->buf[1], sbytes + 256);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_cls, (size_t)s->plane);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_tflag, (size_t)s->nb_bulk);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_bn, sizeof(BNode) * std::max(1, s->nbn));
