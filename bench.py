@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--split-boundary", action="store_true")
     ap.add_argument("--xcd-group", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-pairs", action="store_true",
+                    help="one cell per thread (no paired-cell kernel) for FP16 / FP32")
     ap.add_argument("--sample-every", type=int, default=8,
                     help="HIP events around every N-th step-kernel launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -130,7 +132,8 @@ def main():
               resid_kind=lbm.RESID_NONE, use_stop_rule=0, block_size=args.block,
               tune_flags=(lbm.TUNE_XCD_REMAP if args.xcd_remap else 0) |
               (lbm.TUNE_SPLIT_BOUNDARY if args.split_boundary else 0) |
-              (lbm.TUNE_NO_GRAPH if args.no_graph else 0),
+              (lbm.TUNE_NO_GRAPH if args.no_graph else 0) |
+              (lbm.TUNE_NO_PAIRS if args.no_pairs else 0),
               xcd_group=args.xcd_group)
     if coll == lbm.MRT:
         kw.update(mrt_textbook_meq=1, mrt_rates=rates)
@@ -165,7 +168,7 @@ def main():
     achieved = bytes_per_launch / avg_bulk_s / 1e9
     traffic = None
     default_shape = (args.block in (0, 256) and not args.xcd_remap and not args.split_boundary
-                     and args.xcd_group in (0, 4))
+                     and args.xcd_group in (0, 4) and not args.no_pairs)
     if default_shape and os.path.exists(PMC_SUMMARY):
         try:
             pm = json.load(open(PMC_SUMMARY))

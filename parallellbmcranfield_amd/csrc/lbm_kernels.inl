@@ -379,32 +379,52 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
 
-// FP16 storage, plain steps: the bulk workgroups take PAIRS of tiles
-// (2 x 256 consecutive cells) and each thread two neighbouring cells, so the
-// loads and stores of all-interior pairs move 32-bit words (2 cells) instead
-// of 16-bit halves: twice the bytes per memory instruction.  A pull with
-// cx = +-1 starts on an odd cell; its two halves come from the two aligned
-// words around it.  Same arithmetic per cell as k_step (macro_dev and
-// collide_dev on the same floats), so results are identical.  Boundary
-// workgroups and pairs with a mixed or empty tile run k_step's per-cell code.
-// Launch: nb_bnd + nb_bulk / 2 workgroups of 256; XCD grouping by runs of
-// G / 2 pairs (G even, nb_bulk a multiple of 8 G).
-__device__ inline float h2_lo(uint32_t w, float inv) {
-  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)) * inv;
-}
-__device__ inline float h2_hi(uint32_t w, float inv) {
-  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)) * inv;
-}
+// Plain steps with FP16 or FP32 storage: the bulk workgroups take PAIRS of
+// tiles (2 x 256 consecutive cells) and each thread two neighbouring cells,
+// so the loads and stores of all-interior pairs move one 32-bit (FP16) or
+// 64-bit (FP32) word per two cells: twice the bytes per memory instruction.
+// A pull with cx = +-1 starts on an odd cell; its two halves come from the
+// two aligned words around it.  Same arithmetic per cell as k_step
+// (macro_dev and collide_dev on the same floats), so results are identical.
+// Boundary workgroups and pairs with a mixed or empty tile run k_step's
+// per-cell code.  Launch: nb_bnd + nb_bulk / 2 workgroups of 256; XCD
+// grouping by runs of G / 2 pairs (G even, nb_bulk a multiple of 8 G).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <class S>
+struct Pair;
+template <>
+struct Pair<StF16> {
+  using W = uint32_t;
+  __device__ static float lo(W w, const KParams& p) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)) * p.inv_scale_f;
+  }
+  __device__ static float hi(W w, const KParams& p) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)) * p.inv_scale_f;
+  }
+  __device__ static W pack(float a, float b, const KParams& p) {
+    const uint16_t l = __builtin_bit_cast(uint16_t, (_Float16)(a * p.scale_f));
+    const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)(b * p.scale_f));
+    return (uint32_t)l | ((uint32_t)h << 16);
+  }
+};
+template <>
+struct Pair<StF32> {
+  using W = f32x2;
+  __device__ static float lo(W w, const KParams&) { return w.x; }
+  __device__ static float hi(W w, const KParams&) { return w.y; }
+  __device__ static W pack(float a, float b, const KParams&) { return W{a, b}; }
+};
 
-template <int COLL>
-__global__ __launch_bounds__(256) void k_step_h2(const _Float16* __restrict__ src,
-                                                 _Float16* __restrict__ dst,
-                                                 const uint8_t* __restrict__ cls,
-                                                 const uint8_t* __restrict__ tflag,
-                                                 const BNode* __restrict__ bn, int nbn, KParams p,
-                                                 KOut o) {
-  using S = StF16;
+template <class S, int COLL>
+__global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restrict__ src,
+                                                   typename S::T* __restrict__ dst,
+                                                   const uint8_t* __restrict__ cls,
+                                                   const uint8_t* __restrict__ tflag,
+                                                   const BNode* __restrict__ bn, int nbn,
+                                                   KParams p, KOut o) {
   using R = float;
+  using PW = Pair<S>;
+  using W = typename PW::W;
   const int nbb = p.nb_bnd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if ((int)blockIdx.x < nbb) {
@@ -428,21 +448,18 @@ __global__ __launch_bounds__(256) void k_step_h2(const _Float16* __restrict__ sr
   const unsigned t0 = 2 * q;
   if (tflag[t0] == 1 && tflag[t0 + 1] == 1) {
     const int c0 = p.bulk_begin + (int)(t0 * 256 + 2 * threadIdx.x);  // even
-    const float inv = p.inv_scale_f;
-    const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src);
+    const W* sw = reinterpret_cast<const W*>(src);
     R F0[9], F1[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       const int s0 = k * p.plane + c0 - CX[k] - CY[k] * p.pitch;  // source of cell c0
       if (CX[k] == 0) {
-        const uint32_t w = ld(s32 + (s0 >> 1));
-        F0[k] = h2_lo(w, inv);
-        F1[k] = h2_hi(w, inv);
+        const W w = ld(sw + (s0 >> 1));
+        F0[k] = PW::lo(w, p);
+        F1[k] = PW::hi(w, p);
       } else {  // s0 odd: high half of the word before, low half of the word after
-        const uint32_t wa = ld(s32 + ((s0 - 1) >> 1));
-        const uint32_t wb = ld(s32 + ((s0 + 1) >> 1));
-        F0[k] = h2_hi(wa, inv);
-        F1[k] = h2_lo(wb, inv);
+        F0[k] = PW::hi(ld(sw + ((s0 - 1) >> 1)), p);
+        F1[k] = PW::lo(ld(sw + ((s0 + 1) >> 1)), p);
       }
     }
     R out0[9], out1[9];
@@ -453,14 +470,10 @@ __global__ __launch_bounds__(256) void k_step_h2(const _Float16* __restrict__ sr
       macro_dev(F1, p.rho0_f, dr, Rh, U, V);
       collide_dev<COLL>(F1, dr, Rh, U, V, coll_par(p, -1), out1);
     }
-    uint32_t* d32 = reinterpret_cast<uint32_t*>(dst);
+    W* dw = reinterpret_cast<W*>(dst);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const uint16_t lo = __builtin_bit_cast(uint16_t, (_Float16)(out0[k] * p.scale_f));
-      const uint16_t hi = __builtin_bit_cast(uint16_t, (_Float16)(out1[k] * p.scale_f));
-      __builtin_nontemporal_store((uint32_t)lo | ((uint32_t)hi << 16),
-                                  d32 + ((k * p.plane + c0) >> 1));
-    }
+    for (int k = 0; k < 9; k++)
+      __builtin_nontemporal_store(PW::pack(out0[k], out1[k], p), dw + ((k * p.plane + c0) >> 1));
     return;
   }
   for (int h = 0; h < 2; h++) {  // per-cell path of k_step for each tile of the pair

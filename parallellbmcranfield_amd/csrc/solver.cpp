@@ -85,11 +85,11 @@ Launch make_launch(lbm_solver* s, int src_buf) {
     L.nb_bulk = s->nb_list;
     L.kp.tile_list = s->d_tile_list;
   }
-  // FP16, fused, single lattice, even XCD grouping (the pair grouping must
-  // stay a permutation): paired-cell plain steps
-  L.h2 = s->prec == 2 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
-         s->kp.xcd_group % 2 == 0 && s->nb_bulk % (8 * s->kp.xcd_group) == 0 &&
-         !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
+  // FP16 / FP32, fused, single lattice, even XCD grouping (the pair
+  // grouping must stay a permutation): paired-cell plain steps
+  L.pairs = s->prec != 0 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
+            s->kp.xcd_group % 2 == 0 && s->nb_bulk % (8 * s->kp.xcd_group) == 0 &&
+            !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
   L.out.rho = s->d_rho;
   L.out.u = s->d_u;
   L.out.v = s->d_v;
@@ -500,8 +500,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   const size_t sbytes = (size_t)9 * s->plane * s->elem;
   const size_t rsz = s->prec == 0 ? sizeof(double) : sizeof(float);
   rc = LBM_OK;
-  // + 256 B: the paired-cell FP16 kernel reads whole 32-bit words around a
-  // pull, which can reach one word past the last plane's last row
+  // + 256 B: the paired-cell kernel reads whole 2-cell words around a pull,
+  // which can reach one word past the last plane's last row
   if (!rc) rc = dev_alloc(s, &s->buf[0], sbytes + 256);
   if (!rc) rc = dev_alloc(s, &s->buf[1], sbytes + 256);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_cls, (size_t)s->plane);

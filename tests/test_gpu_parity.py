@@ -445,19 +445,20 @@ def test_parallel_writer_matches_oracle_bytes(tmp_path):
         assert (tmp_path / "o.csv").read_bytes() == (tmp_path / "g.csv").read_bytes(), (final, bug)
 
 
+@pytest.mark.parametrize("prec", [lbm.PREC_F16, lbm.PREC_F32])
 @pytest.mark.parametrize("coll", [lbm.BGKW, lbm.TRT, lbm.MRT])
-def test_fp16_paired_cell_kernel_is_bit_identical(coll):
-    """FP16 plain steps run two cells per thread with 32-bit loads and stores;
-    the per-cell kernel (LBM_TUNE_NO_PAIRS) gives the same bits, on a cavity
-    and on a channel whose cylinder makes mixed tiles, with residual and
-    observe steps in between."""
-    print("fp16 pairs", coll, flush=True)
+def test_paired_cell_kernel_is_bit_identical(coll, prec):
+    """FP16 / FP32 plain steps run two cells per thread with 2-cell loads and
+    stores; the per-cell kernel (LBM_TUNE_NO_PAIRS) gives the same bits, on a
+    cavity and on a channel whose cylinder makes mixed tiles, with residual
+    and observe steps in between."""
+    print("pairs", coll, prec, flush=True)
     for make in (lambda: lbm.Case.cavity(300), lambda: lbm.Case.channel(400, 130, 120.3, 66.1, 17.5)):
         case = make()
         out = []
         for flags in (0, lbm.TUNE_NO_PAIRS):
             o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.02,
-                                 precision=lbm.PREC_F16, init=lbm.INIT_REST,
+                                 precision=prec, init=lbm.INIT_REST,
                                  resid_kind=lbm.RESID_VELOCITY, resid_every=7, curved=1,
                                  inlet_profile=1, trt_lambda=0.25, tune_flags=flags)
             s = lbm.Solver(case, o)
