@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--xcd-group", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-pairs", action="store_true",
-                    help="one cell per thread (no paired-cell kernel) for FP16 / FP32")
+                    help="FP16: one cell per thread (no paired-cell kernel)")
     ap.add_argument("--sample-every", type=int, default=8,
                     help="HIP events around every N-th step-kernel launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -116,8 +116,8 @@ typedef struct lbm_options {
  * bulk pass instead of as the first workgroups of one fused launch.
  * LBM_TUNE_NO_GRAPH: launch every step directly instead of replaying captured
  * two-step HIP graphs for steps without residual evaluation.
- * LBM_TUNE_NO_PAIRS: FP16 / FP32 storage without the paired-cell plain-step
- * kernel (one cell per thread, one element per load and store). */
+ * LBM_TUNE_NO_PAIRS: FP16 storage without the paired-cell plain-step kernel
+ * (one cell per thread, 16-bit loads and stores). */
 enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2, LBM_TUNE_NO_GRAPH = 4,
        LBM_TUNE_NO_PAIRS = 8 };
 

@@ -379,17 +379,17 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
 
-// Plain steps with FP16 or FP32 storage: the bulk workgroups take PAIRS of
-// tiles (2 x 256 consecutive cells) and each thread two neighbouring cells,
-// so the loads and stores of all-interior pairs move one 32-bit (FP16) or
-// 64-bit (FP32) word per two cells: twice the bytes per memory instruction.
+// Plain steps with FP16 storage: the bulk workgroups take PAIRS of tiles
+// (2 x 256 consecutive cells) and each thread two neighbouring cells, so the
+// loads and stores of all-interior pairs move one 32-bit word per two cells:
+// twice the bytes per memory instruction (+15 % at 4097^2; the same pairing
+// with 64-bit words made FP32 7 % slower, so FP32 keeps one cell per thread).
 // A pull with cx = +-1 starts on an odd cell; its two halves come from the
 // two aligned words around it.  Same arithmetic per cell as k_step
 // (macro_dev and collide_dev on the same floats), so results are identical.
 // Boundary workgroups and pairs with a mixed or empty tile run k_step's
 // per-cell code.  Launch: nb_bnd + nb_bulk / 2 workgroups of 256; XCD
 // grouping by runs of G / 2 pairs (G even, nb_bulk a multiple of 8 G).
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 template <class S>
 struct Pair;
 template <>
@@ -406,13 +406,6 @@ struct Pair<StF16> {
     const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)(b * p.scale_f));
     return (uint32_t)l | ((uint32_t)h << 16);
   }
-};
-template <>
-struct Pair<StF32> {
-  using W = f32x2;
-  __device__ static float lo(W w, const KParams&) { return w.x; }
-  __device__ static float hi(W w, const KParams&) { return w.y; }
-  __device__ static W pack(float a, float b, const KParams&) { return W{a, b}; }
 };
 
 template <class S, int COLL>

@@ -18,7 +18,7 @@ struct Launch {
   int norows = 0;
   int nb_bulk = 0;             // bulk grid (multiple of 8)
   int block = 256;             // bulk workgroup size
-  bool pairs = false;          // FP16 / FP32 plain steps: paired-cell kernel (k_step_pair)
+  bool pairs = false;          // FP16 plain steps: paired-cell kernel (k_step_pair)
   KParams kp{};
   KOut out{};
   hipStream_t stream = nullptr;

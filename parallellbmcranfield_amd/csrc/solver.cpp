@@ -85,9 +85,9 @@ Launch make_launch(lbm_solver* s, int src_buf) {
     L.nb_bulk = s->nb_list;
     L.kp.tile_list = s->d_tile_list;
   }
-  // FP16 / FP32, fused, single lattice, even XCD grouping (the pair
-  // grouping must stay a permutation): paired-cell plain steps
-  L.pairs = s->prec != 0 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
+  // FP16, fused, single lattice, even XCD grouping (the pair grouping must
+  // stay a permutation): paired-cell plain steps
+  L.pairs = s->prec == 2 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
             s->kp.xcd_group % 2 == 0 && s->nb_bulk % (8 * s->kp.xcd_group) == 0 &&
             !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
   L.out.rho = s->d_rho;
