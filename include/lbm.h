@@ -173,8 +173,7 @@ int lbm_case_channel(int NX, int NY, double cx, double cy, double r, lbm_case** 
  * aspect ratio and obstacles). */
 int lbm_case_cavity_ex(int NX, int NY, int lid_type, const int* rects, int nrect,
                        lbm_case** out);
-/* Packs `count` cases side by side (shelf packing, tallest first; an instance after
- * the first of a shelf starts on a 64-cell wave boundary of the storage row) into one
+/* Packs `count` cases side by side (shelf packing, tallest first) into one
  * lattice for a batched solver.  Instances keep their own coordinates and
  * boundary links; rects (may be NULL) receives (x0, y0, n, m) per instance
  * in the packed lattice.  No counterpart in the reference. */

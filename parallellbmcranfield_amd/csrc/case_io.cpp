@@ -434,21 +434,16 @@ int lbm_case_pack(const lbm_case* const* cases, int count, lbm_case** out, int* 
   }
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return cases[a]->m > cases[b]->m; });
-  // Instances after the first of a shelf start where a 64-cell wave of the
-  // padded storage row starts (storage column x + 1 a multiple of 64): a
-  // wave then holds cells of one instance (or none), not two half-empty ones.
-  auto wave_start = [](int x) { return x == 0 ? 0 : ((x + 1 + 63) / 64) * 64 - 1; };
-  const int W = std::max(maxw, (int)std::ceil(std::sqrt((double)area * 1.25)));
+  const int W = std::max(maxw, (int)std::ceil(std::sqrt((double)area * 1.15)));
   std::vector<int> x0(count), y0(count);
   int x = 0, y = 0, shelf = 0;
   for (int k : order) {
     const lbm_case* c = cases[k];
-    if (wave_start(x) + c->n > W) {
+    if (x + c->n > W) {
       y += shelf;
       x = 0;
       shelf = 0;
     }
-    x = wave_start(x);
     x0[k] = x;
     y0[k] = y;
     x += c->n;
