@@ -264,8 +264,9 @@ size_t lbm_solver_device_bytes(const lbm_solver* s);
  * loads into a solver of the same case, precision and residual kind. */
 int lbm_solver_save(lbm_solver* s, const char* path);
 int lbm_solver_load(lbm_solver* s, const char* path);
-/* Kernel layout (for profiling): 1 if interior and boundary-list nodes share
- * one launch per step, 0 if they have separate launches. */
+/* Kernel layout (for profiling): bit 0 set if interior and boundary-list
+ * nodes share one launch per step (else separate launches); bit 1 set if
+ * plain steps run the paired-cell FP16 kernel. */
 int lbm_solver_fused(const lbm_solver* s);
 
 /* Bandwidth of a plain device-to-device copy kernel over a buffer of `bytes`

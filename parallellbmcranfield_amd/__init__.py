@@ -418,7 +418,12 @@ class Solver:
 
     @property
     def fused(self) -> bool:
-        return bool(lib().lbm_solver_fused(self.h))
+        return bool(lib().lbm_solver_fused(self.h) & 1)
+
+    @property
+    def paired(self) -> bool:
+        """Plain steps run the paired-cell FP16 kernel."""
+        return bool(lib().lbm_solver_fused(self.h) & 2)
 
     def save(self, path: str) -> None:
         """Binary checkpoint (lbm_solver_save)."""

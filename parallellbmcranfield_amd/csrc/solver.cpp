@@ -1121,7 +1121,10 @@ int lbm_solver_timing(const lbm_solver* s, lbm_timing* out) {
 
 size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 0; }
 
-int lbm_solver_fused(const lbm_solver* s) { return s && s->fused ? 1 : 0; }
+int lbm_solver_fused(const lbm_solver* s) {
+  if (!s) return 0;
+  return (s->fused ? 1 : 0) | (make_launch(const_cast<lbm_solver*>(s), 0).pairs ? 2 : 0);
+}
 
 // ---------------------------------------------------------------- checkpoint
 namespace {

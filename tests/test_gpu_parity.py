@@ -450,10 +450,13 @@ def test_paired_cell_kernel_is_bit_identical(coll):
     """FP16 plain steps run two cells per thread with 32-bit loads and stores;
     the per-cell kernel (LBM_TUNE_NO_PAIRS) gives the same bits, on a cavity
     and on a channel whose cylinder makes mixed tiles, with residual and
-    observe steps in between."""
+    observe steps in between.  The paired kernel runs on lattices of at least
+    16384 tiles."""
     prec = lbm.PREC_F16
     print("pairs", coll, flush=True)
-    for make in (lambda: lbm.Case.cavity(300), lambda: lbm.Case.channel(400, 130, 120.3, 66.1, 17.5)):
+    # lattices of >= 16384 tiles of 256 cells (the kernel's threshold)
+    for make in (lambda: lbm.Case.cavity(2050),
+                 lambda: lbm.Case.channel(4000, 1050, 1200.3, 530.1, 60.5)):
         case = make()
         out = []
         for flags in (0, lbm.TUNE_NO_PAIRS):
@@ -462,6 +465,7 @@ def test_paired_cell_kernel_is_bit_identical(coll):
                                  resid_kind=lbm.RESID_VELOCITY, resid_every=7, curved=1,
                                  inlet_profile=1, trt_lambda=0.25, tune_flags=flags)
             s = lbm.Solver(case, o)
+            assert s.paired == (flags == 0)
             s.step(61)
             out.append((s.macro(), s.residuals()))
         for a, b in zip(out[0][0], out[1][0]):
