@@ -117,9 +117,12 @@ typedef struct lbm_options {
  * LBM_TUNE_NO_GRAPH: launch every step directly instead of replaying captured
  * two-step HIP graphs for steps without residual evaluation.
  * LBM_TUNE_NO_PAIRS: FP16 storage without the paired-cell plain-step kernel
- * (one cell per thread, 16-bit loads and stores). */
+ * (one cell per thread, 16-bit loads and stores).
+ * LBM_TUNE_NO_TILE_WALLS: every boundary node in the boundary list (by
+ * default nodes whose only rule is straight-wall bounce-back run in their
+ * bulk tile). */
 enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2, LBM_TUNE_NO_GRAPH = 4,
-       LBM_TUNE_NO_PAIRS = 8 };
+       LBM_TUNE_NO_PAIRS = 8, LBM_TUNE_NO_TILE_WALLS = 16 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;

@@ -69,6 +69,7 @@ struct lbm_solver {
   int cur = 0;  // buf[cur] holds METAF_{t+1}
   uint8_t* d_cls = nullptr;
   uint8_t* d_tflag = nullptr;
+  uint32_t* d_wmask = nullptr;  // class-4 (in-tile wall) cells: stream and wall-link bits
   lbmdev::BNode* d_bn = nullptr;
   int nbn = 0;
   lbmdev::BNode* d_on = nullptr;

@@ -239,6 +239,32 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
   }
 }
 
+// Pull of a bulk-tile cell.  Class 1: interior, plain pull.  Class 4: a
+// boundary node whose only rule is straight-wall bounce-back (curved = 2, no
+// inlet/outlet links, no flags, not a force body): StreamingStep with its
+// StreamLattice mask (serial/Iterate.c:470-488, bits 0-8 of wmask), then
+// WallBC (serial/BoundaryConditions.c:104-118, bits 16-24 = BC_ID[k] == 1) in
+// the same order as the boundary-list path, so it runs in its tile with the
+// interior cells instead of as a scattered boundary-list thread.
+template <class S>
+__device__ inline void pull_cell(typename S::R F[9], const typename S::T* src, const KParams& p,
+                                 int idx, int c) {
+  if (c == 4) {
+    const uint32_t w = p.wmask[idx];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int off = ((w >> k) & 1) ? -CX[k] - CY[k] * p.pitch : 0;
+      F[k] = S::load(src + k * p.plane + idx + off, p);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if ((w >> (16 + k)) & 1) F[OPP[k]] = F[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+  }
+}
+
 // ---------------------------------------------------------------- kernels
 // Batched instances: the instance of a bulk node (per tile when the tile is
 // all interior, else per cell) and whether it is still being advanced.
@@ -277,7 +303,8 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
     if (tf == 0) return;
   }
   const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
-  bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+  const int c = tf == 1 ? 1 : (tf == 2 && idx < p.bulk_end ? (int)cls[idx] : 0);
+  bool active = c == 1 || c == 4;
   const int inst = bulk_instance(p, b, tf, idx, active);
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if constexpr (MODE != 1) {
@@ -285,8 +312,7 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
   }
   if (active) {
     R F[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+    pull_cell<S>(F, src, p, idx, c);
     finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst);
   }
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
@@ -364,15 +390,15 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
       if (tf == 0) return;
     }
     const int idx = p.bulk_begin + (int)(b * BS + threadIdx.x);
-    bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
+    const int c = tf == 1 ? 1 : (tf == 2 && idx < p.bulk_end ? (int)cls[idx] : 0);
+    bool active = c == 1 || c == 4;
     const int inst = bulk_instance(p, b, tf, idx, active);
     if constexpr (MODE != 1) {
       if (!active) return;
     }
     if (active) {
       R F[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+      pull_cell<S>(F, src, p, idx, c);
       finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst);
     }
   }
@@ -474,11 +500,10 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
     const int tf = tflag[b];
     if (tf == 0) continue;
     const int idx = p.bulk_begin + (int)(b * 256 + threadIdx.x);
-    const bool active = tf == 1 || (tf == 2 && idx < p.bulk_end && cls[idx] == 1);
-    if (!active) continue;
+    const int c = tf == 1 ? 1 : (idx < p.bulk_end ? (int)cls[idx] : 0);
+    if (c != 1 && c != 4) continue;
     R F[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) F[k] = S::load(src + k * p.plane + idx - CX[k] - CY[k] * p.pitch, p);
+    pull_cell<S>(F, src, p, idx, c);
     finish_node<S, COLL, 0, false>(F, src, dst, p, o, idx, nullptr, acc, -1);
   }
 }

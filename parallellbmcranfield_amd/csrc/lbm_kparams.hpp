@@ -45,6 +45,7 @@ struct KParams {
   const uint16_t* tile_inst;    // instance of each all-interior bulk tile
   const uint16_t* node_inst;    // instance of each padded cell (0xffff: none)
   const uint8_t* inst_active;   // 0: converged, frozen
+  const uint32_t* wmask;        // class-4 cells: StreamLattice bits 0-8, wall links 16-24
   const uint32_t* tile_list;    // bulk tiles to run, in launch order (null: all);
                                 // 0xffffffff pads the list to the XCD grouping
 };

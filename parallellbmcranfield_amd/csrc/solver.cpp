@@ -340,6 +340,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   }
   std::vector<BNode> bn, on;
   std::vector<int> orow;
+  std::vector<uint32_t> wmask;  // class-4 cells: StreamLattice and wall-link bits
   int last_row = -1;
   for (long lin = 0; lin < nc; lin++) {
     if (c->fluid[lin] != 1) continue;
@@ -395,6 +396,21 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
       on.push_back(r);
     } else {
       if (side_of[lin] >= 0) r.flags |= BN_FEEDER;
+      // straight-wall-only node: runs in its bulk tile (class 4, see pull_cell)
+      bool wall_only = o->curved == 2 && r.flags == 0 &&
+                       !(o->force_id != 0 && r.boundary_id == o->force_id) &&
+                       !(o->tune_flags & LBM_TUNE_NO_TILE_WALLS);
+      uint32_t wm = r.stream & 0x1ffu;
+      for (int k = 0; k < 9; k++) {
+        wall_only = wall_only && (r.bc[k] == 0 || r.bc[k] == 1);
+        if (r.bc[k] == 1) wm |= 1u << (16 + k);
+      }
+      if (wall_only) {
+        cls[idx] = 4;
+        if (wmask.empty()) wmask.assign((size_t)s->plane, 0u);
+        wmask[idx] = wm;
+        continue;
+      }
       cls[idx] = 2;
       bn.push_back(r);
     }
@@ -421,13 +437,14 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   // per-tile summary of the classes (tiles of BS cells in launch order)
   std::vector<uint8_t> tflag((size_t)s->nb_bulk, 0);
   for (int t = 0; t < s->nb_bulk; t++) {
-    int nb = 0, other = 0;
+    int nb = 0, nw = 0, other = 0;
     for (int e = 0; e < BS; e++) {
       const long idx = (long)kp.bulk_begin + (long)t * BS + e;
       if (idx < kp.bulk_end && cls[idx] == 1) nb++;
+      else if (idx < kp.bulk_end && cls[idx] == 4) nw++;
       else other++;
     }
-    tflag[t] = nb == 0 ? 0 : (other == 0 ? 1 : 2);
+    tflag[t] = nb + nw == 0 ? 0 : (nw == 0 && other == 0 ? 1 : 2);
   }
   kp.nb_bulk = s->nb_bulk;
   // ---- batched instances: parameter rows, instance of each cell / pure tile
@@ -480,7 +497,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
       const size_t e0 = s->tile_ids.size();
       for (int e = 0; e < BS; e++) {
         const long idx = (long)kp.bulk_begin + (long)t * BS + e;
-        if (idx >= kp.bulk_end || cls[idx] != 1) continue;
+        if (idx >= kp.bulk_end || (cls[idx] != 1 && cls[idx] != 4)) continue;
         const uint16_t k = node_inst[idx];
         bool seen = false;
         for (size_t q = e0; q < s->tile_ids.size(); q++) seen = seen || s->tile_ids[q] == k;
@@ -614,6 +631,11 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   }
   if (!rc) rc = check_hip(hipMemcpy(s->d_cls, cls.data(), cls.size(), hipMemcpyHostToDevice), "upload classes");
   if (!rc) rc = check_hip(hipMemcpy(s->d_tflag, tflag.data(), tflag.size(), hipMemcpyHostToDevice), "upload tile flags");
+  if (!rc && !wmask.empty()) {
+    rc = dev_alloc(s, (void**)&s->d_wmask, sizeof(uint32_t) * wmask.size());
+    if (!rc) rc = check_hip(hipMemcpy(s->d_wmask, wmask.data(), sizeof(uint32_t) * wmask.size(), hipMemcpyHostToDevice), "upload wall masks");
+    kp.wmask = s->d_wmask;
+  }
   if (!rc && s->nbn) rc = check_hip(hipMemcpy(s->d_bn, bn.data(), sizeof(BNode) * bn.size(), hipMemcpyHostToDevice), "upload boundary list");
   if (!rc && s->non) rc = check_hip(hipMemcpy(s->d_on, on.data(), sizeof(BNode) * on.size(), hipMemcpyHostToDevice), "upload outlet list");
   if (!rc) rc = check_hip(hipMemcpy(s->d_orow, orow.data(), sizeof(int) * orow.size(), hipMemcpyHostToDevice), "upload outlet rows");
@@ -753,7 +775,8 @@ void lbm_solver_destroy(lbm_solver* s) {
   void* ptrs[] = {s->buf[0], s->buf[1], s->d_cls, s->d_tflag, s->d_bn, s->d_on, s->d_orow, s->d_side,
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
                   s->d_part, s->d_hist, s->d_inst_par, s->d_tile_inst, s->d_node_inst,
-                  s->d_inst_active, s->d_inst_rect, s->d_inst_res, s->d_tile_list};
+                  s->d_inst_active, s->d_inst_rect, s->d_inst_res, s->d_tile_list,
+                  s->d_wmask};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
