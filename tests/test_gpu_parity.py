@@ -471,3 +471,28 @@ def test_paired_cell_kernel_is_bit_identical(coll):
         for a, b in zip(out[0][0], out[1][0]):
             assert np.array_equal(a, b)
         assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("prec,coll", [(lbm.PREC_F64, lbm.BGKW), (lbm.PREC_F32, lbm.MRT),
+                                       (lbm.PREC_F16, lbm.TRT)])
+def test_in_tile_walls_equal_boundary_list(prec, coll):
+    """Straight-wall-only nodes run in their bulk tiles (class 4); with
+    LBM_TUNE_NO_TILE_WALLS they run as boundary-list threads.  Same bits, on a
+    cavity and on a channel whose cylinder (straight walls here) makes wall
+    cells in mixed tiles, through residual and observe steps."""
+    print("tile walls", prec, coll, flush=True)
+    for case in (lbm.Case.cavity(150), lbm.Case.channel(300, 90, 80.2, 45.3, 12.5)):
+        out = []
+        for flags in (0, lbm.TUNE_NO_TILE_WALLS):
+            o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
+                                 precision=prec, init=lbm.INIT_REST, curved=2,
+                                 resid_kind=lbm.RESID_FULL, resid_every=5, trt_lambda=0.25,
+                                 tune_flags=flags)
+            s = lbm.Solver(case, o)
+            s.step(73)
+            f, mf = s.distributions()
+            out.append((s.macro(), f, s.residuals()))
+        for a, b in zip(out[0][0], out[1][0]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(out[0][1], out[1][1])
+        assert np.allclose(out[0][2], out[1][2], rtol=1e-12, atol=0)
