@@ -83,12 +83,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--workload", default="cavity",
-                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix", "cavity129"],
+                    choices=["cavity", "sweep", "warmstart", "netstep", "matrix", "cavity129",
+                             "cylinder"],
                     help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
                          "warmstart: learned start vs rest start on held-out cavities; "
                          "netstep: training steps of the warm-start network; "
                          "matrix: every collision x storage at --sizes, error vs FP64; "
-                         "cavity129: the 129^2 Re=100 cavity beside the reference program")
+                         "cavity129: the 129^2 Re=100 cavity beside the reference program; "
+                         "cylinder: the 2049x513 cylinder wake, Strouhal and drag")
     ap.add_argument("--count", type=int, default=256, help="sweep: number of cavities")
     ap.add_argument("--side", default="65,513", help="sweep: side-length range")
     ap.add_argument("--crit", type=float, default=1e-6, help="sweep: residual criterion")
@@ -121,6 +123,8 @@ def main():
         return bench_matrix(args)
     if args.workload == "cavity129":
         return bench_cavity129(args)
+    if args.workload == "cylinder":
+        return bench_cylinder(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -319,6 +323,85 @@ def bench_sweep(args):
             "bytes_per_update": bytes_per_update,
         },
         "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_cylinder(args):
+    """Channel of 2049 x 513 fluid nodes with a D = 64 cylinder at (512, 260)
+    (analytic-q curved links), parabolic inlet u_max = 0.05, rho = 1 pressure
+    outlet, MRT, Re = 100 on the mean inlet speed; 100 000 steps in FP64 and
+    FP32 with the momentum-exchange force every 10 steps: MLUPS, Strouhal
+    number and drag coefficient of each, and the CPU oracle on a bounded
+    sample of the same FP64 case (1 core)."""
+    import numpy as np
+    import parallellbmcranfield_amd as lbm
+    NX, NY, D, steps, every = 2049, 513, 64, 100000, 10
+    case = lbm.Case.channel(NX, NY, 512.0, 260.0, D / 2)
+    umax = 0.05
+    uavg = umax / 1.5
+    nu = uavg * D / 100.0
+    tau = 3 * float(np.float32(nu)) + 0.5
+    rates = [0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau]
+    nf = case.info.num_fluid
+    res = {}
+    for pname, prec in (("f64", lbm.PREC_F64), ("f32", lbm.PREC_F32)):
+        o = lbm.make_options(collision=lbm.MRT, uavg=uavg, rho_ini=1.0, viscosity=nu,
+                             inlet_profile=1, curved=1, outlet_profile=lbm.OUTLET_PRESSURE,
+                             init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE, mrt_textbook_meq=1,
+                             mrt_rates=rates, force_id=5, force_every=every, precision=prec,
+                             outlet_rho=1.0, use_stop_rule=0)
+        s = lbm.Solver(case, o)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        _, f = s.force_history()
+        s.close()
+        fx, fy = f[:, 0], f[:, 1]
+        half = len(fy) // 2
+        st = lbm.strouhal(fy, every, D, uavg)
+        cd = 2 * fx[half:].mean() / (uavg * uavg * D)
+        res[pname] = dict(mlups=round(nf * steps / wall / 1e6, 1), wall_s=round(wall, 2),
+                          strouhal=round(float(st), 5), cd=round(float(cd), 5))
+        log(f"cylinder {pname}: {res[pname]}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        import shutil
+        import tempfile
+        import oracle as O
+        d = tempfile.mkdtemp(prefix="lbm_cylinder_")
+        case.write(os.path.join(d, "n.dat"), os.path.join(d, "c.dat"))
+        p = O.params(collision=3, uavg=uavg, rho_ini=1.0, viscosity=nu, inlet_profile=1,
+                     curved=1, outlet_profile=4, init_rest=1, mrt_textbook=1, mrt_s=rates,
+                     resid_kind=3, outlet_rho=1.0)
+        orc = O.Oracle.from_files(os.path.join(d, "n.dat"), os.path.join(d, "c.dat"), p)
+        shutil.rmtree(d, ignore_errors=True)
+        orc.step(1)
+        n_cpu = 10
+        t0 = time.perf_counter()
+        orc.step(n_cpu)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(nf * n_cpu / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
+               "kind": "port", "sample": f"the same FP64 channel, {n_cpu} steps after 1 "
+                                          f"(oracle/lbm_oracle.c, 1 thread, {dt:.1f} s)"}
+    out = {
+        "metric": "2049x513 cylinder wake, MRT, Re=100: MLUPS, Strouhal, drag",
+        "value": res["f32"]["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(res["f32"]["wall_s"] * 1e3 / steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 run beside it)",
+        "data": "synthetic",
+        "config": {"workload": "channel 2049x513 fluid nodes, cylinder D=64 at (512,260) with "
+                               "analytic-q curved links, parabolic inlet u_max=0.05, rho=1 "
+                               "pressure outlet, MRT, Re=100 on the mean inlet speed, 100000 "
+                               "steps, force every 10", "fluid_nodes": nf,
+                   "parallelism": "single GPU"},
+        "runs": res,
+        "strouhal_rel_diff_f32_vs_f64": abs(res["f32"]["strouhal"] - res["f64"]["strouhal"]) /
+        res["f64"]["strouhal"],
+        "cd_rel_diff_f32_vs_f64": abs(res["f32"]["cd"] - res["f64"]["cd"]) / abs(res["f64"]["cd"]),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
