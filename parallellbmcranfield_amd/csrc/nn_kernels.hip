@@ -142,9 +142,17 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
 // part[chunk][co][t][ci] = sum over the chunk's row segments of
 // dout[p][co] * in(nbr(p, t))[ci].  K runs over 32-pixel row segments
 // (pixels (b, y, xs .. xs+31)); the 4 waves of a workgroup take every 4th
-// segment of the chunk and are summed through LDS in a fixed order.
+// segment of the chunk.  Each wave loads its segment's 32 x 32 blocks of dout
+// and of the (shifted) input with 16-byte loads and stores them transposed
+// ([channel][pixel]) in its own LDS slice, so every MFMA operand is one
+// 16-byte LDS read.  All waves run the same number of iterations (a wave past
+// the chunk's end loads zeros), so every barrier is reached by all threads.
+// The 4 waves' results are summed through LDS in a fixed order.
+constexpr int WG_LD = 40;  // LDS row length (bf16): 80 B rows keep 16-byte reads aligned
 __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ dout, int Cout,
                                                float* __restrict__ part, int seg_per_chunk) {
+  __shared__ __attribute__((aligned(16))) bf16 sa[4][32][WG_LD];  // dout^T: [co][px]
+  __shared__ __attribute__((aligned(16))) bf16 sb[4][32][WG_LD];  // input^T: [ci][px]
   __shared__ float red[4][32][33];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
@@ -157,40 +165,52 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   const int nseg = s.B * s.H * segs_row;
   const int s0 = blockIdx.x * seg_per_chunk;
   const int s1 = min(nseg, s0 + seg_per_chunk);
+  const int nit = (s1 - s0 + 3) / 4;  // the same for every wave
+  // loader lanes: pixel lp (two passes of 16) x 8-channel group lc
+  const int lp = lane >> 2, lc = (lane & 3) * 8;
   f32x4 acc[2][2];
 #pragma unroll
   for (int g = 0; g < 2; g++)
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int sg = s0 + wave; sg < s1; sg += 4) {
-    const int xs = (sg % segs_row) * 32;
-    const int q = sg / segs_row;
-    const int y = q % s.H, b = q / s.H;
-    const size_t rowp = ((size_t)b * s.H + y) * s.W;
+  for (int it = 0; it < nit; it++) {
+    const int sg = s0 + it * 4 + wave;
+    const bool live = sg < s1;
+    int xs = 0, y = 0, b = 0;
+    if (live) {
+      xs = (sg % segs_row) * 32;
+      const int q = sg / segs_row;
+      y = q % s.H;
+      b = q / s.H;
+    }
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+      const int px = lp + 16 * pass;
+      const int x = xs + px;
+      bf16x8 va = zero8(), vb = zero8();
+      if (live && x < s.W) {
+        const size_t pix = ((size_t)b * s.H + y) * s.W + x;
+        if (cob + lc < Cout) va = *reinterpret_cast<const bf16x8*>(dout + pix * Cout + cob + lc);
+        vb = load8(s, b, y + dy, x + dx, cib + lc);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        sa[wave][lc + e][px] = va[e];
+        sb[wave][lc + e][px] = vb[e];
+      }
+    }
+    __syncthreads();
     bf16x8 a[2], bm[2];
 #pragma unroll
-    for (int g = 0; g < 2; g++) {  // A: dout^T, rows = co, K = pixels
-      const int co = cob + g * 16 + r;
+    for (int g = 0; g < 2; g++) a[g] = *reinterpret_cast<const bf16x8*>(&sa[wave][g * 16 + r][kb * 8]);
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int x = xs + kb * 8 + e;
-        a[g][e] = (co < Cout && x < s.W) ? dout[(rowp + x) * Cout + co] : (__bf16)0.0f;
-      }
-    }
-#pragma unroll
-    for (int h = 0; h < 2; h++) {  // B: input, K = pixels, cols = ci
-      const int ci = cib + h * 16 + r;
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int x = xs + kb * 8 + e;
-        bm[h][e] = x < s.W ? (__bf16)load1(s, b, y + dy, x + dx, ci) : (__bf16)0.0f;
-      }
-    }
+    for (int h = 0; h < 2; h++) bm[h] = *reinterpret_cast<const bf16x8*>(&sb[wave][h * 16 + r][kb * 8]);
 #pragma unroll
     for (int g = 0; g < 2; g++)
 #pragma unroll
       for (int h = 0; h < 2; h++)
         acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
+    __syncthreads();
   }
 #pragma unroll
   for (int g = 0; g < 2; g++)
