@@ -104,11 +104,12 @@ static ConvSrc src1(const lbm_net* n, const u16* x, int C, int l, int B, int dil
   return s;
 }
 
-// chunking of a conv's weight gradient: about 4096 workgroups per launch
+// chunking of a conv's weight gradient: about 2048 workgroups per launch
+// (fewer chunks = less partial-sum traffic for the fixed-order reduction)
 static int seg_per_chunk(const lbm_net* n, const Conv& c, const ConvSrc& s) {
   const long nseg = (long)s.B * s.H * ((s.W + 31) / 32);
   const long per = (long)((c.Cout + 31) / 32) * 9 * ((c.C0 + c.C1) / 32);
-  long chunks = std::max(1L, 4096 / per);
+  long chunks = std::max(1L, 2048 / per);
   (void)n;
   return (int)std::max(1L, (nseg + chunks - 1) / chunks);
 }

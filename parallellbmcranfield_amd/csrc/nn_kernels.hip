@@ -258,14 +258,22 @@ __global__ __launch_bounds__(256) void k_colsum_part(const bf16* __restrict__ g,
   }
 }
 
-// out[i] = sum_c part[c][i], chunks in order (deterministic).
+// out[i] = sum_c part[c][i] in a fixed order (deterministic): four running
+// sums over c = 0, 1, 2, 3 (mod 4), so four loads are in flight per step.
 __global__ void k_reduce_chunks(const float* __restrict__ part, int nchunk, long n,
                                 float* __restrict__ out) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  float s = 0.0f;
-  for (int c = 0; c < nchunk; c++) s += part[(size_t)c * n + i];
-  out[i] = s;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  int c = 0;
+  for (; c + 4 <= nchunk; c += 4) {
+    a0 += part[(size_t)c * n + i];
+    a1 += part[(size_t)(c + 1) * n + i];
+    a2 += part[(size_t)(c + 2) * n + i];
+    a3 += part[(size_t)(c + 3) * n + i];
+  }
+  for (; c < nchunk; c++) a0 += part[(size_t)c * n + i];
+  out[i] = (a0 + a1) + (a2 + a3);
 }
 
 // ------------------------------------------------------------ elementwise
