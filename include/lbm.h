@@ -306,7 +306,7 @@ int lbm_iteration(const char* NodeDataFile, const char* BCconnectorDataFile, flo
  * and Adam) mapping per-pixel cavity descriptors (fluid mask, normalised
  * log Re, lid speed, wall distances, ...) on an H x W grid to the steady
  * (u, v, rho) there.  `levels` resolutions with base << l channels
- * (l = 0 .. levels-1); base a multiple of 32.  Inputs are
+ * (l = 0 .. levels-1); base a multiple of 32, base << (levels-1) <= 2048.  Inputs are
  * [sample][H][W][in_channels] FP32 (in_channels <= 32), channel 0 the fluid
  * mask (1/0) that weighs the loss; targets and predictions [sample][H][W][3].
  * No counterpart in the reference. */

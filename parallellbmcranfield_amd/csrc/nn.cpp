@@ -106,12 +106,12 @@ static ConvSrc src1(const lbm_net* n, const u16* x, int C, int l, int B, int dil
 
 // chunking of a conv's weight gradient: about 2048 workgroups per launch
 // (fewer chunks = less partial-sum traffic for the fixed-order reduction)
-static int seg_per_chunk(const lbm_net* n, const Conv& c, const ConvSrc& s) {
-  const long nseg = (long)s.B * s.H * ((s.W + 31) / 32);
+static int grp_per_chunk(const lbm_net* n, const Conv& c, const ConvSrc& s) {
+  const long ngrp = wgrad_groups(s);
   const long per = (long)((c.Cout + 31) / 32) * 9 * ((c.C0 + c.C1) / 32);
   long chunks = std::max(1L, 2048 / per);
   (void)n;
-  return (int)std::max(1L, (nseg + chunks - 1) / chunks);
+  return (int)std::max(1L, (ngrp + chunks - 1) / chunks);
 }
 
 // uniform (0,1] from a 64-bit LCG: deterministic initialisation on every host
@@ -123,6 +123,7 @@ static double urand(uint64_t& st) {
 int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channels, int dilation,
                    unsigned seed, lbm_net** out) {
   if (!out || H < 2 || W < 2 || batch < 1 || base < 32 || base % 32 || levels < 2 || levels > 6 ||
+      ((long)base << (levels - 1)) > 2048 ||
       in_channels < 1 || in_channels > CIN0 || dilation < 1 || dilation > 64)
     return LBM_EINVAL;
   *out = nullptr;
@@ -219,7 +220,7 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channel
   for (const Conv& c : n->conv) {
     ConvSrc s = src1(n, nullptr, c.C0, c.level, B);
     s.C1 = c.C1;
-    const int ch = wgrad_chunks(s, seg_per_chunk(n, c, s));
+    const int ch = wgrad_chunks(s, grp_per_chunk(n, c, s));
     cap = std::max(cap, (size_t)ch * c.Cout * 9 * (c.C0 + c.C1));
     cap = std::max(cap, (size_t)((P_of(n, c.level, B) + RED_CHUNK - 1) / RED_CHUNK) * c.Cout);
   }
@@ -333,7 +334,7 @@ static int forward(lbm_net* n, int B) {
 
 // weight + bias gradient of conv c (input s, output gradient gout) into n->g
 static int conv_grads(lbm_net* n, const Conv& c, const ConvSrc& s, const u16* gout) {
-  const int spc = seg_per_chunk(n, c, s);
+  const int spc = grp_per_chunk(n, c, s);
   const int ch = wgrad_chunks(s, spc);
   const long nw = (long)c.Cout * 9 * (c.C0 + c.C1);
   if ((size_t)ch * nw > n->part_cap) {

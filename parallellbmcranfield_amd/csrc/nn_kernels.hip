@@ -138,22 +138,36 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
   }
 }
 
+// Orders a wave's LDS accesses: what every lane of the wave stored before it
+// is visible to every lane's loads after it (a wave's LDS instructions run in
+// order; the fences stop the compiler moving memory accesses across).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------ weight gradient
-// part[chunk][co][t][ci] = sum over the chunk's row segments of
-// dout[p][co] * in(nbr(p, t))[ci].  K runs over 32-pixel row segments
-// (pixels (b, y, xs .. xs+31)); the 4 waves of a workgroup take every 4th
-// segment of the chunk.  Each wave loads its segment's 32 x 32 blocks of dout
-// and of the (shifted) input with 16-byte loads and stores them transposed
-// ([channel][pixel]) in its own LDS slice, so every MFMA operand is one
-// 16-byte LDS read.  All waves run the same number of iterations (a wave past
-// the chunk's end loads zeros), so every barrier is reached by all threads.
-// The 4 waves' results are summed through LDS in a fixed order.
-constexpr int WG_LD = 40;  // LDS row length (bf16): 80 B rows keep 16-byte reads aligned
+// part[chunk][co][t][ci] = sum over the chunk's pixels p of
+// dout[p][co] * in(nbr(p, t))[ci].  K runs over groups of 64 consecutive
+// pixels of the flattened (b, y, x) order; the 4 waves of a workgroup take
+// every 4th group of the chunk.  A wave loads its group's 64 x 32 blocks of
+// dout and of the (shifted) input with 16-byte loads, four pixels per lane,
+// and stores them transposed ([channel][pixel]) in its own LDS slice with
+// 8-byte writes (the four pixels of one channel), so every MFMA operand is
+// one 16-byte LDS read; the next group is loaded into registers while the
+// current one is multiplied.  The slices are wave-private, so the loop syncs
+// waves, not the workgroup; a wave past the chunk's end loads zeros.  The 4
+// waves' results are summed in a fixed order through LDS (reusing the
+// slices) after a workgroup barrier.
+constexpr int WG_PX = 64;          // pixels per K step of a wave
+constexpr int WG_LD = WG_PX + 8;   // LDS row (bf16): 144-byte rows keep 16-byte reads aligned
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ dout, int Cout,
-                                               float* __restrict__ part, int seg_per_chunk) {
-  __shared__ __attribute__((aligned(16))) bf16 sa[4][32][WG_LD];  // dout^T: [co][px]
-  __shared__ __attribute__((aligned(16))) bf16 sb[4][32][WG_LD];  // input^T: [ci][px]
-  __shared__ float red[4][32][33];
+                                               float* __restrict__ part, int grp_per_chunk) {
+  // [0]: dout^T [co][px], [1]: input^T [ci][px], per wave (36 864 B); the
+  // final reduction reads it as float [4][32][33] (16 896 B)
+  __shared__ __attribute__((aligned(16))) bf16 lds[2][4][32][WG_LD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
   const int Cin = s.C0 + s.C1;
@@ -161,57 +175,77 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   const int nci = Cin / 32;
   const int t = blockIdx.z / nci, cib = (blockIdx.z % nci) * 32;
   const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
-  const int segs_row = (s.W + 31) / 32;
-  const int nseg = s.B * s.H * segs_row;
-  const int s0 = blockIdx.x * seg_per_chunk;
-  const int s1 = min(nseg, s0 + seg_per_chunk);
-  const int nit = (s1 - s0 + 3) / 4;  // the same for every wave
-  // loader lanes: pixel lp (two passes of 16) x 8-channel group lc
-  const int lp = lane >> 2, lc = (lane & 3) * 8;
+  const int P = s.B * s.H * s.W;
+  const int ngrp = (P + WG_PX - 1) / WG_PX;
+  const int g0 = blockIdx.x * grp_per_chunk;
+  const int g1 = min(ngrp, g0 + grp_per_chunk);
+  const int nit = (g1 - g0 + 3) / 4;  // the same for every wave
+  // loader lanes: pixels 4 lq .. 4 lq + 3 of the group x 8-channel group lc
+  const int lq = lane >> 2, lc = (lane & 3) * 8;
+  bf16x8 va[4], vb[4];
+  auto fetch = [&](int it) {
+    const int gi = g0 + it * 4 + wave;
+    const bool live = it < nit && gi < g1;
+    const int p0 = gi * WG_PX + lq * 4;
+    int x = 0, y = 0, b = 0;
+    if (live) {
+      const int q = p0 / s.W;
+      x = p0 - q * s.W;
+      b = q / s.H;
+      y = q - b * s.H;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      va[j] = zero8();
+      vb[j] = zero8();
+      if (live && p0 + j < P) {
+        va[j] = *reinterpret_cast<const bf16x8*>(dout + (size_t)(p0 + j) * Cout + cob + lc);
+        vb[j] = load8(s, b, y + dy, x + dx, cib + lc);
+      }
+      if (++x == s.W) {  // next pixel in (b, y, x) order
+        x = 0;
+        if (++y == s.H) {
+          y = 0;
+          b++;
+        }
+      }
+    }
+  };
   f32x4 acc[2][2];
 #pragma unroll
   for (int g = 0; g < 2; g++)
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  fetch(0);
   for (int it = 0; it < nit; it++) {
-    const int sg = s0 + it * 4 + wave;
-    const bool live = sg < s1;
-    int xs = 0, y = 0, b = 0;
-    if (live) {
-      xs = (sg % segs_row) * 32;
-      const int q = sg / segs_row;
-      y = q % s.H;
-      b = q / s.H;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      *reinterpret_cast<bf16x4*>(&lds[0][wave][lc + e][lq * 4]) =
+          bf16x4{va[0][e], va[1][e], va[2][e], va[3][e]};
+      *reinterpret_cast<bf16x4*>(&lds[1][wave][lc + e][lq * 4]) =
+          bf16x4{vb[0][e], vb[1][e], vb[2][e], vb[3][e]};
     }
+    wave_sync();
+    fetch(it + 1);  // the next group's loads are in flight during the MFMAs
 #pragma unroll
-    for (int pass = 0; pass < 2; pass++) {
-      const int px = lp + 16 * pass;
-      const int x = xs + px;
-      bf16x8 va = zero8(), vb = zero8();
-      if (live && x < s.W) {
-        const size_t pix = ((size_t)b * s.H + y) * s.W + x;
-        if (cob + lc < Cout) va = *reinterpret_cast<const bf16x8*>(dout + pix * Cout + cob + lc);
-        vb = load8(s, b, y + dy, x + dx, cib + lc);
-      }
+    for (int ks = 0; ks < WG_PX / 32; ks++) {
+      bf16x8 a[2], bm[2];
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        sa[wave][lc + e][px] = va[e];
-        sb[wave][lc + e][px] = vb[e];
-      }
-    }
-    __syncthreads();
-    bf16x8 a[2], bm[2];
-#pragma unroll
-    for (int g = 0; g < 2; g++) a[g] = *reinterpret_cast<const bf16x8*>(&sa[wave][g * 16 + r][kb * 8]);
-#pragma unroll
-    for (int h = 0; h < 2; h++) bm[h] = *reinterpret_cast<const bf16x8*>(&sb[wave][h * 16 + r][kb * 8]);
-#pragma unroll
-    for (int g = 0; g < 2; g++)
+      for (int g = 0; g < 2; g++)
+        a[g] = *reinterpret_cast<const bf16x8*>(&lds[0][wave][g * 16 + r][ks * 32 + kb * 8]);
 #pragma unroll
       for (int h = 0; h < 2; h++)
-        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
-    __syncthreads();
+        bm[h] = *reinterpret_cast<const bf16x8*>(&lds[1][wave][h * 16 + r][ks * 32 + kb * 8]);
+#pragma unroll
+      for (int g = 0; g < 2; g++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
+    }
+    wave_sync();
   }
+  __syncthreads();  // every wave is done with its slice
+  float(*red)[32][33] = reinterpret_cast<float(*)[32][33]>(&lds[0][0][0][0]);
 #pragma unroll
   for (int g = 0; g < 2; g++)
 #pragma unroll
@@ -222,39 +256,35 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   for (int e = threadIdx.x; e < 32 * 32; e += 256) {
     const int row = e / 32, col = e % 32;
     const int co = cob + row, ci = cib + col;
-    if (co >= Cout) continue;
     const float v = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
     part[(((size_t)blockIdx.x * Cout + co) * 9 + t) * Cin + ci] = v;
   }
 }
 
 // part[chunk][c] = sum over the chunk's pixels of g[p][c] (bias gradient).
-// Threads are (lane, channel) pairs, lane = t / C over 256 / C pixel lanes,
-// so a wave reads whole pixel rows; lanes are summed in a fixed order in LDS.
+// Threads are (lane, group) pairs: a group is 8 consecutive channels read as
+// one 16-byte load, lane = t / (C/8) over 256 / (C/8) pixel lanes, so a wave
+// reads whole pixel rows with wide loads; lanes are summed in a fixed order
+// in LDS.  C % 8 == 0 and C <= 2048 (the host checks).
 __global__ __launch_bounds__(256) void k_colsum_part(const bf16* __restrict__ g, int P, int C,
                                                      int px_per_chunk, float* __restrict__ part) {
-  __shared__ float red[256];
+  __shared__ float red[256][9];  // 9: odd row stride, conflict-free column reads
   const int p0 = blockIdx.x * px_per_chunk, p1 = min(P, p0 + px_per_chunk);
-  const int lanes = C <= 256 ? 256 / C : 1;
-  const int t = threadIdx.x;
-  if (C > 256) {  // wide layers: one thread per channel
-    for (int c = t; c < C; c += 256) {
-      float s = 0.0f;
-      for (int p = p0; p < p1; p++) s += (float)g[(size_t)p * C + c];
-      part[(size_t)blockIdx.x * C + c] = s;
-    }
-    return;
-  }
-  const int c = t % C, lane = t / C;
-  float s = 0.0f;
+  const int G = C / 8, lanes = 256 / G;
+  const int t = threadIdx.x, grp = t % G, lane = t / G;
+  float s[8];
+  for (int e = 0; e < 8; e++) s[e] = 0.0f;
   if (lane < lanes)
-    for (int p = p0 + lane; p < p1; p += lanes) s += (float)g[(size_t)p * C + c];
-  red[t] = s;
+    for (int p = p0 + lane; p < p1; p += lanes) {
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(g + (size_t)p * C + grp * 8);
+      for (int e = 0; e < 8; e++) s[e] += (float)v[e];
+    }
+  for (int e = 0; e < 8; e++) red[t][e] = s[e];
   __syncthreads();
-  if (t < C) {
+  for (int c = t; c < C; c += 256) {
     float r = 0.0f;
-    for (int l = 0; l < lanes; l++) r += red[l * C + t];
-    part[(size_t)blockIdx.x * C + t] = r;
+    for (int l = 0; l < lanes; l++) r += red[l * G + c / 8][c % 8];
+    part[(size_t)blockIdx.x * C + c] = r;
   }
 }
 
@@ -294,9 +324,16 @@ __global__ void k_to_bf16(const float* __restrict__ a, bf16* __restrict__ b, lon
 }
 
 // g *= (y > 0) (ReLU backward; y is the stored post-activation output).
-__global__ void k_relu_bwd(bf16* __restrict__ g, const bf16* __restrict__ y, long n) {
+// g[i] = 0 where y[i] <= 0; eight elements per thread (n % 8 == 0, 16-byte
+// aligned buffers: the host checks)
+__global__ void k_relu_bwd(bf16* __restrict__ g, const bf16* __restrict__ y, long n8) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n && !((float)y[i] > 0.0f)) g[i] = (__bf16)0.0f;
+  if (i >= n8) return;
+  const bf16x8 yv = reinterpret_cast<const bf16x8*>(y)[i];
+  bf16x8 gv = reinterpret_cast<const bf16x8*>(g)[i];
+  for (int e = 0; e < 8; e++)
+    if (!((float)yv[e] > 0.0f)) gv[e] = (__bf16)0.0f;
+  reinterpret_cast<bf16x8*>(g)[i] = gv;
 }
 
 // 2x2 average pooling, ceil size: out (ceil(H/2), ceil(W/2)).
@@ -530,21 +567,27 @@ hipError_t conv3x3(const ConvSrc& cs, const u16* w, const float* bias, u16* out,
   return hipGetLastError();
 }
 
-int wgrad_chunks(const ConvSrc& cs, int seg_per_chunk) {
-  const long nseg = (long)cs.B * cs.H * ((cs.W + 31) / 32);
-  return (int)((nseg + seg_per_chunk - 1) / seg_per_chunk);
+int wgrad_groups(const ConvSrc& cs) {
+  return (int)(((long)cs.B * cs.H * cs.W + WG_PX - 1) / WG_PX);
 }
 
-hipError_t wgrad(const ConvSrc& cs, const u16* dout, int Cout, float* part, int seg_per_chunk,
+int wgrad_chunks(const ConvSrc& cs, int grp_per_chunk) {
+  return (wgrad_groups(cs) + grp_per_chunk - 1) / grp_per_chunk;
+}
+
+hipError_t wgrad(const ConvSrc& cs, const u16* dout, int Cout, float* part, int grp_per_chunk,
                  hipStream_t st) {
   const int Cin = cs.C0 + cs.C1;
-  dim3 g((unsigned)wgrad_chunks(cs, seg_per_chunk), (unsigned)((Cout + 31) / 32),
-         (unsigned)(9 * (Cin / 32)));
-  k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, seg_per_chunk);
+  if (Cout % 32 || Cin % 32 || grp_per_chunk < 1 ||
+      (long)cs.B * cs.H * cs.W + WG_PX > 0x7fffffffL)
+    return hipErrorInvalidValue;
+  dim3 g((unsigned)wgrad_chunks(cs, grp_per_chunk), (unsigned)(Cout / 32), (unsigned)(9 * (Cin / 32)));
+  k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk);
   return hipGetLastError();
 }
 
 hipError_t colsum_part(const u16* g, int P, int C, int px_per_chunk, float* part, hipStream_t st) {
+  if (C % 8 || C > 2048 || (reinterpret_cast<uintptr_t>(g) & 15)) return hipErrorInvalidValue;
   k_colsum_part<<<(unsigned)((P + px_per_chunk - 1) / px_per_chunk), 256, 0, st>>>(
       cb(g), P, C, px_per_chunk, part);
   return hipGetLastError();
@@ -566,7 +609,9 @@ hipError_t to_bf16(const float* a, u16* b, long n, hipStream_t st) {
 }
 
 hipError_t relu_bwd(u16* g, const u16* y, long n, hipStream_t st) {
-  k_relu_bwd<<<nblk(n), 256, 0, st>>>(mb(g), cb(y), n);
+  if (n % 8 || ((reinterpret_cast<uintptr_t>(g) | reinterpret_cast<uintptr_t>(y)) & 15))
+    return hipErrorInvalidValue;
+  k_relu_bwd<<<nblk(n / 8), 256, 0, st>>>(mb(g), cb(y), n / 8);
   return hipGetLastError();
 }
 

@@ -24,9 +24,10 @@ struct ConvSrc {
 hipError_t conv3x3(const ConvSrc& s, const u16* w, const float* bias, u16* out, int Cout,
                    bool relu, hipStream_t st);
 // Weight-gradient partial sums, one [Cout][9][Cin] block per chunk of
-// seg_per_chunk 32-pixel row segments.
-int wgrad_chunks(const ConvSrc& s, int seg_per_chunk);
-hipError_t wgrad(const ConvSrc& s, const u16* dout, int Cout, float* part, int seg_per_chunk,
+// grp_per_chunk groups of 64 consecutive pixels (Cout, Cin multiples of 32).
+int wgrad_groups(const ConvSrc& s);
+int wgrad_chunks(const ConvSrc& s, int grp_per_chunk);
+hipError_t wgrad(const ConvSrc& s, const u16* dout, int Cout, float* part, int grp_per_chunk,
                  hipStream_t st);
 hipError_t colsum_part(const u16* g, int P, int C, int px_per_chunk, float* part, hipStream_t st);
 hipError_t reduce_chunks(const float* part, int nchunk, long n, float* out, hipStream_t st);
