@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=120, help="warmstart: training epochs")
     ap.add_argument("--batch", type=int, default=16, help="warmstart: training batch")
     ap.add_argument("--lr", type=float, default=1e-3, help="warmstart: peak learning rate")
+    ap.add_argument("--rho-weight", type=float, default=None,
+                    help="warmstart: density weight in the loss (default: the package's)")
+    ap.add_argument("--weight-decay", type=float, default=None,
+                    help="warmstart: decoupled weight decay (default: the package's)")
     ap.add_argument("--sizes", default="1025,4097", help="matrix: cavity sizes")
     ap.add_argument("--save-net", default="", help="warmstart: write the trained weights here")
     ap.add_argument("--data-max-steps", type=int, default=200000,
@@ -646,7 +650,10 @@ def bench_warmstart(args):
     net = lbm.Net(G, G, batch=args.batch, base=32, levels=4, seed=0, in_channels=WS.IN_CH,
                   dilation=WS.DILATION)
     t1 = time.perf_counter()
-    hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log)
+    rho_w = WS.RHO_WEIGHT if args.rho_weight is None else args.rho_weight
+    wd = WS.WEIGHT_DECAY if args.weight_decay is None else args.weight_decay
+    hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log, rho_weight=rho_w,
+                    weight_decay=wd)
     t_train = time.perf_counter() - t1
     if args.save_net:
         net.save(args.save_net)
@@ -678,6 +685,9 @@ def bench_warmstart(args):
                         f"32-256 channels; MRT f32 lattice, residual {args.crit:g}",
             "params": net.num_params,
             "bottleneck_dilation": WS.DILATION,
+            "input_channels": WS.IN_CH,
+            "rho_loss_weight": rho_w,
+            "weight_decay": wd,
             "training_samples": int(len(hist) and args.train_count),
             "epochs": args.epochs,
             "batch": args.batch,

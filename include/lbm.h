@@ -334,6 +334,9 @@ int lbm_net_gradients(lbm_net* n, const float* in, const float* target, int nb, 
  * (b1 0.9, b2 0.999, eps 1e-8) on the samples idx[0..nb). */
 int lbm_net_set_data(lbm_net* n, const float* in, const float* target, int count);
 int lbm_net_train(lbm_net* n, const int* idx, int nb, float lr, double* loss);
+/* Decoupled weight decay of the Adam step (AdamW): every parameter also
+ * moves by -lr * wd * p.  0 (the default) is plain Adam; 0 <= wd <= 1. */
+int lbm_net_set_weight_decay(lbm_net* n, float wd);
 int lbm_net_save(lbm_net* n, const char* path);
 int lbm_net_load(lbm_net* n, const char* path);
 

@@ -47,7 +47,7 @@ EXPORTS = [
     "lbm_run_checkpointed", "lbm_iteration",
     "lbm_net_create", "lbm_net_destroy", "lbm_net_num_params", "lbm_net_get_params",
     "lbm_net_set_params", "lbm_net_predict", "lbm_net_gradients", "lbm_net_set_data",
-    "lbm_net_train", "lbm_net_save", "lbm_net_load",
+    "lbm_net_train", "lbm_net_set_weight_decay", "lbm_net_save", "lbm_net_load",
 ]
 
 
@@ -167,6 +167,7 @@ def lib() -> C.CDLL:
         "lbm_net_gradients": (I, [P, P, P, I, C.POINTER(D), P]),
         "lbm_net_set_data": (I, [P, P, P, I]),
         "lbm_net_train": (I, [P, P, I, C.c_float, C.POINTER(D)]),
+        "lbm_net_set_weight_decay": (I, [P, C.c_float]),
         "lbm_net_save": (I, [P, C.c_char_p]),
         "lbm_net_load": (I, [P, C.c_char_p]),
         "lbm_iteration": (I, [C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
@@ -505,6 +506,10 @@ class Net:
         loss = C.c_double()
         _check(lib().lbm_net_train(self.h, _ptr(idx), idx.size, lr, C.byref(loss)))
         return loss.value
+
+    def set_weight_decay(self, wd: float) -> None:
+        """Decoupled weight decay of the Adam step (AdamW); 0 is plain Adam."""
+        _check(lib().lbm_net_set_weight_decay(self.h, wd))
 
     def save(self, path: str) -> None:
         _check(lib().lbm_net_save(self.h, path.encode()))

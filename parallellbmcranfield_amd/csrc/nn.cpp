@@ -49,6 +49,7 @@ struct Conv {
 
 struct lbm_net {
   int H = 0, W = 0, Bmax = 0, base = 32, L = 4, cin = 3;  // cin: input channels
+  float wd = 0.0f;  // decoupled weight decay of the Adam step
   int dil = 1;  // bottleneck dilation: its convs use dil and 2 dil
   hipStream_t st = nullptr;
   std::vector<Conv> conv;
@@ -519,9 +520,16 @@ int lbm_net_train(lbm_net* n, const int* idx, int nb, float lr, double* loss) {
   n->step++;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
-  NC(adam(n->p, n->g, n->m, n->v, n->pb, (long)n->nparam, lr, b1, b2, 1e-8f, c1, c2, n->st));
+  NC(adam(n->p, n->g, n->m, n->v, n->pb, (long)n->nparam, lr, b1, b2, 1e-8f, c1, c2, n->wd,
+          n->st));
   for (const Conv& c : n->conv) NC(flip_w(n->pb + c.w, n->wt + c.w, c.Cout, c.C0 + c.C1, n->st));
   if (!loss) NC(hipStreamSynchronize(n->st));
+  return LBM_OK;
+}
+
+int lbm_net_set_weight_decay(lbm_net* n, float wd) {
+  if (!n || !(wd >= 0.0f) || wd > 1.0f) return LBM_EINVAL;
+  n->wd = wd;
   return LBM_OK;
 }
 

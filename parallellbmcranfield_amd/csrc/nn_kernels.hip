@@ -63,70 +63,85 @@ __device__ inline float load1(const Src& s, int b, int y, int x, int ci) {
 // ------------------------------------------------------------ conv forward
 // out[p][co] = act(bias[co] + sum_{t, ci} in(nbr(p, t))[ci] * w[co][t][ci]),
 // nbr(p, t) = p + dil * (t / 3 - 1, t % 3 - 1).
-// Workgroup: 4 waves, each 32 pixels x 32 output channels (2 x 2 MFMA
-// tiles); COW waves along the output channels, 4/COW along the pixels.
-template <int COW, bool RELU>
+// Implicit GEMM, M = pixels, N = output channels, K = 9 taps x Cin in steps
+// of 32 channels of one tap.  Both operands are already in MFMA layout in
+// memory (a lane's 8 consecutive K values are 8 consecutive channels: one
+// 16-byte load), so no LDS is needed.  Each wave computes TM x TN tiles of
+// 16 x 16 (16 TM pixels x 16 TN channels); the 4 waves of a workgroup take
+// consecutive pixel blocks of the same channel block, so they share the
+// weight rows in the caches.  The next K step's operands are loaded while
+// the current step's MFMAs run.  Cout % (16 TN) == 0 (the host checks).
+template <int TM, int TN, bool RELU>
 __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__ w,
                                                   const float* __restrict__ bias,
                                                   bf16* __restrict__ out, int Cout) {
-  constexpr int PXW = 4 / COW;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
   const int P = s.B * s.H * s.W;
   const int Cin = s.C0 + s.C1;
-  const int pb = blockIdx.x * (32 * PXW) + (wave % PXW) * 32;
-  const int cb = blockIdx.y * (32 * COW) + (wave / PXW) * 32;
-  if (cb >= Cout || pb >= P) return;  // no LDS, no barrier: whole waves may leave
-  int pb_[2], py[2], px[2];
-  bool pv[2];
+  const int pb = (blockIdx.x * 4 + wave) * (16 * TM);
+  const int cb = blockIdx.y * (16 * TN);
+  if (pb >= P) return;  // no LDS, no barrier: whole waves may leave
+  int pbb[TM], py[TM], px[TM];
+  bool pv[TM];
 #pragma unroll
-  for (int g = 0; g < 2; g++) {
+  for (int g = 0; g < TM; g++) {
     const int p = pb + g * 16 + r;
     pv[g] = p < P;
     const int pp = pv[g] ? p : 0;
     px[g] = pp % s.W;
     const int q = pp / s.W;
     py[g] = q % s.H;
-    pb_[g] = q / s.H;
+    pbb[g] = q / s.H;
   }
-  bool cv[2];
-  const bf16* wrow[2];
+  const bf16* wrow[TN];
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int co = cb + h * 16 + r;
-    cv[h] = co < Cout;
-    wrow[h] = w + (size_t)(cv[h] ? co : 0) * 9 * Cin + kb * 8;
-  }
-  f32x4 acc[2][2];
+  for (int h = 0; h < TN; h++) wrow[h] = w + (size_t)(cb + h * 16 + r) * 9 * Cin + kb * 8;
+  f32x4 acc[TM][TN];
 #pragma unroll
-  for (int g = 0; g < 2; g++)
+  for (int g = 0; g < TM; g++)
 #pragma unroll
-    for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int t = 0; t < 9; t++) {
-    const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
-    for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
-      bf16x8 a[2], bm[2];
+    for (int h = 0; h < TN; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // K position of the next load: tap lt, channels lci .. lci + 31
+  int lt = 0, lci = 0;
+  auto load = [&](bf16x8 (&A)[TM], bf16x8 (&Bm)[TN]) {
+    const int dy = (lt / 3 - 1) * s.dil, dx = (lt % 3 - 1) * s.dil;
 #pragma unroll
-      for (int g = 0; g < 2; g++)
-        a[g] = pv[g] ? load8(s, pb_[g], py[g] + dy, px[g] + dx, ci0 + kb * 8) : zero8();
+    for (int g = 0; g < TM; g++)
+      A[g] = pv[g] ? load8(s, pbb[g], py[g] + dy, px[g] + dx, lci + kb * 8) : zero8();
 #pragma unroll
-      for (int h = 0; h < 2; h++)
-        bm[h] = cv[h] ? *reinterpret_cast<const bf16x8*>(wrow[h] + (size_t)t * Cin + ci0)
-                      : zero8();
-#pragma unroll
-      for (int g = 0; g < 2; g++)
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-          acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
+    for (int h = 0; h < TN; h++)
+      Bm[h] = *reinterpret_cast<const bf16x8*>(wrow[h] + (size_t)lt * Cin + lci);
+    lci += 32;
+    if (lci == Cin) {
+      lci = 0;
+      lt++;
     }
+  };
+  auto mma = [&](const bf16x8 (&A)[TM], const bf16x8 (&Bm)[TN]) {
+#pragma unroll
+    for (int g = 0; g < TM; g++)
+#pragma unroll
+      for (int h = 0; h < TN; h++)
+        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[g], Bm[h], acc[g][h], 0, 0, 0);
+  };
+  const int nk = 9 * (Cin / 32);
+  bf16x8 a0[TM], b0[TN], a1[TM], b1[TN];
+  load(a0, b0);
+  for (int kk = 0;;) {  // two steps per trip, operand sets swapped in place
+    if (kk + 1 < nk) load(a1, b1);
+    mma(a0, b0);
+    if (++kk == nk) break;
+    if (kk + 1 < nk) load(a0, b0);
+    mma(a1, b1);
+    if (++kk == nk) break;
   }
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
+  for (int h = 0; h < TN; h++) {
     const int co = cb + h * 16 + r;
-    if (co >= Cout) continue;
     const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-    for (int g = 0; g < 2; g++)
+    for (int g = 0; g < TM; g++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int p = pb + g * 16 + kb * 4 + i;
@@ -336,30 +351,51 @@ __global__ void k_relu_bwd(bf16* __restrict__ g, const bf16* __restrict__ y, lon
   reinterpret_cast<bf16x8*>(g)[i] = gv;
 }
 
+// Eight consecutive channels as floats, and back.
+__device__ inline void unpack8(const bf16x8 v, float* f) {
+#pragma unroll
+  for (int e = 0; e < 8; e++) f[e] = (float)v[e];
+}
+__device__ inline bf16x8 pack8(const float* f) {
+  bf16x8 v;
+#pragma unroll
+  for (int e = 0; e < 8; e++) v[e] = (__bf16)f[e];
+  return v;
+}
+__device__ inline bf16x8 ld8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ inline void st8(bf16* p, const bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
+
+// The elementwise kernels below take eight consecutive channels per thread
+// (16-byte accesses; every channel count is a multiple of 8 and the element
+// counts fit 32-bit indices: the host checks).
+
 // 2x2 average pooling, ceil size: out (ceil(H/2), ceil(W/2)).
 __global__ void k_pool(const bf16* __restrict__ in, bf16* __restrict__ out, int B, int H, int W,
                        int C) {
-  const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
-  const long n = (long)B * Ho * Wo * C;
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int c = (int)(i % C);
-  long q = i / C;
-  const int xo = (int)(q % Wo);
+  const int Ho = (H + 1) / 2, Wo = (W + 1) / 2, C8 = C / 8;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * Ho * Wo * C8) return;
+  const int cg = i % C8;
+  int q = i / C8;
+  const int xo = q % Wo;
   q /= Wo;
-  const int yo = (int)(q % Ho);
-  const int b = (int)(q / Ho);
-  float s = 0.0f;
+  const int yo = q % Ho;
+  const int b = q / Ho;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
   int cnt = 0;
   for (int dy = 0; dy < 2; dy++)
     for (int dx = 0; dx < 2; dx++) {
       const int y = 2 * yo + dy, x = 2 * xo + dx;
       if (y < H && x < W) {
-        s += (float)in[(((size_t)b * H + y) * W + x) * C + c];
+        unpack8(ld8(in + ((size_t)(b * H + y) * W + x) * C + cg * 8), f);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += f[e];
         cnt++;
       }
     }
-  out[i] = (__bf16)(s / (float)cnt);
+#pragma unroll
+  for (int e = 0; e < 8; e++) s[e] /= (float)cnt;
+  st8(out + (size_t)i * 8, pack8(s));
 }
 
 // Gradient of a skip tensor s (full size, C channels): the skip half of the
@@ -368,22 +404,28 @@ __global__ void k_pool(const bf16* __restrict__ in, bf16* __restrict__ out, int 
 __global__ void k_skip_grad(const bf16* __restrict__ dcat, int Ccat, const bf16* __restrict__ dpool,
                             const bf16* __restrict__ s, bf16* __restrict__ ds, int B, int H, int W,
                             int C) {
-  const long n = (long)B * H * W * C;
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int c = (int)(i % C);
-  long q = i / C;
-  const int x = (int)(q % W);
+  const int C8 = C / 8;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * H * W * C8) return;
+  const int cg = i % C8;
+  int q = i / C8;
+  const int x = q % W;
   q /= W;
-  const int y = (int)(q % H);
-  const int b = (int)(q / H);
+  const int y = q % H;
+  const int b = q / H;
   const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
   const int yo = y >> 1, xo = x >> 1;
-  const int cnt = (2 * yo + 1 < H ? 2 : 1) * (2 * xo + 1 < W ? 2 : 1);
-  float v = (float)dcat[(((size_t)b * H + y) * W + x) * Ccat + c] +
-            (float)dpool[(((size_t)b * Ho + yo) * Wo + xo) * C + c] / (float)cnt;
-  if (!((float)s[i] > 0.0f)) v = 0.0f;
-  ds[i] = (__bf16)v;
+  const float inv = 1.0f / (float)((2 * yo + 1 < H ? 2 : 1) * (2 * xo + 1 < W ? 2 : 1));
+  float v[8], d[8], sv[8];
+  unpack8(ld8(dcat + ((size_t)(b * H + y) * W + x) * Ccat + cg * 8), v);
+  unpack8(ld8(dpool + ((size_t)(b * Ho + yo) * Wo + xo) * C + cg * 8), d);
+  unpack8(ld8(s + (size_t)i * 8), sv);
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    v[e] += d[e] * inv;
+    if (!(sv[e] > 0.0f)) v[e] = 0.0f;
+  }
+  st8(ds + (size_t)i * 8, pack8(v));
 }
 
 // Gradient of the coarse (upsampled) input of a decoder conv: sum of the
@@ -392,38 +434,48 @@ __global__ void k_skip_grad(const bf16* __restrict__ dcat, int Ccat, const bf16*
 __global__ void k_up_grad(const bf16* __restrict__ dcat, int Ccat, int C0,
                           const bf16* __restrict__ y, bf16* __restrict__ dy_, int B, int H, int W,
                           int C1) {
-  const int Hc = (H + 1) / 2, Wc = (W + 1) / 2;
-  const long n = (long)B * Hc * Wc * C1;
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int c = (int)(i % C1);
-  long q = i / C1;
-  const int xc = (int)(q % Wc);
+  const int Hc = (H + 1) / 2, Wc = (W + 1) / 2, C8 = C1 / 8;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * Hc * Wc * C8) return;
+  const int cg = i % C8;
+  int q = i / C8;
+  const int xc = q % Wc;
   q /= Wc;
-  const int yc = (int)(q % Hc);
-  const int b = (int)(q / Hc);
-  float s = 0.0f;
+  const int yc = q % Hc;
+  const int b = q / Hc;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
   for (int dy = 0; dy < 2; dy++)
     for (int dx = 0; dx < 2; dx++) {
       const int yy = 2 * yc + dy, xx = 2 * xc + dx;
-      if (yy < H && xx < W) s += (float)dcat[(((size_t)b * H + yy) * W + xx) * Ccat + C0 + c];
+      if (yy < H && xx < W) {
+        unpack8(ld8(dcat + ((size_t)(b * H + yy) * W + xx) * Ccat + C0 + cg * 8), f);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += f[e];
+      }
     }
-  if (!((float)y[i] > 0.0f)) s = 0.0f;
-  dy_[i] = (__bf16)s;
+  unpack8(ld8(y + (size_t)i * 8), f);
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+    if (!(f[e] > 0.0f)) s[e] = 0.0f;
+  st8(dy_ + (size_t)i * 8, pack8(s));
 }
 
 // ------------------------------------------------------------ head, loss
-// out[p][k] = b[k] + sum_ci x[p][ci] w[k][ci]   (1x1 conv, FP32 output)
+// out[p][k] = b[k] + sum_ci x[p][ci] w[k][ci]   (1x1 conv, FP32 output; one
+// thread per pixel, K <= 3)
 __global__ void k_head_fwd(const bf16* __restrict__ x, const float* __restrict__ w,
                            const float* __restrict__ b, float* __restrict__ out, int P, int C,
                            int K) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= P) return;
-  for (int k = 0; k < K; k++) {
-    float s = b[k];
-    for (int c = 0; c < C; c++) s += (float)x[(size_t)p * C + c] * w[k * C + c];
-    out[(size_t)p * K + k] = s;
+  float s[3] = {0.f, 0.f, 0.f}, f[8];
+  for (int c0 = 0; c0 < C; c0 += 8) {
+    unpack8(ld8(x + (size_t)p * C + c0), f);
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) s[k] += f[e] * w[k * C + c0 + e];
   }
+  for (int k = 0; k < K; k++) out[(size_t)p * K + k] = b[k] + s[k];
 }
 
 // Masked mean-square error: per-workgroup partial sums of mask*(y-t)^2 and
@@ -454,53 +506,66 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const 
 // dx[p][c] = relu'(x) * sum_k dy[p][k] w[k][c]
 __global__ void k_head_bwd(const float* __restrict__ dy, const float* __restrict__ w,
                            const bf16* __restrict__ x, bf16* __restrict__ dx, int P, int C, int K) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (long)P * C) return;
-  const int c = (int)(i % C);
-  const long p = i / C;
-  float s = 0.0f;
-  for (int k = 0; k < K; k++) s += dy[p * K + k] * w[k * C + c];
-  if (!((float)x[i] > 0.0f)) s = 0.0f;
-  dx[i] = (__bf16)s;
+  const int C8 = C / 8;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P * C8) return;
+  const int cg = i % C8, p = i / C8;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+  for (int k = 0; k < K; k++) {
+    const float d = dy[(size_t)p * K + k];
+#pragma unroll
+    for (int e = 0; e < 8; e++) s[e] += d * w[k * C + cg * 8 + e];
+  }
+  unpack8(ld8(x + (size_t)i * 8), f);
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+    if (!(f[e] > 0.0f)) s[e] = 0.0f;
+  st8(dx + (size_t)i * 8, pack8(s));
 }
 
 // part[chunk][k*C + c] (weights) and part[chunk][K*C + k] (bias) of the head
-// (K = 3 outputs, C <= 256 inputs).  Threads are (lane, channel) pairs as in
-// k_colsum_part; channel-0 threads also sum the bias terms.
+// (K <= 3 outputs, C <= 2048 inputs).  Threads are (lane, group) pairs as in
+// k_colsum_part; group-0 threads also sum the bias terms.
 __global__ __launch_bounds__(256) void k_head_wgrad_part(const float* __restrict__ dy,
                                                          const bf16* __restrict__ x,
                                                          float* __restrict__ part, int P, int C,
                                                          int K, int px_per_chunk) {
-  __shared__ float red[256][7];
+  __shared__ float red[256][28];  // 24 weight + 3 bias sums (+1: odd row stride)
   const int p0 = blockIdx.x * px_per_chunk, p1 = min(P, p0 + px_per_chunk);
-  const int lanes = 256 / C;
-  const int t = threadIdx.x, c = t % C, lane = t / C;
-  float a[3] = {0.f, 0.f, 0.f}, bsum[3] = {0.f, 0.f, 0.f};
+  const int G = C / 8, lanes = 256 / G;
+  const int t = threadIdx.x, grp = t % G, lane = t / G;
+  float a[3][8], bsum[3] = {0.f, 0.f, 0.f}, f[8];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int e = 0; e < 8; e++) a[k][e] = 0.0f;
   if (lane < lanes)
     for (int p = p0 + lane; p < p1; p += lanes) {
-      const float xv = (float)x[(size_t)p * C + c];
+      unpack8(ld8(x + (size_t)p * C + grp * 8), f);
       for (int k = 0; k < K; k++) {
         const float d = dy[(size_t)p * K + k];
-        a[k] += d * xv;
+#pragma unroll
+        for (int e = 0; e < 8; e++) a[k][e] += d * f[e];
         bsum[k] += d;
       }
     }
+#pragma unroll
   for (int k = 0; k < 3; k++) {
-    red[t][k] = a[k];
-    red[t][3 + k] = bsum[k];
+#pragma unroll
+    for (int e = 0; e < 8; e++) red[t][k * 8 + e] = a[k][e];
+    red[t][24 + k] = bsum[k];
   }
   __syncthreads();
   const int nw = K * C + K;
-  if (t < C) {
-    for (int k = 0; k < K; k++) {
-      float r = 0.0f;
-      for (int l = 0; l < lanes; l++) r += red[l * C + t][k];
-      part[(size_t)blockIdx.x * nw + k * C + t] = r;
-    }
-  }
-  if (t < K) {  // bias: the channel-0 threads of every lane saw each pixel once
+  for (int o = t; o < K * C; o += 256) {
+    const int k = o / C, c = o % C;
     float r = 0.0f;
-    for (int l = 0; l < lanes; l++) r += red[l * C][3 + t];
+    for (int l = 0; l < lanes; l++) r += red[l * G + c / 8][k * 8 + c % 8];
+    part[(size_t)blockIdx.x * nw + o] = r;
+  }
+  if (t < K) {  // bias: the group-0 threads of every lane saw each pixel once
+    float r = 0.0f;
+    for (int l = 0; l < lanes; l++) r += red[l * G][24 + t];
     part[(size_t)blockIdx.x * nw + K * C + t] = r;
   }
 }
@@ -509,7 +574,7 @@ __global__ __launch_bounds__(256) void k_head_wgrad_part(const float* __restrict
 // Adam on the FP32 master copy; bf16 copy of the same element refreshed.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, bf16* __restrict__ pb, long n, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
+                       float b2, float eps, float c1, float c2, float wd) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -518,7 +583,7 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   m[i] = mi;
   v[i] = vi;
   const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  const float np = p[i] - upd;
+  const float np = p[i] - upd - lr * wd * p[i];  // decoupled weight decay (AdamW)
   p[i] = np;
   pb[i] = (__bf16)np;
 }
@@ -530,15 +595,19 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 __global__ void k_gather(const float* __restrict__ inputs, const float* __restrict__ targets,
                          const int* __restrict__ idx, bf16* __restrict__ x, float* __restrict__ t,
                          float* __restrict__ mask, int B, int HW, int Ci, int Cp) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (long)B * HW) return;
-  const int b = (int)(i / HW);
-  const long q = i % HW;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * HW) return;
+  const int b = i / HW, q = i % HW;
   const size_t src = ((size_t)idx[b] * HW + q) * Ci;
-  for (int c = 0; c < Cp; c++) x[i * Cp + c] = (__bf16)(c < Ci ? inputs[src + c] : 0.0f);
+  float f[8];
+  for (int c0 = 0; c0 < Cp; c0 += 8) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) f[e] = c0 + e < Ci ? inputs[src + c0 + e] : 0.0f;
+    st8(x + (size_t)i * Cp + c0, pack8(f));
+  }
   if (t) {
     const size_t ts = ((size_t)idx[b] * HW + q) * 3;
-    for (int c = 0; c < 3; c++) t[i * 3 + c] = targets[ts + c];
+    for (int c = 0; c < 3; c++) t[(size_t)i * 3 + c] = targets[ts + c];
     mask[i] = inputs[src];
   }
 }
@@ -551,18 +620,29 @@ static inline Src src_of(const ConvSrc& c) {
   return Src{cb(c.in0), c.C0, cb(c.in1), c.C1, c.B, c.H, c.W, c.dil};
 }
 
+template <int TM, int TN>
+static void launch_conv(const Src& s, const u16* w, const float* bias, u16* out, int Cout,
+                        long P, bool relu, hipStream_t st) {
+  dim3 g((unsigned)((P + 64 * TM - 1) / (64 * TM)), (unsigned)(Cout / (16 * TN)));
+  if (relu) k_conv3x3<TM, TN, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+  else k_conv3x3<TM, TN, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+}
+
+// Tile choice: 64 output channels per wave where Cout allows, 64 pixels per
+// wave while that still gives >= 1024 workgroups (4 per CU), else 32.
 hipError_t conv3x3(const ConvSrc& cs, const u16* w, const float* bias, u16* out, int Cout,
                    bool relu, hipStream_t st) {
   const Src s = src_of(cs);
   const long P = (long)cs.B * cs.H * cs.W;
-  if (Cout >= 64) {
-    dim3 g((unsigned)((P + 63) / 64), (unsigned)((Cout + 63) / 64));
-    if (relu) k_conv3x3<2, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
-    else k_conv3x3<2, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+  if (Cout % 32 || (cs.C0 + cs.C1) % 32 || P + 256 > 0x7fffffffL) return hipErrorInvalidValue;
+  const int TN = Cout % 64 == 0 ? 4 : 2;
+  const long wg4 = (P + 255) / 256 * (Cout / (16 * TN));
+  if (TN == 4) {
+    if (wg4 >= 1024) launch_conv<4, 4>(s, w, bias, out, Cout, P, relu, st);
+    else launch_conv<2, 4>(s, w, bias, out, Cout, P, relu, st);
   } else {
-    dim3 g((unsigned)((P + 127) / 128), (unsigned)((Cout + 31) / 32));
-    if (relu) k_conv3x3<1, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
-    else k_conv3x3<1, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+    if (wg4 >= 1024) launch_conv<4, 2>(s, w, bias, out, Cout, P, relu, st);
+    else launch_conv<2, 2>(s, w, bias, out, Cout, P, relu, st);
   }
   return hipGetLastError();
 }
@@ -615,28 +695,35 @@ hipError_t relu_bwd(u16* g, const u16* y, long n, hipStream_t st) {
   return hipGetLastError();
 }
 
+// the eight-channel elementwise kernels: C % 8 == 0 and n / 8 fits an int
+static inline bool vec_ok(long n, int C) { return C % 8 == 0 && n / 8 < 0x7fffff00L; }
+
 hipError_t pool(const u16* in, u16* out, int B, int H, int W, int C, hipStream_t st) {
-  k_pool<<<nblk((long)B * ((H + 1) / 2) * ((W + 1) / 2) * C), 256, 0, st>>>(cb(in), mb(out), B, H,
-                                                                            W, C);
+  const long n = (long)B * ((H + 1) / 2) * ((W + 1) / 2) * C;
+  if (!vec_ok((long)B * H * W * C, C)) return hipErrorInvalidValue;
+  k_pool<<<nblk(n / 8), 256, 0, st>>>(cb(in), mb(out), B, H, W, C);
   return hipGetLastError();
 }
 
 hipError_t skip_grad(const u16* dcat, int Ccat, const u16* dpool, const u16* s, u16* ds, int B,
                      int H, int W, int C, hipStream_t st) {
-  k_skip_grad<<<nblk((long)B * H * W * C), 256, 0, st>>>(cb(dcat), Ccat, cb(dpool), cb(s), mb(ds),
-                                                         B, H, W, C);
+  if (!vec_ok((long)B * H * W * Ccat, C) || Ccat % 8) return hipErrorInvalidValue;
+  k_skip_grad<<<nblk((long)B * H * W * C / 8), 256, 0, st>>>(cb(dcat), Ccat, cb(dpool), cb(s),
+                                                             mb(ds), B, H, W, C);
   return hipGetLastError();
 }
 
 hipError_t up_grad(const u16* dcat, int Ccat, int C0, const u16* y, u16* dy, int B, int H, int W,
                    int C1, hipStream_t st) {
-  k_up_grad<<<nblk((long)B * ((H + 1) / 2) * ((W + 1) / 2) * C1), 256, 0, st>>>(
+  if (!vec_ok((long)B * H * W * Ccat, C1) || Ccat % 8 || C0 % 8) return hipErrorInvalidValue;
+  k_up_grad<<<nblk((long)B * ((H + 1) / 2) * ((W + 1) / 2) * C1 / 8), 256, 0, st>>>(
       cb(dcat), Ccat, C0, cb(y), mb(dy), B, H, W, C1);
   return hipGetLastError();
 }
 
 hipError_t head_fwd(const u16* x, const float* w, const float* b, float* out, int P, int C, int K,
                     hipStream_t st) {
+  if (!vec_ok((long)P * C, C) || K > 3) return hipErrorInvalidValue;
   k_head_fwd<<<nblk(P), 256, 0, st>>>(cb(x), w, b, out, P, C, K);
   return hipGetLastError();
 }
@@ -649,26 +736,28 @@ hipError_t mse(const float* y, const float* t, const float* mask, float* dy, flo
 
 hipError_t head_bwd(const float* dy, const float* w, const u16* x, u16* dx, int P, int C, int K,
                     hipStream_t st) {
-  k_head_bwd<<<nblk((long)P * C), 256, 0, st>>>(dy, w, cb(x), mb(dx), P, C, K);
+  if (!vec_ok((long)P * C, C)) return hipErrorInvalidValue;
+  k_head_bwd<<<nblk((long)P * C / 8), 256, 0, st>>>(dy, w, cb(x), mb(dx), P, C, K);
   return hipGetLastError();
 }
 
 hipError_t head_wgrad_part(const float* dy, const u16* x, float* part, int P, int C, int K,
                            int px_per_chunk, hipStream_t st) {
-  if (C > 256 || 256 % C || K > 3) return hipErrorInvalidValue;
+  if (C % 8 || C > 2048 || K > 3) return hipErrorInvalidValue;
   k_head_wgrad_part<<<(unsigned)((P + px_per_chunk - 1) / px_per_chunk), 256, 0, st>>>(
       dy, cb(x), part, P, C, K, px_per_chunk);
   return hipGetLastError();
 }
 
 hipError_t adam(float* p, const float* g, float* m, float* v, u16* pb, long n, float lr, float b1,
-                float b2, float eps, float c1, float c2, hipStream_t st) {
-  k_adam<<<nblk(n), 256, 0, st>>>(p, g, m, v, mb(pb), n, lr, b1, b2, eps, c1, c2);
+                float b2, float eps, float c1, float c2, float wd, hipStream_t st) {
+  k_adam<<<nblk(n), 256, 0, st>>>(p, g, m, v, mb(pb), n, lr, b1, b2, eps, c1, c2, wd);
   return hipGetLastError();
 }
 
 hipError_t gather(const float* inputs, const float* targets, const int* idx, u16* x, float* t,
                   float* mask, int B, int HW, int Ci, int Cp, hipStream_t st) {
+  if (Cp % 8 || Ci > Cp || !vec_ok((long)B * HW * Cp, Cp)) return hipErrorInvalidValue;
   k_gather<<<nblk((long)B * HW), 256, 0, st>>>(inputs, targets, idx, mb(x), t, mask, B, HW, Ci,
                                               Cp);
   return hipGetLastError();

@@ -48,7 +48,7 @@ hipError_t head_bwd(const float* dy, const float* w, const u16* x, u16* dx, int 
 hipError_t head_wgrad_part(const float* dy, const u16* x, float* part, int P, int C, int K,
                            int px_per_chunk, hipStream_t st);
 hipError_t adam(float* p, const float* g, float* m, float* v, u16* pb, long n, float lr, float b1,
-                float b2, float eps, float c1, float c2, hipStream_t st);
+                float b2, float eps, float c1, float c2, float wd, hipStream_t st);
 hipError_t gather(const float* inputs, const float* targets, const int* idx, u16* x, float* t,
                   float* mask, int B, int HW, int Ci, int Cp, hipStream_t st);
 

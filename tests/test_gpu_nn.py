@@ -101,3 +101,26 @@ def test_full_size_grid_runs():
     l0 = net.train(np.arange(4), 1e-3)
     y = net.predict(inp)
     assert y.shape == inp.shape and np.isfinite(y).all() and np.isfinite(l0)
+
+
+def test_weight_decay_is_decoupled():
+    # the same net and batch with and without weight decay: the Adam part of
+    # the step is identical, so the parameters differ by -lr * wd * p
+    H = W = 17
+    inp, tgt = _sample(4, H, W, 7)
+    nets = [lbm.Net(H, W, batch=4, base=32, levels=3, seed=5) for _ in range(2)]
+    for n in nets:
+        n.set_data(inp, tgt)
+    nets[1].set_weight_decay(0.1)
+    p0 = nets[0].params()
+    lr = 1e-3
+    for n in nets:
+        n.train(np.arange(4), lr)
+    d = nets[1].params().astype(np.float64) - nets[0].params()
+    # float32 parameters: the difference is exact to about one ulp of p
+    assert (np.abs(d + lr * 0.1 * p0) <= 2.5e-7 * np.abs(p0) + 1e-10).all()
+    assert np.abs(d).max() > 0
+    with pytest.raises(lbm.LbmError):
+        nets[0].set_weight_decay(-1.0)
+    for n in nets:
+        n.close()

@@ -32,6 +32,15 @@ def test_encode_channels_and_mask():
     assert x[0, 0, 3] == 0 and x[128, 256, 4] == 0 and x[128, 100, 6] == 0
     assert np.isclose(x[100, 50, 3], 25 / 128) and np.isclose(x[100, 50, 5], 50 / 128)
     assert not x[129:, :, 3:].any()
+    # distance to the nearest solid node: 1 node next to the west wall, 0 in
+    # the obstacle; at node (64, 32) the south and north walls are 33 away
+    # (obstacle 45, west/east 65); at node (100, 32) the east wall, 29 away
+    assert np.isclose(x[100, 0, 7], 1 / 128)
+    assert x[2 * 15, 2 * 15, 7] == 0 and x[2 * 15, 2 * 40, 7] > 0
+    assert np.isclose(x[2 * 32, 2 * 64, 7], 33 / 128)
+    assert np.isclose(x[2 * 32, 2 * 100, 7], 29 / 128)
+    # next to the obstacle's east face (node i = 20 is solid, 1-based)
+    assert np.isclose(x[2 * 15, 2 * 20, 7], 1 / 128)
 
 
 def test_resampling_round_trip_is_exact_on_grid_aligned_nodes():
@@ -79,3 +88,21 @@ def test_channel_scales_are_masked_rms():
     t[1] = 100.0  # outside the fluid: ignored
     s = WS.channel_scales(inp, t)
     assert np.allclose(s, [3.0, 1.0, 0.5])
+
+
+def test_separable_bilinear_equals_the_pointwise_formula():
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=(9, 7, 3))
+    gy = np.concatenate([rng.uniform(-1, 9, 11), [0.0, 8.0]])
+    gx = np.concatenate([rng.uniform(-1, 7, 5), [0.0, 6.0, 2.5]])
+    out = WS._bilinear(f, gy, gx)
+    assert out.shape == (gy.size, gx.size, 3)
+    for a, y in enumerate(gy):
+        for b, x in enumerate(gx):
+            y_, x_ = min(max(y, 0), 8), min(max(x, 0), 6)
+            y0, x0 = int(np.floor(y_)), int(np.floor(x_))
+            y1, x1 = min(y0 + 1, 8), min(x0 + 1, 6)
+            ty, tx = y_ - y0, x_ - x0
+            ref = ((1 - ty) * (1 - tx) * f[y0, x0] + (1 - ty) * tx * f[y0, x1] +
+                   ty * (1 - tx) * f[y1, x0] + ty * tx * f[y1, x1])
+            assert np.allclose(out[a, b], ref, rtol=1e-12, atol=1e-12)
