@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--workload", default="cavity",
                     choices=["cavity", "sweep", "warmstart", "netstep", "matrix", "cavity129",
-                             "cylinder"],
+                             "cylinder", "io"],
                     help="cavity: the single 4097^2 lattice; sweep: the batched cavity sweep; "
                          "warmstart: learned start vs rest start on held-out cavities; "
                          "netstep: training steps of the warm-start network; "
@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--max-steps", type=int, default=400000, help="sweep: step cap")
     ap.add_argument("--single-sample", type=int, default=8,
                     help="sweep: cavities also run one at a time")
+    ap.add_argument("--io-final", action="store_true",
+                    help="io: also time the final-layout CSV (f0..f8 added, about 2.5x the size)")
     ap.add_argument("--train-count", type=int, default=2000, help="warmstart: training cavities")
     ap.add_argument("--test-count", type=int, default=200, help="warmstart: held-out cavities")
     ap.add_argument("--epochs", type=int, default=120, help="warmstart: training epochs")
@@ -129,6 +131,8 @@ def main():
         return bench_cavity129(args)
     if args.workload == "cylinder":
         return bench_cylinder(args)
+    if args.workload == "io":
+        return bench_io(args)
 
     N = args.N
     coll = {"bgk": lbm.BGKW, "trt": lbm.TRT, "mrt": lbm.MRT}[args.collision]
@@ -405,6 +409,114 @@ def bench_cylinder(args):
         "strouhal_rel_diff_f32_vs_f64": abs(res["f32"]["strouhal"] - res["f64"]["strouhal"]) /
         res["f64"]["strouhal"],
         "cd_rel_diff_f32_vs_f64": abs(res["f32"]["cd"] - res["f64"]["cd"]) / abs(res["f64"]["cd"]),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_io(args):
+    """Result files and checkpoints at scale: a 4097^2 cavity (MRT, FP32
+    storage) after a few steps; times WriteResults-style CSV and Tecplot
+    files (x, y, u, v, |u|, rho, p, fluid per cell), the final-layout CSV
+    (plus f0..f8), the binary checkpoint save + load, each on the host of
+    the GPU in use (device-to-host copy included), and the mesh readers on
+    the cavity's own D2node.dat / BCconnectors.dat.  CPU baseline: the
+    oracle's sequential fprintf writer (the reference's WriteResults,
+    serial/FileIO.c:298-348) on a 1025^2 cavity, 1 thread."""
+    import tempfile
+    import parallellbmcranfield_amd as lbm
+    N = args.N
+    visc = 0.1 * (N - 1) / 1000.0
+    tau = 3 * float(__import__("numpy").float32(visc)) + 0.5
+    rates = [0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, 1 / tau, 1 / tau]
+    o = lbm.make_options(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=visc,
+                         init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE, use_stop_rule=0,
+                         mrt_textbook_meq=1, mrt_rates=rates)
+    o.precision = lbm.PREC_F32
+    case = lbm.Case.cavity(N)
+    s = lbm.Solver(case, o)
+    s.step(100)
+    s.sync()
+    res = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as d:
+        for name, fmt, final in (("csv", lbm.FORMAT_CSV, False), ("tecplot", lbm.FORMAT_TECPLOT, False),
+                                 ("csv_final", lbm.FORMAT_CSV, True)):
+            if final and not args.io_final:
+                continue
+            path = os.path.join(d, name)
+            t0 = time.perf_counter()
+            s.write_results(path, fmt, final=final)
+            dt = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            os.remove(path)
+            res[name] = {"seconds": round(dt, 3), "bytes": size, "MB_per_s": round(size / dt / 1e6, 1)}
+            log(f"io: {name} {size / 1e9:.2f} GB in {dt:.2f} s ({size / dt / 1e6:.0f} MB/s)")
+        path = os.path.join(d, "ckpt")
+        t0 = time.perf_counter()
+        s.save(path)
+        t_save = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        s.load(path)
+        s.sync()
+        t_load = time.perf_counter() - t0
+        os.remove(path)
+        res["checkpoint"] = {"bytes": size, "save_s": round(t_save, 3), "load_s": round(t_load, 3),
+                             "save_GB_per_s": round(size / t_save / 1e9, 2),
+                             "load_GB_per_s": round(size / t_load / 1e9, 2)}
+        log(f"io: checkpoint {size / 1e9:.2f} GB, save {t_save:.2f} s, load {t_load:.2f} s")
+    s.close()
+    # the mesh readers (ReadNodes + ReadBCconn + CellIni) on the same 4097^2
+    # cavity's mesh files in the reference's text format
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as d:
+        pn, pc = os.path.join(d, "D2node.dat"), os.path.join(d, "BCconnectors.dat")
+        case.write(pn, pc)
+        size = os.path.getsize(pn) + os.path.getsize(pc)
+        t0 = time.perf_counter()
+        cm = lbm.Case.load(pn, pc)
+        dt = time.perf_counter() - t0
+        assert cm.info.num_fluid == case.info.num_fluid
+        del cm
+    res["mesh_read"] = {"bytes": size, "seconds": round(dt, 3), "MB_per_s": round(size / dt / 1e6, 1)}
+    log(f"io: mesh read {size / 1e6:.0f} MB in {dt:.2f} s ({size / dt / 1e6:.0f} MB/s)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        import meshgen
+        import oracle as O
+        Nc = 1025
+        nodes, conn = meshgen.cavity_arrays_fast(Nc)
+        orc = O.Oracle(nodes, conn, O.params(collision=3, init_rest=1, uavg=0.1, rho_ini=1.0,
+                                             viscosity=0.1 * (Nc - 1) / 1000.0))
+        del nodes, conn
+        orc.step(1)
+        with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as d:
+            path = os.path.join(d, "o.csv")
+            t0 = time.perf_counter()
+            orc.write_results(path, final=False)
+            dt = time.perf_counter() - t0
+            size = os.path.getsize(path)
+        cpu = {"value": round(size / dt / 1e6, 1), "unit": "MB/s", "cores": 1, "kind": "port",
+               "sample": f"{Nc}x{Nc} cavity results CSV ({size / 1e6:.0f} MB) by the oracle's "
+                         f"sequential fprintf writer (oracle/lbm_oracle.c, 1 thread, {dt:.1f} s)"}
+        log(f"io: cpu baseline {cpu['value']} MB/s")
+    out = {
+        "metric": "results CSV writer throughput",
+        "value": res["csv"]["MB_per_s"],
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "text (f32 lattice)",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {N}x{N} fluid nodes, MRT, FP32 storage, after "
+                               f"100 steps; results files and checkpoint on the host's temp dir",
+                   "parallelism": "single GPU + host threads"},
+        "io": res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

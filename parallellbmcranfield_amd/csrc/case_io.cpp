@@ -1,6 +1,7 @@
 // case_io.cpp — case input (ini, mesh files), mesh generators and the
 // per-cell set-up of the reference's CellIni, restated in O(N).
 #include <algorithm>
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -9,6 +10,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lbm_host.hpp"
@@ -53,9 +55,94 @@ static bool scan_float(const char*& p, float* out) {
 // serial/FileIO.c:14-82 (ReadNodes) and :91-174 (ReadBCconn): count the
 // records that parse with the typed pattern, then read that many records
 // with float conversions for every field.
+// The same tables parsed on several threads, for the usual file shape: one
+// record per line, integer fields written as integers.  The buffer is cut at
+// line ends; each piece parses its lines with the same strtof calls as the
+// sequential reader.  Returns the record count, or -2 when a line does not
+// hold exactly one clean record (two records on a line, a record across
+// lines, "3.5" in an integer field): the sequential reader then decides,
+// since fscanf's counting pass and its reading pass can disagree there.
+static int read_table_lines(const std::string& buf, int cols, const int* pat,
+                            std::vector<float>& t) {
+  const long N = (long)buf.size();
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+  std::vector<long> cut(nt + 1, N);
+  cut[0] = 0;
+  for (unsigned k = 1; k < nt; k++) {  // piece k starts after a newline
+    long q = std::max(cut[k - 1], N * (long)k / (long)nt);
+    while (q < N && buf[(size_t)q] != '\n') q++;
+    cut[k] = q < N ? q + 1 : N;
+  }
+  struct Piece {
+    std::vector<float> v;
+    bool stop = false;   // a field failed to parse: the table ends here
+    bool clean = true;   // every record was one clean line
+  };
+  std::vector<Piece> pc(nt);
+  auto work = [&](unsigned k) {
+    Piece& P = pc[k];
+    const char* p = buf.c_str() + cut[k];
+    const char* end = buf.c_str() + cut[k + 1];
+    P.v.reserve((size_t)((cut[k + 1] - cut[k]) / 6));
+    for (;;) {
+      while (p < end && std::isspace((unsigned char)*p)) p++;
+      if (p >= end) return;
+      float rec[8];
+      for (int c = 0; c < cols; c++) {
+        char* e;
+        if (pat[c] == 0) {
+          (void)std::strtol(p, &e, 10);
+          if (e == p) {
+            P.stop = true;
+            return;
+          }
+          if (*e && !std::isspace((unsigned char)*e)) {  // "3.5", "0x10" in an integer field
+            P.clean = false;
+            return;
+          }
+        }
+        rec[c] = std::strtof(p, &e);
+        if (e == p) {
+          P.stop = true;
+          return;
+        }
+        p = e;
+        // a record must not run past its line, nor stop before it ends
+        const bool last = c == cols - 1;
+        const char* q = p;
+        while (q < end && *q != '\n' && std::isspace((unsigned char)*q)) q++;
+        const bool eol = q >= end || *q == '\n';
+        if (eol != last || p > end) {
+          P.clean = false;
+          return;
+        }
+      }
+      P.v.insert(P.v.end(), rec, rec + cols);
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned k = 1; k < nt; k++) th.emplace_back(work, k);
+  work(0);
+  for (auto& x : th) x.join();
+  t.clear();
+  for (unsigned k = 0; k < nt; k++) {
+    if (!pc[k].clean) return -2;
+    t.insert(t.end(), pc[k].v.begin(), pc[k].v.end());
+    if (pc[k].stop) break;  // fscanf's count stops at the first failing field
+  }
+  return (int)(t.size() / (size_t)cols);
+}
+
 static int read_table(const char* path, int cols, std::vector<float>& t) {
   std::string buf;
   if (!read_file(path, buf)) return -1;
+  static const int node_pat5[5] = {0, 0, 1, 1, 0};
+  static const int conn_pat7[7] = {0, 0, 0, 0, 1, 1, 0};
+  if (buf.size() >= (1u << 20)) {
+    const int lines = read_table_lines(buf, cols, cols == 5 ? node_pat5 : conn_pat7, t);
+    if (lines >= 0) return lines;
+  }
   const char* p = buf.c_str();
   // typed pattern: nodes "%d %d %f %f %d", connectors "%d %d %d %d %f %f %d"
   static const int node_pat[5] = {0, 0, 1, 1, 0};

@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "fmt_fixed.hpp"
 #include "lbm_host.hpp"
 
 using namespace lbmhost;
@@ -31,10 +32,34 @@ std::string join(const std::string& dir, const char* name) {
 
 namespace {
 
+// One result-file line: the values a[0..na) as printf "%f", the int flag as
+// "%d", then b[0..nb) as "%f", separated by sep, ended by a newline: the
+// bytes fprintf gives for the reference's format strings, without the
+// printf machinery.  buf holds 8192 characters (a "%f" of a double is at
+// most 317).
+int fmt_row(char* buf, const double* a, int na, int flag, const double* b, int nb,
+            const char* sep) {
+  const int ls = (int)std::strlen(sep);
+  int len = 0;
+  for (int k = 0; k < na; k++) {
+    len += lbmfmt::fmt_f6(a[k], buf + len, 8192 - len);
+    std::memcpy(buf + len, sep, (size_t)ls);
+    len += ls;
+  }
+  len += lbmfmt::fmt_int(flag, buf + len);
+  for (int k = 0; k < nb; k++) {
+    std::memcpy(buf + len, sep, (size_t)ls);
+    len += ls;
+    len += lbmfmt::fmt_f6(b[k], buf + len, 8192 - len);
+  }
+  buf[len++] = '\n';
+  return len;
+}
+
 // Formats rows [r0, r1) of a result file into `out` (row -> one text line).
 template <class RowFn>
 void format_rows(long r0, long r1, RowFn row, std::string& out) {
-  char line[512];
+  char line[8192];
   for (long r = r0; r < r1; r++) {
     const int len = row(r, line, sizeof line);
     out.append(line, (size_t)(len < (int)sizeof line ? len : (int)sizeof line - 1));
@@ -43,7 +68,8 @@ void format_rows(long r0, long r1, RowFn row, std::string& out) {
 
 // Writes `nrows` lines produced by `row` after `header`, formatting chunks of
 // rows on several threads and writing them in order (same bytes as a
-// sequential fprintf loop).
+// sequential fprintf loop).  Two sets of chunk buffers alternate: one batch
+// is written by a writer thread while the next is formatted.
 template <class RowFn>
 int write_lines(const char* path, const std::string& header, long nrows, RowFn row) {
   FILE* fp = std::fopen(path, "w");
@@ -56,24 +82,32 @@ int write_lines(const char* path, const std::string& header, long nrows, RowFn r
   if (nt == 0) nt = 1;
   if (nt > 16) nt = 16;
   if (nrows < 20000) nt = 1;
-  const long chunk = 1 << 18;  // rows per block
-  std::vector<std::string> bufs(nt);
+  const long chunk = 1 << 16;  // rows per block
+  std::vector<std::string> bufs[2] = {std::vector<std::string>(nt), std::vector<std::string>(nt)};
+  std::thread writer;
+  int cur = 0;
   for (long base = 0; base < nrows; base += chunk * (long)nt) {
+    std::vector<std::string>& bs = bufs[cur];
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; t++) {
       const long r0 = base + (long)t * chunk;
       const long r1 = std::min(nrows, r0 + chunk);
-      bufs[t].clear();
+      bs[t].clear();
       if (r0 >= r1) continue;
       if (nt == 1)
-        format_rows(r0, r1, row, bufs[t]);
+        format_rows(r0, r1, row, bs[t]);
       else
-        th.emplace_back([&, t, r0, r1] { format_rows(r0, r1, row, bufs[t]); });
+        th.emplace_back([&, t, r0, r1] { format_rows(r0, r1, row, bs[t]); });
     }
     for (auto& x : th) x.join();
-    for (unsigned t = 0; t < nt; t++)
-      if (!bufs[t].empty()) std::fwrite(bufs[t].data(), 1, bufs[t].size(), fp);
+    if (writer.joinable()) writer.join();  // the previous batch is on its way out
+    writer = std::thread([&bs, fp] {
+      for (const std::string& b : bs)
+        if (!b.empty()) std::fwrite(b.data(), 1, b.size(), fp);
+    });
+    cur ^= 1;
   }
+  if (writer.joinable()) writer.join();
   const bool bad = std::ferror(fp) != 0;
   std::fclose(fp);
   if (bad) {
@@ -112,11 +146,12 @@ extern "C" int lbm_solver_write_results(lbm_solver* s, const char* path, int for
       std::snprintf(head, sizeof head, "Zone i=%d, j=%d, f=point\n", n, m);
       header += head;
     }
-    const char* fmt = csv ? "%f, %f, %f, %f, %f, %f, %f, %d\n" : "%f %f %f %f %f %f %f %d\n";
-    return write_lines(path, header, nc, [&](long i, char* buf, size_t cap) {
-      return std::snprintf(buf, cap, fmt, c->x(i), c->y(i), u[i], v[i],
-                           std::sqrt(u[i] * u[i] + v[i] * v[i]), rho[i], rho[i] / 3.0,
-                           c->fluid[i]);
+    // "%f, %f, %f, %f, %f, %f, %f, %d\n" (CSV) or space-separated (Tecplot)
+    const char* sep = csv ? ", " : " ";
+    return write_lines(path, header, nc, [&](long i, char* buf, size_t) {
+      const double a[7] = {c->x(i), c->y(i), u[i], v[i], std::sqrt(u[i] * u[i] + v[i] * v[i]),
+                           rho[i], rho[i] / 3.0};
+      return fmt_row(buf, a, 7, c->fluid[i], nullptr, 0, sep);
     });
   }
   if (csv) {
@@ -127,14 +162,13 @@ extern "C" int lbm_solver_write_results(lbm_solver* s, const char* path, int for
     std::snprintf(head, sizeof head, "Zone i=%d, j=%d, f=point\n", n, m);
     header += head;
   }
-  const char* fmt = csv ? "%f, %f, %f, %f, %f, %f, %f, %d, %f, %f, %f, %f, %f, %f, %f, %f, %f\n"
-                        : "%f %f %f %f %f %f %f %d %f %f %f %f %f %f %f %f %f\n";
-  return write_lines(path, header, nc, [&](long r, char* buf, size_t cap) {
+  // "%f, %f, %f, %f, %f, %f, %f, %d, %f x 9\n" (CSV) or space-separated (Tecplot)
+  const char* sep = csv ? ", " : " ";
+  return write_lines(path, header, nc, [&](long r, char* buf, size_t) {
     const long q = row0_bug ? (r % n) : r;  // FileIO.c:368 indexes Cells+i for every row j
-    const double* F = &f[(size_t)q * 9];
-    return std::snprintf(buf, cap, fmt, c->x(q), c->y(q), u[q], v[q],
-                         std::sqrt(u[q] * u[q] + v[q] * v[q]), rho[q], rho[q] / 3,
-                         c->fluid[q], F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8]);
+    const double a[7] = {c->x(q), c->y(q), u[q], v[q], std::sqrt(u[q] * u[q] + v[q] * v[q]),
+                         rho[q], rho[q] / 3};
+    return fmt_row(buf, a, 7, c->fluid[q], &f[(size_t)q * 9], 9, sep);
   });
 }
 

@@ -78,6 +78,30 @@ def test_file_parser_matches_fscanf(tmp_path):
     _cells_match(case, o)
 
 
+@pytest.mark.parametrize("variant", ["clean", "joined_lines", "stop_token"])
+def test_large_mesh_parser_matches_fscanf(tmp_path, variant):
+    # files over 1 MB are parsed on several threads when every line holds one
+    # clean record; otherwise (two records on a line) the sequential reader
+    # decides.  A token that fails to parse ends the table in both.
+    mesh = meshgen.cavity(257)
+    mesh.write(str(tmp_path))
+    nf = str(tmp_path / "Mesh" / "D2node.dat")
+    bf = str(tmp_path / "Mesh" / "BCconnectors.dat")
+    assert os.path.getsize(nf) > (1 << 20)
+    if variant != "clean":
+        lines = open(nf).read().split("\n")
+        if variant == "joined_lines":
+            lines[40000] = lines[40000] + "  " + lines.pop(40001)
+        else:
+            lines.insert(50000, "END")
+        open(nf, "w").write("\n".join(lines))
+    case = lbm.Case.load(nf, bf)
+    o = O.Oracle.from_files(nf, bf, O.params())
+    nn = 50000 if variant == "stop_token" else 259 * 260
+    assert (case.n, case.m, case.info.num_nodes) == (o.n, o.m, nn)
+    _cells_match(case, o)
+
+
 def test_shipped_coarse_case_parses_identically():
     path = "/root/reference/cases/coarse/Mesh"
     if not os.path.isdir(path):
