@@ -43,6 +43,7 @@ DILATION = 4  # bottleneck stencils dilated by 4 and 8 at 33^2: the field of vie
 # poorly, and a lighter density loss leaves capacity to the velocity (held-out
 # seed error 0.13 -> 0.10 at 0.1)
 RHO_WEIGHT = 0.1
+RHO_ASINH = 0.0  # > 0: density target through asinh(x / s0), s0 this quantile of |x|
 WEIGHT_DECAY = 0.05  # decoupled weight decay of the Adam step (AdamW)
 RE_LO, RE_SPAN = math.log(100.0), math.log(50.0)
 
@@ -186,19 +187,49 @@ def channel_scales(inputs: np.ndarray, targets: np.ndarray) -> np.ndarray:
     return np.sqrt(np.maximum(ms, 1e-12)).astype(np.float32)
 
 
+def to_net(net: Net, targets: np.ndarray) -> np.ndarray:
+    """Normalised targets [..., 3] (``field_to_grid``) -> the network's
+    units: the density channel through asinh(x / s0) when ``net.rho_s0`` is
+    set (linear in the bulk, logarithmic at the lid corners' pressure
+    peaks), then every channel divided by ``net.target_scales``."""
+    t = np.array(targets, np.float32, copy=True)
+    s0 = getattr(net, "rho_s0", None)
+    if s0:
+        t[..., 2] = np.arcsinh(t[..., 2] / s0)
+    return t / getattr(net, "target_scales", np.ones(3, np.float32))
+
+
+def from_net(net: Net, pred: np.ndarray) -> np.ndarray:
+    """The inverse of ``to_net``."""
+    t = pred * getattr(net, "target_scales", np.ones(3, np.float32))
+    s0 = getattr(net, "rho_s0", None)
+    if s0:
+        t[..., 2] = s0 * np.sinh(t[..., 2])
+    return t
+
+
 def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
           lr: float = 1e-3, seed: int = 0, log=None, rho_weight: float = RHO_WEIGHT,
-          weight_decay: float = WEIGHT_DECAY) -> list[float]:
+          weight_decay: float = WEIGHT_DECAY, rho_asinh: float = RHO_ASINH) -> list[float]:
     """Adam over the set with a cosine learning-rate decay on targets scaled
     per channel (``channel_scales``, kept as ``net.target_scales``); returns
     the mean loss of every epoch.  rho_weight: weight of the density channel
     in the loss relative to u and v (its scale grows by 1/sqrt(rho_weight)),
-    weight_decay: decoupled weight decay of the Adam step."""
-    sc = channel_scales(inputs, targets)
+    weight_decay: decoupled weight decay of the Adam step, rho_asinh: if
+    > 0, the density target passes through asinh(x / s0) with s0 the
+    rho_asinh quantile of |x| over the fluid points (``to_net``)."""
+    net.rho_s0 = None
+    if rho_asinh > 0:
+        x = np.abs(targets[..., 2][inputs[..., 0] > 0])
+        net.rho_s0 = float(max(np.quantile(x, rho_asinh), 1e-6))
+    net.target_scales = np.ones(3, np.float32)
+    t = to_net(net, targets)
+    sc = channel_scales(inputs, t)
     sc[2] /= math.sqrt(rho_weight)
     net.target_scales = sc
     net.set_weight_decay(weight_decay)
-    net.set_data(inputs, targets / net.target_scales)
+    net.set_data(inputs, t / sc)
+    del t
     rng = np.random.default_rng(seed)
     N, B = inputs.shape[0], net.batch
     steps_per_epoch = max(1, N // B)
@@ -222,14 +253,13 @@ def warm_fields(net: Net, specs, density: bool = True):
     The predicted density is shifted so the cavity holds the mass of the rest
     start (mean rho = 1 over its fluid nodes), as the walls conserve mass;
     density=False seeds rho = 1 with the predicted velocity."""
-    scales = getattr(net, "target_scales", np.ones(3, np.float32))
     X = np.empty((len(specs), net.H, net.H, IN_CH), np.float32)
 
     def enc(k):
         X[k] = encode(specs[k], net.H)
 
     list(_pool().map(enc, range(len(specs))))
-    pred = net.predict(X) * scales
+    pred = from_net(net, net.predict(X))
     return list(_pool().map(lambda sp: _seed(sp[0], sp[1], net.H, density), zip(specs, pred)))
 
 
@@ -327,7 +357,6 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
     diff, pred = np.asarray(diff), np.asarray(pred)
     # held-out loss per channel, in the network's scaled units, against the
     # rest start's converged fields
-    scales = getattr(net, "target_scales", np.ones(3, np.float32))
     X = np.stack([encode(sp, net.H) for sp in specs])
     P = net.predict(X)
     mse = np.zeros(3)
@@ -336,8 +365,8 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
     for k, sp in enumerate(specs):
         x0, y0, w, h = packed.rects[k]
         sl = (slice(y0, y0 + h), slice(x0, x0 + w))
-        T = field_to_grid(sp, *(a.reshape(packed.m, packed.n)[sl].ravel()
-                                for a in (rest_rho, rest["u"], rest["v"])), net.H) / scales
+        T = to_net(net, field_to_grid(sp, *(a.reshape(packed.m, packed.n)[sl].ravel()
+                                            for a in (rest_rho, rest["u"], rest["v"])), net.H))
         m = X[k, ..., 0:1]
         mse += (m * (P[k] - T) ** 2).sum(axis=(0, 1))
         wsum += float(m.sum())
@@ -358,6 +387,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         predict_s=warm["predict_s"])
 
 
-__all__ = ["GRID", "IN_CH", "DILATION", "RHO_WEIGHT", "WEIGHT_DECAY", "encode", "field_to_grid",
+__all__ = ["GRID", "IN_CH", "DILATION", "RHO_WEIGHT", "WEIGHT_DECAY", "RHO_ASINH", "encode",
+           "field_to_grid", "to_net", "from_net",
            "grid_to_field", "steady_states", "channel_scales", "train", "warm_fields",
            "packed_field", "compare_starts"]

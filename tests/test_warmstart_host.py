@@ -106,3 +106,18 @@ def test_separable_bilinear_equals_the_pointwise_formula():
             ref = ((1 - ty) * (1 - tx) * f[y0, x0] + (1 - ty) * tx * f[y0, x1] +
                    ty * (1 - tx) * f[y1, x0] + ty * tx * f[y1, x1])
             assert np.allclose(out[a, b], ref, rtol=1e-12, atol=1e-12)
+
+
+def test_density_transform_round_trip():
+    class _Net:  # the attributes train() leaves on a network
+        target_scales = np.array([0.2, 0.15, 1.3], np.float32)
+        rho_s0 = 0.05
+    rng = np.random.default_rng(4)
+    t = rng.normal(size=(2, 5, 5, 3)).astype(np.float32)
+    t[..., 2] *= 10.0  # corner-peak-like magnitudes
+    n = WS.to_net(_Net, t)
+    assert np.allclose(n[..., :2], t[..., :2] / _Net.target_scales[:2])
+    assert np.allclose(n[..., 2], np.arcsinh(t[..., 2] / 0.05) / 1.3, rtol=1e-5)
+    assert np.allclose(WS.from_net(_Net, n), t, rtol=1e-4, atol=1e-5)
+    _Net.rho_s0 = None
+    assert np.allclose(WS.to_net(_Net, t), t / _Net.target_scales)
