@@ -303,22 +303,31 @@ __global__ __launch_bounds__(256) void k_colsum_part(const bf16* __restrict__ g,
   }
 }
 
-// out[i] = sum_c part[c][i] in a fixed order (deterministic): four running
-// sums over c = 0, 1, 2, 3 (mod 4), so four loads are in flight per step.
-__global__ void k_reduce_chunks(const float* __restrict__ part, int nchunk, long n,
-                                float* __restrict__ out) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-  int c = 0;
-  for (; c + 4 <= nchunk; c += 4) {
-    a0 += part[(size_t)c * n + i];
-    a1 += part[(size_t)(c + 1) * n + i];
-    a2 += part[(size_t)(c + 2) * n + i];
-    a3 += part[(size_t)(c + 3) * n + i];
+// out[i] = sum_c part[c][i] in a fixed order (deterministic): a workgroup
+// takes 32 consecutive outputs; its 8 chunk lanes each sum every 8th chunk
+// (c = lane, lane + 8, ...) with two running sums, and the 8 lane sums are
+// added in lane order through LDS.
+__global__ __launch_bounds__(256) void k_reduce_chunks(const float* __restrict__ part, int nchunk,
+                                                       long n, float* __restrict__ out) {
+  __shared__ float red[8][33];
+  const int o = threadIdx.x & 31, lane = threadIdx.x >> 5;
+  const long i = (long)blockIdx.x * 32 + o;
+  float a0 = 0.0f, a1 = 0.0f;
+  if (i < n) {
+    int c = lane;
+    for (; c + 8 < nchunk; c += 16) {
+      a0 += part[(size_t)c * n + i];
+      a1 += part[(size_t)(c + 8) * n + i];
+    }
+    if (c < nchunk) a0 += part[(size_t)c * n + i];
   }
-  for (; c < nchunk; c++) a0 += part[(size_t)c * n + i];
-  out[i] = (a0 + a1) + (a2 + a3);
+  red[lane][o] = a0 + a1;
+  __syncthreads();
+  if (lane == 0 && i < n) {
+    float r = 0.0f;
+    for (int l = 0; l < 8; l++) r += red[l][o];
+    out[i] = r;
+  }
 }
 
 // ------------------------------------------------------------ elementwise
@@ -674,7 +683,7 @@ hipError_t colsum_part(const u16* g, int P, int C, int px_per_chunk, float* part
 }
 
 hipError_t reduce_chunks(const float* part, int nchunk, long n, float* out, hipStream_t st) {
-  k_reduce_chunks<<<nblk(n), 256, 0, st>>>(part, nchunk, n, out);
+  k_reduce_chunks<<<(unsigned)((n + 31) / 32), 256, 0, st>>>(part, nchunk, n, out);
   return hipGetLastError();
 }
 
