@@ -337,6 +337,11 @@ int lbm_net_train(lbm_net* n, const int* idx, int nb, float lr, double* loss);
 /* Decoupled weight decay of the Adam step (AdamW): every parameter also
  * moves by -lr * wd * p.  0 (the default) is plain Adam; 0 <= wd <= 1. */
 int lbm_net_set_weight_decay(lbm_net* n, float wd);
+/* Adds weight * sum d^2 to the loss (same normalisation as the squared
+ * error), d the central-difference divergence su dy0/dx + sv dy1/dy of the
+ * predicted velocity channels (su, sv: their scales to physical units) at
+ * pixels whose four neighbours are fluid.  0 (the default) turns it off. */
+int lbm_net_set_div_weight(lbm_net* n, float weight, float su, float sv);
 int lbm_net_save(lbm_net* n, const char* path);
 int lbm_net_load(lbm_net* n, const char* path);
 

@@ -168,8 +168,34 @@ def forward(params, inp, levels, base, keep=False, dil=1):
     return y
 
 
-def loss_and_grad(params, inp, target, levels, base, dil=1):
-    """Masked MSE and d loss / d params (canonical layout)."""
+def div_penalty(y, mask, su, sv):
+    """Discrete divergence of the predicted velocity, d = su (y0[x+1] -
+    y0[x-1]) / 2 + sv (y1[y+1] - y1[y-1]) / 2 at points whose four neighbours
+    are fluid (0 elsewhere), and the adjoint map g -> d(sum g d) / dy."""
+    m = mask[..., 0] > 0
+    valid = np.zeros_like(m)
+    valid[:, 1:-1, 1:-1] = (m[:, 1:-1, 1:-1] & m[:, 1:-1, 2:] & m[:, 1:-1, :-2] & m[:, 2:, 1:-1]
+                            & m[:, :-2, 1:-1])
+    d = np.zeros(m.shape)
+    d[:, 1:-1, 1:-1] = (su * (y[:, 1:-1, 2:, 0] - y[:, 1:-1, :-2, 0]) / 2 +
+                        sv * (y[:, 2:, 1:-1, 1] - y[:, :-2, 1:-1, 1]) / 2)
+    d *= valid
+
+    def adjoint(g):
+        g = g * valid
+        dy = np.zeros_like(y)
+        dy[:, :, 1:, 0] += su / 2 * g[:, :, :-1]
+        dy[:, :, :-1, 0] -= su / 2 * g[:, :, 1:]
+        dy[:, 1:, :, 1] += sv / 2 * g[:, :-1, :]
+        dy[:, :-1, :, 1] -= sv / 2 * g[:, 1:, :]
+        return dy
+    return d, adjoint
+
+
+def loss_and_grad(params, inp, target, levels, base, dil=1, div_w=0.0, su=1.0, sv=1.0):
+    """Masked MSE (+ div_w times the squared divergence of the predicted
+    velocity, ``div_penalty``, on the same normalisation) and d loss / d
+    params (canonical layout)."""
     y, c = forward(params, inp, levels, base, keep=True, dil=dil)
     convs, (wh, bh) = unpack(params, levels, base)
     grads = np.zeros_like(params)
@@ -179,6 +205,10 @@ def loss_and_grad(params, inp, target, levels, base, dil=1):
     d = mask * (y - target)
     loss = float((d * d).sum() / (KOUT * nf))
     dy = 2 * d / (KOUT * nf)
+    if div_w:
+        dv, adjoint = div_penalty(y, mask, su, sv)
+        loss += float(div_w * (dv * dv).sum() / (KOUT * nf))
+        dy = dy + adjoint(2 * div_w * dv / (KOUT * nf))
     gwh[...] = dy.reshape(-1, KOUT).T @ c["dd"][0].reshape(-1, base)
     gbh[...] = dy.reshape(-1, KOUT).sum(0)
     g = (dy @ wh) * (c["dd"][0] > 0)

@@ -47,7 +47,7 @@ EXPORTS = [
     "lbm_run_checkpointed", "lbm_iteration",
     "lbm_net_create", "lbm_net_destroy", "lbm_net_num_params", "lbm_net_get_params",
     "lbm_net_set_params", "lbm_net_predict", "lbm_net_gradients", "lbm_net_set_data",
-    "lbm_net_train", "lbm_net_set_weight_decay", "lbm_net_save", "lbm_net_load",
+    "lbm_net_train", "lbm_net_set_weight_decay", "lbm_net_set_div_weight", "lbm_net_save", "lbm_net_load",
 ]
 
 
@@ -168,6 +168,7 @@ def lib() -> C.CDLL:
         "lbm_net_set_data": (I, [P, P, P, I]),
         "lbm_net_train": (I, [P, P, I, C.c_float, C.POINTER(D)]),
         "lbm_net_set_weight_decay": (I, [P, C.c_float]),
+        "lbm_net_set_div_weight": (I, [P, C.c_float, C.c_float, C.c_float]),
         "lbm_net_save": (I, [P, C.c_char_p]),
         "lbm_net_load": (I, [P, C.c_char_p]),
         "lbm_iteration": (I, [C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
@@ -506,6 +507,11 @@ class Net:
         loss = C.c_double()
         _check(lib().lbm_net_train(self.h, _ptr(idx), idx.size, lr, C.byref(loss)))
         return loss.value
+
+    def set_div_weight(self, weight: float, su: float = 1.0, sv: float = 1.0) -> None:
+        """Divergence penalty of the predicted velocity in the loss
+        (lbm_net_set_div_weight); 0 turns it off."""
+        _check(lib().lbm_net_set_div_weight(self.h, weight, su, sv))
 
     def set_weight_decay(self, wd: float) -> None:
         """Decoupled weight decay of the Adam step (AdamW); 0 is plain Adam."""

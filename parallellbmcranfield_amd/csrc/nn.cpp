@@ -50,6 +50,7 @@ struct Conv {
 struct lbm_net {
   int H = 0, W = 0, Bmax = 0, base = 32, L = 4, cin = 3;  // cin: input channels
   float wd = 0.0f;  // decoupled weight decay of the Adam step
+  float div_w = 0.0f, div_su = 1.0f, div_sv = 1.0f;  // divergence penalty of the loss
   int dil = 1;  // bottleneck dilation: its convs use dil and 2 dil
   hipStream_t st = nullptr;
   std::vector<Conv> conv;
@@ -63,6 +64,7 @@ struct lbm_net {
   std::vector<u16*> a, s, xp, da, dd;  // xp[l] = pool(s[l]) at level l+1
   std::vector<u16*> ga, gs, gxp, gda, gdd, dcat;
   float *y = nullptr, *dy = nullptr, *t = nullptr, *mask = nullptr, *lpart = nullptr;
+  float *dv = nullptr, *dvpart = nullptr;  // divergence per pixel, its block sums
   float* part = nullptr;
   size_t part_cap = 0;
   int* idx = nullptr;
@@ -212,6 +214,8 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channel
   if (!rc) rc = nalloc_t(n, &n->t, P0 * KOUT);
   if (!rc) rc = nalloc_t(n, &n->mask, P0);
   if (!rc) rc = nalloc_t(n, &n->lpart, (P0 + 255) / 256);
+  if (!rc) rc = nalloc_t(n, &n->dv, P0);
+  if (!rc) rc = nalloc_t(n, &n->dvpart, (P0 + 255) / 256);
   if (!rc) rc = nalloc_t(n, &n->idx, (size_t)B);
   if (!rc) rc = nalloc_t(n, &n->stage_in, P0 * in_channels);
   if (!rc) rc = nalloc_t(n, &n->stage_t, P0 * 3);
@@ -363,6 +367,9 @@ static int backward(lbm_net* n, int B, double fluid, double* loss) {
   const int P0 = (int)P_of(n, 0, B);
   const float inv_norm = (float)(1.0 / (KOUT * std::max(1.0, fluid)));
   NC(mse(n->y, n->t, n->mask, n->dy, n->lpart, P0, KOUT, inv_norm, n->st));
+  if (n->div_w > 0.0f)
+    NC(div_penalty(n->y, n->mask, n->dv, n->dvpart, n->dy, B, n->H, n->W, n->div_su, n->div_sv,
+                   2.0f * n->div_w * inv_norm, n->st));
   // head
   NC(head_bwd(n->dy, n->p + n->head_w, n->dd[0], n->gdd[0], P0, n->base, KOUT, n->st));
   const int hch = (P0 + RED_CHUNK - 1) / RED_CHUNK;
@@ -412,13 +419,17 @@ static int backward(lbm_net* n, int B, double fluid, double* loss) {
     }
   }
   if (loss) {
-    std::vector<float> lp((size_t)(P0 + 255) / 256);
+    std::vector<float> lp((size_t)(P0 + 255) / 256), dp(lp.size(), 0.0f);
     NC(hipMemcpyAsync(lp.data(), n->lpart, sizeof(float) * lp.size(), hipMemcpyDeviceToHost,
                       n->st));
+    if (n->div_w > 0.0f)
+      NC(hipMemcpyAsync(dp.data(), n->dvpart, sizeof(float) * dp.size(), hipMemcpyDeviceToHost,
+                        n->st));
     NC(hipStreamSynchronize(n->st));
-    double sum = 0.0;
+    double sum = 0.0, dsum = 0.0;
     for (float f : lp) sum += f;
-    *loss = sum / (KOUT * std::max(1.0, fluid));
+    for (float f : dp) dsum += f;
+    *loss = (sum + n->div_w * dsum) / (KOUT * std::max(1.0, fluid));
   }
   return LBM_OK;
 }
@@ -524,6 +535,14 @@ int lbm_net_train(lbm_net* n, const int* idx, int nb, float lr, double* loss) {
           n->st));
   for (const Conv& c : n->conv) NC(flip_w(n->pb + c.w, n->wt + c.w, c.Cout, c.C0 + c.C1, n->st));
   if (!loss) NC(hipStreamSynchronize(n->st));
+  return LBM_OK;
+}
+
+int lbm_net_set_div_weight(lbm_net* n, float weight, float su, float sv) {
+  if (!n || !(weight >= 0.0f) || !(su > 0.0f) || !(sv > 0.0f)) return LBM_EINVAL;
+  n->div_w = weight;
+  n->div_su = su;
+  n->div_sv = sv;
   return LBM_OK;
 }
 

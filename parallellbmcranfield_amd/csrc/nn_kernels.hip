@@ -512,6 +512,48 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const 
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Divergence penalty of the predicted velocity (channels 0, 1 of y, scaled
+// to physical units by su, sv): d[p] = su (y0[x+1] - y0[x-1]) / 2 +
+// sv (y1[y+1] - y1[y-1]) / 2 where p and its four neighbours are fluid, else
+// 0; part[block] = sum of d^2 over the block's pixels.
+__global__ __launch_bounds__(256) void k_div_fwd(const float* __restrict__ y,
+                                                 const float* __restrict__ mask,
+                                                 float* __restrict__ d, float* __restrict__ part,
+                                                 int B, int H, int W, float su, float sv) {
+  __shared__ float red[256];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  float v = 0.0f;
+  if (p < B * H * W) {
+    const int x = p % W, yy = (p / W) % H;
+    if (x > 0 && x < W - 1 && yy > 0 && yy < H - 1 && mask[p] > 0.0f && mask[p - 1] > 0.0f &&
+        mask[p + 1] > 0.0f && mask[p - W] > 0.0f && mask[p + W] > 0.0f)
+      v = 0.5f * su * (y[(size_t)(p + 1) * 3] - y[(size_t)(p - 1) * 3]) +
+          0.5f * sv * (y[(size_t)(p + W) * 3 + 1] - y[(size_t)(p - W) * 3 + 1]);
+    d[p] = v;
+  }
+  red[threadIdx.x] = v * v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// dy += the gradient of g/2 * sum d^2 (the adjoint of k_div_fwd's stencil):
+// dy[q][0] += g su/2 (d[q-1] - d[q+1]), dy[q][1] += g sv/2 (d[q-W] - d[q+W]),
+// neighbours outside the image contributing 0.
+__global__ void k_div_bwd(const float* __restrict__ d, float* __restrict__ dy, int B, int H, int W,
+                          float gsu, float gsv) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= B * H * W) return;
+  const int x = p % W, yy = (p / W) % H;
+  const float dl = x > 0 ? d[p - 1] : 0.0f, dr = x < W - 1 ? d[p + 1] : 0.0f;
+  const float du = yy > 0 ? d[p - W] : 0.0f, dd = yy < H - 1 ? d[p + W] : 0.0f;
+  dy[(size_t)p * 3] += gsu * (dl - dr);
+  dy[(size_t)p * 3 + 1] += gsv * (du - dd);
+}
+
 // dx[p][c] = relu'(x) * sum_k dy[p][k] w[k][c]
 __global__ void k_head_bwd(const float* __restrict__ dy, const float* __restrict__ w,
                            const bf16* __restrict__ x, bf16* __restrict__ dx, int P, int C, int K) {
@@ -740,6 +782,15 @@ hipError_t head_fwd(const u16* x, const float* w, const float* b, float* out, in
 hipError_t mse(const float* y, const float* t, const float* mask, float* dy, float* part, int P,
                int K, float inv_norm, hipStream_t st) {
   k_mse<<<nblk(P), 256, 0, st>>>(y, t, mask, dy, part, P, K, inv_norm);
+  return hipGetLastError();
+}
+
+hipError_t div_penalty(const float* y, const float* mask, float* d, float* part, float* dy, int B,
+                       int H, int W, float su, float sv, float g, hipStream_t st) {
+  const long P = (long)B * H * W;
+  if (P > 0x7fffff00L) return hipErrorInvalidValue;
+  k_div_fwd<<<nblk(P), 256, 0, st>>>(y, mask, d, part, B, H, W, su, sv);
+  k_div_bwd<<<nblk(P), 256, 0, st>>>(d, dy, B, H, W, 0.5f * g * su, 0.5f * g * sv);
   return hipGetLastError();
 }
 

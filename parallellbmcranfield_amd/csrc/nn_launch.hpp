@@ -34,6 +34,10 @@ hipError_t reduce_chunks(const float* part, int nchunk, long n, float* out, hipS
 hipError_t flip_w(const u16* w, u16* wt, int Cout, int Cin, hipStream_t st);
 hipError_t to_bf16(const float* a, u16* b, long n, hipStream_t st);
 hipError_t relu_bwd(u16* g, const u16* y, long n, hipStream_t st);
+// Divergence penalty of the predicted velocity: d (per pixel) and its
+// per-256-pixel sums of d^2 into part; dy += the gradient of (g / 2) sum d^2.
+hipError_t div_penalty(const float* y, const float* mask, float* d, float* part, float* dy, int B,
+                       int H, int W, float su, float sv, float g, hipStream_t st);
 hipError_t pool(const u16* in, u16* out, int B, int H, int W, int C, hipStream_t st);
 hipError_t skip_grad(const u16* dcat, int Ccat, const u16* dpool, const u16* s, u16* ds, int B,
                      int H, int W, int C, hipStream_t st);

@@ -44,6 +44,7 @@ DILATION = 4  # bottleneck stencils dilated by 4 and 8 at 33^2: the field of vie
 # seed error 0.13 -> 0.10 at 0.1)
 RHO_WEIGHT = 0.1
 RHO_ASINH = 0.0  # > 0: density target through asinh(x / s0), s0 this quantile of |x|
+DIV_WEIGHT = 0.0  # weight of the predicted velocity's squared divergence in the loss
 WEIGHT_DECAY = 0.05  # decoupled weight decay of the Adam step (AdamW)
 RE_LO, RE_SPAN = math.log(100.0), math.log(50.0)
 
@@ -210,14 +211,17 @@ def from_net(net: Net, pred: np.ndarray) -> np.ndarray:
 
 def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
           lr: float = 1e-3, seed: int = 0, log=None, rho_weight: float = RHO_WEIGHT,
-          weight_decay: float = WEIGHT_DECAY, rho_asinh: float = RHO_ASINH) -> list[float]:
+          weight_decay: float = WEIGHT_DECAY, rho_asinh: float = RHO_ASINH,
+          div_weight: float = DIV_WEIGHT) -> list[float]:
     """Adam over the set with a cosine learning-rate decay on targets scaled
     per channel (``channel_scales``, kept as ``net.target_scales``); returns
     the mean loss of every epoch.  rho_weight: weight of the density channel
     in the loss relative to u and v (its scale grows by 1/sqrt(rho_weight)),
     weight_decay: decoupled weight decay of the Adam step, rho_asinh: if
     > 0, the density target passes through asinh(x / s0) with s0 the
-    rho_asinh quantile of |x| over the fluid points (``to_net``)."""
+    rho_asinh quantile of |x| over the fluid points (``to_net``), div_weight:
+    weight of the predicted velocity's squared divergence in the loss
+    (``Net.set_div_weight``; compressive seed errors cost the most)."""
     net.rho_s0 = None
     if rho_asinh > 0:
         x = np.abs(targets[..., 2][inputs[..., 0] > 0])
@@ -228,6 +232,7 @@ def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
     sc[2] /= math.sqrt(rho_weight)
     net.target_scales = sc
     net.set_weight_decay(weight_decay)
+    net.set_div_weight(div_weight, float(sc[0]), float(sc[1]))
     net.set_data(inputs, t / sc)
     del t
     rng = np.random.default_rng(seed)
@@ -387,7 +392,8 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         predict_s=warm["predict_s"])
 
 
-__all__ = ["GRID", "IN_CH", "DILATION", "RHO_WEIGHT", "WEIGHT_DECAY", "RHO_ASINH", "encode",
+__all__ = ["GRID", "IN_CH", "DILATION", "RHO_WEIGHT", "WEIGHT_DECAY", "RHO_ASINH", "DIV_WEIGHT",
+           "encode",
            "field_to_grid", "to_net", "from_net",
            "grid_to_field", "steady_states", "channel_scales", "train", "warm_fields",
            "packed_field", "compare_starts"]

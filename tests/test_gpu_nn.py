@@ -124,3 +124,25 @@ def test_weight_decay_is_decoupled():
         nets[0].set_weight_decay(-1.0)
     for n in nets:
         n.close()
+
+
+def test_divergence_penalty_gradients_match_oracle():
+    # the loss with the velocity-divergence term, through lbm_net_gradients
+    H, W, L = 37, 33, 3
+    net = lbm.Net(H, W, batch=2, base=32, levels=L, seed=2)
+    p = net.params()
+    inp, tgt = _sample(2, H, W, 3)
+    loss0, g0 = net.gradients(inp, tgt)
+    net.set_div_weight(50.0, 0.2, 0.15)
+    loss, g = net.gradients(inp, tgt)
+    loss_ref, g_ref = NN.loss_and_grad(p.astype(np.float32), inp, tgt, L, 32, 1, div_w=50.0,
+                                       su=0.2, sv=0.15)
+    print("loss without / with / oracle", loss0, loss, loss_ref, flush=True)
+    assert loss > loss0 and abs(loss - loss_ref) <= 3e-2 * loss_ref
+    cos = float((g * g_ref).sum() / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
+    cos0 = float((g0 * g_ref).sum() / (np.linalg.norm(g0) * np.linalg.norm(g_ref)))
+    print("gradient cosine with / without the term", cos, cos0, flush=True)
+    assert cos > 0.99 and _rel(g, g_ref) < 1e-1
+    with pytest.raises(lbm.LbmError):
+        net.set_div_weight(-1.0)
+    net.close()

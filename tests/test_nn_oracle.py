@@ -108,3 +108,43 @@ def test_backward_matches_finite_differences(dil):
     # the padded input channels carry no gradient
     w0 = g[:32 * 9 * 32].reshape(32, 9, 32)
     assert not w0[:, :, 3:].any()
+
+
+def test_divergence_penalty_matches_finite_differences():
+    # the adjoint of the discrete divergence, then the whole loss gradient
+    rng = np.random.default_rng(8)
+    y = rng.normal(size=(2, 6, 7, 3))
+    mask = (rng.random((2, 6, 7, 1)) > 0.15).astype(float)
+    d, adj = NN.div_penalty(y, mask, 0.7, 1.3)
+    g = rng.normal(size=d.shape)
+    eps = 1e-6
+    num = np.zeros_like(y)
+    for idx in np.ndindex(*y.shape):
+        if idx[-1] == 2:
+            continue
+        yp = y.copy()
+        yp[idx] += eps
+        num[idx] = ((NN.div_penalty(yp, mask, 0.7, 1.3)[0] - d) * g).sum() / eps
+    assert np.allclose(adj(g), num, atol=1e-6)
+    # an interior point with all four neighbours fluid: the central difference
+    m1 = np.ones((1, 5, 5, 1))
+    yy = np.zeros((1, 5, 5, 3))
+    yy[0, :, :, 0] = np.arange(5)[None, :] * 2.0   # du/dx = 2
+    yy[0, :, :, 1] = np.arange(5)[:, None] * -0.5  # dv/dy = -0.5
+    d1, _ = NN.div_penalty(yy, m1, 1.0, 2.0)
+    assert np.allclose(d1[0, 1:-1, 1:-1], 2.0 - 1.0) and not d1[0, 0].any()
+    L, c = 3, 32
+    p = NN.init_params(L, c, seed=4)
+    inp, tgt = _sample(2, 11, 10, 5)
+    loss, gr = NN.loss_and_grad(p, inp, tgt, L, c, 1, div_w=5.0, su=0.2, sv=0.15)
+    assert loss > NN.loss_and_grad(p, inp, tgt, L, c, 1)[0]
+    rng = np.random.default_rng(9)
+    n_head = NN.KOUT * c + NN.KOUT
+    for k in list(p.size - n_head + rng.choice(n_head, 4, replace=False)) + list(rng.choice(2000, 3)):
+        q = p.copy()
+        q[k] += 1e-6
+        lp = NN.loss_and_grad(q, inp, tgt, L, c, 1, div_w=5.0, su=0.2, sv=0.15)[0]
+        q[k] -= 2e-6
+        lm = NN.loss_and_grad(q, inp, tgt, L, c, 1, div_w=5.0, su=0.2, sv=0.15)[0]
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - gr[k]) <= 1e-6 + 1e-4 * abs(fd), (k, fd, gr[k])
