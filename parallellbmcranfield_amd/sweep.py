@@ -106,8 +106,10 @@ def run_batched(specs, opts=None, chunk: int = 2000, progress=None) -> dict:
                 mlups=work / wall / 1e6, solver=s, case=packed)
 
 
-def run_single(spec, opts=None, chunk: int = 5000) -> dict:
-    """One cavity alone with the single-lattice solver, to the same criterion."""
+def run_single(spec, opts=None, chunk: int = 5000, fields: bool = False) -> dict:
+    """One cavity alone with the single-lattice solver, to the same criterion
+    (any storage precision); fields=True adds its final rho, u, v (n*m
+    arrays of lbm_case_cavity_ex's lattice) and whether it converged."""
     opts = opts or sweep_options()
     o = make_options(collision=MRT, uavg=spec["lid"], rho_ini=opts.rho_ini,
                      viscosity=spec["nu"], precision=opts.precision, init=INIT_REST,
@@ -125,8 +127,13 @@ def run_single(spec, opts=None, chunk: int = 5000) -> dict:
     s.sync()
     wall = time.perf_counter() - t0
     fl = fluid_nodes(case)
-    return dict(wall_s=wall, iters=int(s.iter), fluid=fl, mlups=fl * s.iter / wall / 1e6,
-                resid=float(s.residuals()[0]))
+    out = dict(wall_s=wall, iters=int(s.iter), fluid=fl, mlups=fl * s.iter / wall / 1e6,
+               resid=float(s.residuals()[0]))
+    if fields:
+        out["rho"], out["u"], out["v"] = s.macro()
+        out["converged"] = bool(s.iter < o.max_iterations)
+    s.close()
+    return out
 
 
 def autotune(case: Case, opts, instances=None, steps: int = 200,

@@ -385,7 +385,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         median_time_rest=float(np.median(rest["conv_s"][both])) if both.any() else None,
         median_time_warm=float(np.median(warm["conv_s"][both])) if both.any() else None,
         wall_rest=rest["wall_s"], wall_warm=warm["wall_s"],
-        rest_run=rest,
+        rest_run=rest, warm_run=warm, packed=packed,
         field_rel_l2=diff, field_rel_l2_median=float(np.median(diff[both])) if both.any() else None,
         seed_rel_l2=pred, seed_rel_l2_median=float(np.median(pred[both])) if both.any() else None,
         heldout_mse=heldout,

@@ -40,3 +40,46 @@ def test_trained_warm_start_beats_rest_start_on_held_out_cavities():
     assert r["median_steps_ratio"] > 1.08
     assert (r["steps_warm"] < r["steps_rest"]).mean() >= 0.6
     assert r["field_rel_l2_median"] < 2e-3
+
+
+def _rel_fields(packed, k, a, ref):
+    """Velocity relative L2 of batched run a (packed arrays) against ref (an
+    n*m field of instance k's own lattice) over the instance's fluid nodes."""
+    x0, y0, w, h = packed.rects[k]
+    sl = (slice(y0, y0 + h), slice(x0, x0 + w))
+    f = packed.cells()["fluid"].reshape(packed.m, packed.n)[sl] == 1
+    ua, va = (x.reshape(packed.m, packed.n)[sl][f] for x in (a["u"], a["v"]))
+    ub, vb = (np.asarray(x).reshape(h, w)[f] for x in (ref["u"], ref["v"]))
+    return float(np.sqrt(((ua - ub) ** 2 + (va - vb) ** 2).sum() / (ub ** 2 + vb ** 2).sum()))
+
+
+def test_warm_and_rest_starts_converge_to_the_fp64_field():
+    # Residual 1e-6 leaves a run ~1e-3 from the steady state (slow modes);
+    # run tighter and compare the FP32 warm- and rest-started batched fields
+    # with each cavity run alone in FP64 from rest to 1e-10: the same field
+    # within 1e-4 relative L2.
+    specs = SW.random_cavities(44, seed=21, side=(24, 40), re=(100.0, 600.0))
+    train_specs, test_specs = specs[:32], specs[32:]
+    opts = SW.sweep_options(crit=1e-6, resid_every=50, max_steps=150000)
+    fields, it, _ = WS.steady_states(train_specs, opts)
+    G = 41
+    X = np.stack([WS.encode(s, G) for s, k in zip(train_specs, it) if k > 0])
+    T = np.stack([WS.field_to_grid(s, *f, G) for s, f, k in zip(train_specs, fields, it) if k > 0])
+    net = lbm.Net(G, G, batch=8, base=32, levels=4, seed=0, in_channels=WS.IN_CH)
+    WS.train(net, X, T, epochs=60, lr=3e-3)
+    tight = SW.sweep_options(crit=1e-8, resid_every=100, max_steps=300000)
+    r = WS.compare_starts(net, test_specs, tight)
+    f64 = SW.sweep_options(precision=lbm.PREC_F64, crit=1e-10, resid_every=100, max_steps=300000)
+    refs = [SW.run_single(sp, f64, fields=True) for sp in test_specs]
+    packed = r["packed"]
+    ok = ((r["steps_rest"] > 0) & (r["steps_warm"] > 0) &
+          np.array([x["converged"] for x in refs]))
+    print("converged", ok.sum(), "of", len(test_specs), "steps rest", r["steps_rest"], "warm",
+          r["steps_warm"], "fp64", [x["iters"] for x in refs], flush=True)
+    assert ok.sum() >= 8
+    idx = np.nonzero(ok)[0]
+    e_w = [_rel_fields(packed, k, r["warm_run"], refs[k]) for k in idx]
+    e_r = [_rel_fields(packed, k, r["rest_run"], refs[k]) for k in idx]
+    print("vs FP64: warm max", max(e_w), "rest max", max(e_r), flush=True)
+    assert max(e_w) < 1e-4 and max(e_r) < 1e-4
+    net.close()
