@@ -66,22 +66,27 @@ __device__ inline float load1(const Src& s, int b, int y, int x, int ci) {
 // Implicit GEMM, M = pixels, N = output channels, K = 9 taps x Cin in steps
 // of 32 channels of one tap.  Both operands are already in MFMA layout in
 // memory (a lane's 8 consecutive K values are 8 consecutive channels: one
-// 16-byte load), so no LDS is needed.  Each wave computes TM x TN tiles of
-// 16 x 16 (16 TM pixels x 16 TN channels); the 4 waves of a workgroup take
-// consecutive pixel blocks of the same channel block, so they share the
-// weight rows in the caches.  The next K step's operands are loaded while
-// the current step's MFMAs run.  Cout % (16 TN) == 0 (the host checks).
+// 16-byte load).  Each wave computes TM x TN tiles of 16 x 16 (16 TM pixels
+// x 16 TN channels); the 4 waves of a workgroup take consecutive pixel
+// blocks of the same channel block.  The weight rows of a K step, the same
+// for all 4 waves, are loaded once per workgroup (one 16-byte load per
+// thread) into a double-buffered LDS tile; each wave loads its own pixels'
+// channels from memory.  The next K step's loads are issued before the
+// current step's MFMAs; one workgroup barrier per step.  Every wave stays in
+// the loop (waves past the last pixel load zeros) so every barrier is
+// reached.  Cout % (16 TN) == 0 (the host checks).
+constexpr int CV_LD = 40;  // LDS row of a weight tile (bf16): 80-byte rows, 16-byte aligned reads
 template <int TM, int TN, bool RELU>
 __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__ w,
                                                   const float* __restrict__ bias,
                                                   bf16* __restrict__ out, int Cout) {
+  __shared__ __attribute__((aligned(16))) bf16 sw[2][16 * TN][CV_LD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
   const int P = s.B * s.H * s.W;
   const int Cin = s.C0 + s.C1;
   const int pb = (blockIdx.x * 4 + wave) * (16 * TM);
   const int cb = blockIdx.y * (16 * TN);
-  if (pb >= P) return;  // no LDS, no barrier: whole waves may leave
   int pbb[TM], py[TM], px[TM];
   bool pv[TM];
 #pragma unroll
@@ -94,9 +99,11 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
     py[g] = q % s.H;
     pbb[g] = q / s.H;
   }
-  const bf16* wrow[TN];
-#pragma unroll
-  for (int h = 0; h < TN; h++) wrow[h] = w + (size_t)(cb + h * 16 + r) * 9 * Cin + kb * 8;
+  // cooperative weight loader: thread t < 64 TN loads row t / 4 (output
+  // channel cb + t / 4), channels 8 (t % 4) .. + 7 of the K step
+  const int wl_row = threadIdx.x >> 2, wl_col = (threadIdx.x & 3) * 8;
+  const bool wl = threadIdx.x < 64 * TN;
+  const bf16* wsrc = w + (size_t)(cb + (wl ? wl_row : 0)) * 9 * Cin + wl_col;
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int g = 0; g < TM; g++)
@@ -104,21 +111,27 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
     for (int h = 0; h < TN; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
   // K position of the next load: tap lt, channels lci .. lci + 31
   int lt = 0, lci = 0;
-  auto load = [&](bf16x8 (&A)[TM], bf16x8 (&Bm)[TN]) {
+  bf16x8 wreg = zero8();
+  auto load = [&](bf16x8 (&A)[TM]) {
     const int dy = (lt / 3 - 1) * s.dil, dx = (lt % 3 - 1) * s.dil;
 #pragma unroll
     for (int g = 0; g < TM; g++)
       A[g] = pv[g] ? load8(s, pbb[g], py[g] + dy, px[g] + dx, lci + kb * 8) : zero8();
-#pragma unroll
-    for (int h = 0; h < TN; h++)
-      Bm[h] = *reinterpret_cast<const bf16x8*>(wrow[h] + (size_t)lt * Cin + lci);
+    if (wl) wreg = *reinterpret_cast<const bf16x8*>(wsrc + (size_t)lt * Cin + lci);
     lci += 32;
     if (lci == Cin) {
       lci = 0;
       lt++;
     }
   };
-  auto mma = [&](const bf16x8 (&A)[TM], const bf16x8 (&Bm)[TN]) {
+  auto stage = [&](int buf) {
+    if (wl) *reinterpret_cast<bf16x8*>(&sw[buf][wl_row][wl_col]) = wreg;
+  };
+  auto mma = [&](const bf16x8 (&A)[TM], int buf) {
+    bf16x8 Bm[TN];
+#pragma unroll
+    for (int h = 0; h < TN; h++)
+      Bm[h] = *reinterpret_cast<const bf16x8*>(&sw[buf][h * 16 + r][kb * 8]);
 #pragma unroll
     for (int g = 0; g < TM; g++)
 #pragma unroll
@@ -126,16 +139,23 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
         acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[g], Bm[h], acc[g][h], 0, 0, 0);
   };
   const int nk = 9 * (Cin / 32);
-  bf16x8 a0[TM], b0[TN], a1[TM], b1[TN];
-  load(a0, b0);
-  for (int kk = 0;;) {  // two steps per trip, operand sets swapped in place
-    if (kk + 1 < nk) load(a1, b1);
-    mma(a0, b0);
+  bf16x8 a0[TM], a1[TM];
+  load(a0);
+  stage(0);
+  __syncthreads();
+  for (int kk = 0;;) {  // two steps per trip: pixel sets and weight buffers alternate
+    if (kk + 1 < nk) load(a1);
+    mma(a0, 0);
+    if (kk + 1 < nk) stage(1);
+    __syncthreads();
     if (++kk == nk) break;
-    if (kk + 1 < nk) load(a0, b0);
-    mma(a1, b1);
+    if (kk + 1 < nk) load(a0);
+    mma(a1, 1);
+    if (kk + 1 < nk) stage(0);
+    __syncthreads();
     if (++kk == nk) break;
   }
+  if (pb >= P) return;  // after the last barrier
 #pragma unroll
   for (int h = 0; h < TN; h++) {
     const int co = cb + h * 16 + r;
