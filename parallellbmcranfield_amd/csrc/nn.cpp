@@ -111,7 +111,8 @@ static ConvSrc src1(const lbm_net* n, const u16* x, int C, int l, int B, int dil
 // (fewer chunks = less partial-sum traffic for the fixed-order reduction)
 static int grp_per_chunk(const lbm_net* n, const Conv& c, const ConvSrc& s) {
   const long ngrp = wgrad_groups(s);
-  const long per = (long)((c.Cout + 31) / 32) * 9 * ((c.C0 + c.C1) / 32);
+  const int T = wgrad_tile(c.Cout, c.C0 + c.C1);
+  const long per = (long)(c.Cout / T) * 9 * ((c.C0 + c.C1) / T);
   long chunks = std::max(1L, 2048 / per);
   (void)n;
   return (int)std::max(1L, (ngrp + chunks - 1) / chunks);

@@ -296,6 +296,109 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   }
 }
 
+// The weight gradient for Cout and Cin multiples of 64: a workgroup takes a
+// 64 x 64 (co, ci) tile of one tap and one group of 64 consecutive pixels
+// per iteration, loaded by all 256 threads (threads 0-127 dout, 128-255 the
+// shifted input; 4 pixels x 8 channels each) and stored transposed in a
+// double-buffered LDS tile that the 4 waves share: wave w multiplies the co
+// half (w & 1) by the ci half (w >> 1), so each wave owns a 32 x 32 block of
+// the result.  The next group's loads are issued before the current group's
+// MFMAs; one workgroup barrier per group, reached by every thread (the
+// iteration count is the workgroup's).
+__global__ __launch_bounds__(256) void k_wgrad64(Src s, const bf16* __restrict__ dout, int Cout,
+                                                 float* __restrict__ part, int grp_per_chunk) {
+  // [buffer][dout^T | input^T][channel][pixel] (36 864 B)
+  __shared__ __attribute__((aligned(16))) bf16 lds[2][2][64][WG_LD];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kb = lane >> 4;
+  const int Cin = s.C0 + s.C1;
+  const int cob = blockIdx.y * 64;
+  const int nci = Cin / 64;
+  const int t = blockIdx.z / nci, cib = (blockIdx.z % nci) * 64;
+  const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
+  const int P = s.B * s.H * s.W;
+  const int ngrp = (P + WG_PX - 1) / WG_PX;
+  const int g0 = blockIdx.x * grp_per_chunk;
+  const int nit = min(ngrp, g0 + grp_per_chunk) - g0;  // >= 1 for every launched workgroup
+  // loader: operand `which`, pixels 4 lq .. 4 lq + 3 of the group, channels lc .. lc + 7
+  const int which = threadIdx.x >> 7, li = threadIdx.x & 127;
+  const int lq = li >> 3, lc = (li & 7) * 8;
+  bf16x8 v[4];
+  auto fetch = [&](int it) {
+    const bool live = it < nit;
+    const int p0 = (g0 + it) * WG_PX + lq * 4;
+    int x = 0, y = 0, b = 0;
+    if (live) {
+      const int q = p0 / s.W;
+      x = p0 - q * s.W;
+      b = q / s.H;
+      y = q - b * s.H;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      v[j] = zero8();
+      if (live && p0 + j < P) {
+        if (which == 0)
+          v[j] = *reinterpret_cast<const bf16x8*>(dout + (size_t)(p0 + j) * Cout + cob + lc);
+        else
+          v[j] = load8(s, b, y + dy, x + dx, cib + lc);
+      }
+      if (++x == s.W) {  // next pixel in (b, y, x) order
+        x = 0;
+        if (++y == s.H) {
+          y = 0;
+          b++;
+        }
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      *reinterpret_cast<bf16x4*>(&lds[buf][which][lc + e][lq * 4]) =
+          bf16x4{v[0][e], v[1][e], v[2][e], v[3][e]};
+  };
+  const int arow = (wave & 1) * 32, brow = (wave >> 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int g = 0; g < 2; g++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  fetch(0);
+  store(0);
+  __syncthreads();
+  for (int it = 0; it < nit; it++) {
+    const int buf = it & 1;
+    fetch(it + 1);
+#pragma unroll
+    for (int ks = 0; ks < WG_PX / 32; ks++) {
+      bf16x8 a[2], bm[2];
+#pragma unroll
+      for (int g = 0; g < 2; g++)
+        a[g] = *reinterpret_cast<const bf16x8*>(&lds[buf][0][arow + g * 16 + r][ks * 32 + kb * 8]);
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        bm[h] = *reinterpret_cast<const bf16x8*>(&lds[buf][1][brow + h * 16 + r][ks * 32 + kb * 8]);
+#pragma unroll
+      for (int g = 0; g < 2; g++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], bm[h], acc[g][h], 0, 0, 0);
+    }
+    if (it + 1 < nit) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int g = 0; g < 2; g++)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int co = cob + arow + g * 16 + kb * 4 + i, ci = cib + brow + h * 16 + r;
+        part[(((size_t)blockIdx.x * Cout + co) * 9 + t) * Cin + ci] = acc[g][h][i];
+      }
+}
+
 // part[chunk][c] = sum over the chunk's pixels of g[p][c] (bias gradient).
 // Threads are (lane, group) pairs: a group is 8 consecutive channels read as
 // one 16-byte load, lane = t / (C/8) over 256 / (C/8) pixel lanes, so a wave
@@ -718,6 +821,8 @@ hipError_t conv3x3(const ConvSrc& cs, const u16* w, const float* bias, u16* out,
   return hipGetLastError();
 }
 
+int wgrad_tile(int Cout, int Cin) { return Cout % 64 == 0 && Cin % 64 == 0 ? 64 : 32; }
+
 int wgrad_groups(const ConvSrc& cs) {
   return (int)(((long)cs.B * cs.H * cs.W + WG_PX - 1) / WG_PX);
 }
@@ -732,8 +837,10 @@ hipError_t wgrad(const ConvSrc& cs, const u16* dout, int Cout, float* part, int 
   if (Cout % 32 || Cin % 32 || grp_per_chunk < 1 ||
       (long)cs.B * cs.H * cs.W + WG_PX > 0x7fffffffL)
     return hipErrorInvalidValue;
-  dim3 g((unsigned)wgrad_chunks(cs, grp_per_chunk), (unsigned)(Cout / 32), (unsigned)(9 * (Cin / 32)));
-  k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk);
+  const int T = wgrad_tile(Cout, Cin);
+  dim3 g((unsigned)wgrad_chunks(cs, grp_per_chunk), (unsigned)(Cout / T), (unsigned)(9 * (Cin / T)));
+  if (T == 64) k_wgrad64<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk);
+  else k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk);
   return hipGetLastError();
 }
 

@@ -25,6 +25,8 @@ hipError_t conv3x3(const ConvSrc& s, const u16* w, const float* bias, u16* out, 
                    bool relu, hipStream_t st);
 // Weight-gradient partial sums, one [Cout][9][Cin] block per chunk of
 // grp_per_chunk groups of 64 consecutive pixels (Cout, Cin multiples of 32).
+// (co, ci) tile edge of a workgroup: 64 when both counts are multiples of 64
+int wgrad_tile(int Cout, int Cin);
 int wgrad_groups(const ConvSrc& s);
 int wgrad_chunks(const ConvSrc& s, int grp_per_chunk);
 hipError_t wgrad(const ConvSrc& s, const u16* dout, int Cout, float* part, int grp_per_chunk,
