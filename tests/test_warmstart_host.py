@@ -121,3 +121,38 @@ def test_density_transform_round_trip():
     assert np.allclose(WS.from_net(_Net, n), t, rtol=1e-4, atol=1e-5)
     _Net.rho_s0 = None
     assert np.allclose(WS.to_net(_Net, t), t / _Net.target_scales)
+
+
+def test_warm_fields_host_pipeline_with_a_stand_in_network():
+    # encode -> predict -> from_net -> resample -> mass shift, threaded over
+    # cavities; a stand-in network returns a known prediction in its units
+    class _Net:
+        H = 33
+        target_scales = np.array([0.5, 0.25, 2.0], np.float32)
+        rho_s0 = None
+
+        def predict(self, X):
+            assert X.shape[1:] == (33, 33, WS.IN_CH) and X.dtype == np.float32
+            out = np.zeros(X.shape[:3] + (3,), np.float32)
+            out[..., 0] = 0.4 / 0.5   # u / lid = 0.4 everywhere
+            out[..., 1] = -0.2 / 0.25  # v / lid = -0.2
+            out[..., 2] = X[..., 3] / 2.0  # (rho - 1) / 3 lid^2 = x / Lmax
+            return out
+    specs = [_spec(17, 9, [(4, 3, 6, 5)], lid=0.05), _spec(12, 20, [], lid=0.1)]
+    out = WS.warm_fields(_Net(), specs)
+    for sp, (rho, u, v) in zip(specs, out):
+        n, m = sp["NX"] + 2, sp["NY"] + 3
+        assert rho.shape == u.shape == v.shape == (n * m,)
+        fl = np.zeros((m, n), bool)
+        fl[1:sp["NY"] + 1, 1:sp["NX"] + 1] = WS._fluid_mask(sp)
+        fl = fl.ravel()
+        assert np.allclose(u[fl], 0.4 * sp["lid"]) and np.allclose(v[fl], -0.2 * sp["lid"])
+        assert (u[~fl] == 0).all() and (rho[~fl] == 1).all()
+        # mass of the rest start kept, the predicted gradient along x kept
+        assert np.isclose(rho[fl].mean(), 1.0)
+        r2 = rho.reshape(m, n)
+        j = 1 + sp["NY"] // 2
+        row = r2[j, 1:sp["NX"] + 1][WS._fluid_mask(sp)[j - 1]]
+        assert (np.diff(row) > 0).all()
+    uni = WS.warm_fields(_Net(), specs, density=False)
+    assert all((r == 1).all() for r, _, _ in uni)
