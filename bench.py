@@ -108,6 +108,9 @@ def main():
                     help="warmstart: density weight in the loss (default: the package's)")
     ap.add_argument("--weight-decay", type=float, default=None,
                     help="warmstart: decoupled weight decay (default: the package's)")
+    ap.add_argument("--rho-clip", type=float, default=None,
+                    help="warmstart: clip the density target to these quantiles (0: off; "
+                         "default: the package's)")
     ap.add_argument("--sizes", default="1025,4097", help="matrix: cavity sizes")
     ap.add_argument("--save-net", default="", help="warmstart: write the trained weights here")
     ap.add_argument("--data-max-steps", type=int, default=200000,
@@ -764,8 +767,9 @@ def bench_warmstart(args):
     t1 = time.perf_counter()
     rho_w = WS.RHO_WEIGHT if args.rho_weight is None else args.rho_weight
     wd = WS.WEIGHT_DECAY if args.weight_decay is None else args.weight_decay
+    rho_clip = WS.RHO_CLIP if args.rho_clip is None else args.rho_clip
     hist = WS.train(net, X, T, epochs=args.epochs, lr=args.lr, log=log, rho_weight=rho_w,
-                    weight_decay=wd)
+                    weight_decay=wd, rho_clip=rho_clip)
     t_train = time.perf_counter() - t1
     if args.save_net:
         net.save(args.save_net)
@@ -800,6 +804,7 @@ def bench_warmstart(args):
             "input_channels": WS.IN_CH,
             "rho_loss_weight": rho_w,
             "weight_decay": wd,
+            "rho_clip": rho_clip,
             "training_samples": int(len(hist) and args.train_count),
             "epochs": args.epochs,
             "batch": args.batch,

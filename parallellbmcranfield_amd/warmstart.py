@@ -45,6 +45,7 @@ DILATION = 4  # bottleneck stencils dilated by 4 and 8 at 33^2: the field of vie
 RHO_WEIGHT = 0.1
 RHO_ASINH = 0.0  # > 0: density target through asinh(x / s0), s0 this quantile of |x|
 DIV_WEIGHT = 0.0  # weight of the predicted velocity's squared divergence in the loss
+RHO_CLIP = 0.0  # > 0: density target clipped to these quantiles (e.g. 0.99)
 WEIGHT_DECAY = 0.05  # decoupled weight decay of the Adam step (AdamW)
 RE_LO, RE_SPAN = math.log(100.0), math.log(50.0)
 
@@ -188,6 +189,16 @@ def channel_scales(inputs: np.ndarray, targets: np.ndarray) -> np.ndarray:
     return np.sqrt(np.maximum(ms, 1e-12)).astype(np.float32)
 
 
+def clip_density(inputs: np.ndarray, targets: np.ndarray, q: float) -> np.ndarray:
+    """Targets with the density channel clipped to its [1 - q, q] quantiles
+    over the fluid points of the set (the other channels unchanged)."""
+    x = targets[..., 2][inputs[..., 0] > 0]
+    lo, hi = np.quantile(x, 1.0 - q), np.quantile(x, q)
+    out = np.array(targets, copy=True)
+    out[..., 2] = np.clip(out[..., 2], lo, hi)
+    return out
+
+
 def to_net(net: Net, targets: np.ndarray) -> np.ndarray:
     """Normalised targets [..., 3] (``field_to_grid``) -> the network's
     units: the density channel through asinh(x / s0) when ``net.rho_s0`` is
@@ -212,7 +223,7 @@ def from_net(net: Net, pred: np.ndarray) -> np.ndarray:
 def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
           lr: float = 1e-3, seed: int = 0, log=None, rho_weight: float = RHO_WEIGHT,
           weight_decay: float = WEIGHT_DECAY, rho_asinh: float = RHO_ASINH,
-          div_weight: float = DIV_WEIGHT) -> list[float]:
+          div_weight: float = DIV_WEIGHT, rho_clip: float = RHO_CLIP) -> list[float]:
     """Adam over the set with a cosine learning-rate decay on targets scaled
     per channel (``channel_scales``, kept as ``net.target_scales``); returns
     the mean loss of every epoch.  rho_weight: weight of the density channel
@@ -221,7 +232,12 @@ def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
     > 0, the density target passes through asinh(x / s0) with s0 the
     rho_asinh quantile of |x| over the fluid points (``to_net``), div_weight:
     weight of the predicted velocity's squared divergence in the loss
-    (``Net.set_div_weight``; compressive seed errors cost the most)."""
+    (``Net.set_div_weight``; compressive seed errors cost the most),
+    rho_clip: if > 0, the density target is clipped to its [1 - rho_clip,
+    rho_clip] quantiles over the set's fluid points (``clip_density``), so
+    the lid corners' pressure peaks leave the loss to the bulk field."""
+    if rho_clip > 0:
+        targets = clip_density(inputs, targets, rho_clip)
     net.rho_s0 = None
     if rho_asinh > 0:
         x = np.abs(targets[..., 2][inputs[..., 0] > 0])
@@ -393,7 +409,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
 
 
 __all__ = ["GRID", "IN_CH", "DILATION", "RHO_WEIGHT", "WEIGHT_DECAY", "RHO_ASINH", "DIV_WEIGHT",
-           "encode",
+           "RHO_CLIP", "clip_density", "encode",
            "field_to_grid", "to_net", "from_net",
            "grid_to_field", "steady_states", "channel_scales", "train", "warm_fields",
            "packed_field", "compare_starts"]

@@ -156,3 +156,19 @@ def test_warm_fields_host_pipeline_with_a_stand_in_network():
         assert (np.diff(row) > 0).all()
     uni = WS.warm_fields(_Net(), specs, density=False)
     assert all((r == 1).all() for r, _, _ in uni)
+
+
+def test_density_clip_bounds_only_the_density_channel():
+    rng = np.random.default_rng(6)
+    inp = np.zeros((3, 9, 9, WS.IN_CH), np.float32)
+    inp[..., 0] = rng.random((3, 9, 9)) > 0.2
+    t = rng.normal(size=(3, 9, 9, 3)).astype(np.float32)
+    t[0, 0, 0, 2] = 50.0  # a corner peak
+    c = WS.clip_density(inp, t, 0.95)
+    x = t[..., 2][inp[..., 0] > 0]
+    lo, hi = np.quantile(x, 0.05), np.quantile(x, 0.95)
+    assert np.array_equal(c[..., :2], t[..., :2])
+    assert c[..., 2].max() <= hi + 1e-6 and c[..., 2].min() >= lo - 1e-6
+    inside = (t[..., 2] >= lo) & (t[..., 2] <= hi)
+    assert np.array_equal(c[..., 2][inside], t[..., 2][inside])
+    assert t[0, 0, 0, 2] == 50.0  # the input set is left as it was
