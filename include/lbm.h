@@ -264,7 +264,11 @@ size_t lbm_solver_device_bytes(const lbm_solver* s);
  * iteration, residuals, velocity-residual snapshot): the restart the
  * reference declares but never implements (ReadFormerData,
  * serial/FileIO.c:287; ReadResults, serial/include/FilesIO.h:25).  A file
- * loads into a solver of the same case, precision and residual kind. */
+ * loads only into a solver with the same lattice, precision, collision model,
+ * rho_ini, FP16 scale and residual kind (and interval, for the velocity
+ * residual); any other file, or one of the wrong length, is refused
+ * (LBM_EINVAL / LBM_EPARSE) before the solver's state changes.  Should a
+ * read fail part-way, the solver refuses to step until a load succeeds. */
 int lbm_solver_save(lbm_solver* s, const char* path);
 int lbm_solver_load(lbm_solver* s, const char* path);
 /* Kernel layout (for profiling): bit 0 set if interior and boundary-list
@@ -273,8 +277,10 @@ int lbm_solver_load(lbm_solver* s, const char* path);
 int lbm_solver_fused(const lbm_solver* s);
 
 /* Bandwidth of a plain device-to-device copy kernel over a buffer of `bytes`
- * (read + write counted), best of `reps` launches: the practical HBM ceiling
- * the step kernels are compared with.  No counterpart in the reference. */
+ * (read + write counted), best of `reps` launches of each of several launch
+ * shapes (1 to 8 independent 16-byte loads in flight per thread): the
+ * practical HBM ceiling the step kernels are compared with.  No counterpart
+ * in the reference. */
 int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs);
 
 /* ---------------------------------------------------------------- driver */

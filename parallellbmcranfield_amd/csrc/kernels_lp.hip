@@ -49,26 +49,66 @@ hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf) {
 }  // namespace lbmdev
 
 // ---------------------------------------------------------------- copy probe
-// A plain streaming copy (16-byte vectors, non-temporal stores): the
-// practical HBM ceiling the step kernels are compared with, and the known-size
-// kernel that calibrates hardware byte counters.
+// A plain streaming copy: the practical HBM ceiling the step kernels are
+// compared with, and the known-size kernel that calibrates hardware byte
+// counters.  Each thread moves U 16-byte vectors per trip, all U loads issued
+// before the first store, so U loads are in flight per lane (one per trip
+// leaves the copy latency-bound, below the step kernel's own rate).  Vector u
+// of a trip sits u * stride elements after the first, so every load and store
+// instruction of a wave covers 1 KiB of consecutive bytes.  Stores are
+// non-temporal; loads are too when NT_LOAD.
 namespace lbmdev {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+template <int U, bool NT_LOAD>
 __global__ __launch_bounds__(256) void k_copy(const v4f* __restrict__ src,
                                               v4f* __restrict__ dst, long n4) {
-  long i = (long)blockIdx.x * 256 + threadIdx.x;
   const long stride = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  for (; i + (long)(U - 1) * stride < n4; i += (long)U * stride) {
+    v4f r[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      r[u] = NT_LOAD ? __builtin_nontemporal_load(src + i + u * stride) : src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; u++) __builtin_nontemporal_store(r[u], dst + i + u * stride);
+  }
   for (; i < n4; i += stride) __builtin_nontemporal_store(src[i], dst + i);
 }
 
-hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+// shape: 0 one vector per trip over a grid of at most 16384 workgroups (the
+// original probe); 1..6 U = 2, 4, 8 vectors per thread, normal or
+// non-temporal loads, one trip per thread over the whole buffer.
+hipError_t launch_copy_shape(const void* src, void* dst, size_t bytes, int shape,
+                             hipStream_t st) {
   const long n4 = (long)(bytes / 16);
-  long blocks = (n4 + 255) / 256;
-  if (blocks > 256 * 64) blocks = 256 * 64;
-  k_copy<<<(unsigned)blocks, 256, 0, st>>>((const v4f*)src, (v4f*)dst, n4);
+  auto grid = [&](int U) {
+    long b = (n4 + 256L * U - 1) / (256L * U);
+    return (unsigned)(b < 1 ? 1 : b);
+  };
+  const v4f* s = (const v4f*)src;
+  v4f* d = (v4f*)dst;
+  switch (shape) {
+    case 0: {
+      long blocks = (n4 + 255) / 256;
+      if (blocks > 256 * 64) blocks = 256 * 64;
+      if (blocks < 1) blocks = 1;
+      k_copy<1, false><<<(unsigned)blocks, 256, 0, st>>>(s, d, n4);
+    } break;
+    case 1: k_copy<2, false><<<grid(2), 256, 0, st>>>(s, d, n4); break;
+    case 2: k_copy<4, false><<<grid(4), 256, 0, st>>>(s, d, n4); break;
+    case 3: k_copy<8, false><<<grid(8), 256, 0, st>>>(s, d, n4); break;
+    case 4: k_copy<2, true><<<grid(2), 256, 0, st>>>(s, d, n4); break;
+    case 5: k_copy<4, true><<<grid(4), 256, 0, st>>>(s, d, n4); break;
+    case 6: k_copy<8, true><<<grid(8), 256, 0, st>>>(s, d, n4); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  return launch_copy_shape(src, dst, bytes, 2, st);
 }
 
 }  // namespace lbmdev

@@ -105,6 +105,7 @@ struct lbm_solver {
   std::vector<double> force_val;  // fx, fy per row
   bool diverged = false;
   int diverged_iter = -1;
+  bool invalid = false;       // a checkpoint load failed after overwriting part of the state
   std::vector<double> rho0, u0, v0;  // initial macros (reference order), for t = 0
   std::vector<double> grad0;         // field start with non-equilibrium: [3][n*m]
   std::vector<float> inst_visc;      // batched: viscosity of each instance

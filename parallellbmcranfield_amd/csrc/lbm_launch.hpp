@@ -48,7 +48,10 @@ hipError_t launch_inst_resid(const double* snap, int plane, int pitch, const int
                              const uint8_t* active, int ninst, double* out, hipStream_t st);
 hipError_t launch_ref_residual(const double* cres, int m, int n, double* out, hipStream_t st);
 
-// Plain device copy of `bytes` (multiple of 16).
+// Plain device copy of `bytes` (multiple of 16); launch_copy_shape picks one
+// of COPY_SHAPES launch shapes (vectors in flight per thread, load kind).
+constexpr int COPY_SHAPES = 7;
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+hipError_t launch_copy_shape(const void* src, void* dst, size_t bytes, int shape, hipStream_t st);
 
 }  // namespace lbmdev

@@ -883,6 +883,10 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
                      int sample_every = 1) {
   if (s->ninst && s->t_first_step < 0) s->t_first_step = wall_seconds();
   if (done) *done = 0;
+  if (s->invalid) {
+    set_error("solver state invalid: a checkpoint load failed part-way (load one again)");
+    return LBM_EINVAL;
+  }
   if (s->diverged) {
     set_error("solver diverged");
     return LBM_DIVERGED;
@@ -1151,11 +1155,48 @@ int lbm_solver_fused(const lbm_solver* s) {
 
 // ---------------------------------------------------------------- checkpoint
 namespace {
+// Everything that fixes how the stored words map to distributions (lattice,
+// layout, storage kind, rho0 of the deviation form, FP16 scale) and what the
+// saved counters mean (collision, residual kind and interval).  A file loads
+// only into a solver that agrees on all of them.
 struct CkptHeader {
-  char magic[8];  // "LBMCKPT1"
+  char magic[8];  // "LBMCKPT2"
   int abi, n, m, prec, coll, pitch, plane, iter, cur, resid_kind, has_snap;
+  int resid_every;
   double resid[4];
+  double rho0;      // rho_ini: FP32/FP16 store f - w_k rho0
+  float f16_scale;  // FP16 stores (f - w_k rho0) * scale
+  int pad;
 };
+
+CkptHeader ckpt_header(const lbm_solver* s) {
+  CkptHeader h;
+  std::memset(&h, 0, sizeof h);
+  std::memcpy(h.magic, "LBMCKPT2", 8);
+  h.abi = LBM_ABI_VERSION;
+  h.n = s->cs->n;
+  h.m = s->cs->m;
+  h.prec = s->prec;
+  h.coll = s->coll;
+  h.pitch = s->pitch;
+  h.plane = s->plane;
+  h.iter = s->iter;
+  h.cur = s->cur;
+  h.resid_kind = s->opt.resid_kind;
+  h.has_snap = s->d_snap ? 1 : 0;
+  h.resid_every = s->opt.resid_every;
+  std::memcpy(h.resid, s->resid, sizeof h.resid);
+  h.rho0 = s->kp.rho0;
+  h.f16_scale = s->kp.scale_f;
+  return h;
+}
+
+// Bytes that follow the header: both distribution buffers, then the
+// velocity-residual snapshot (rho, u, v planes) when the solver keeps one.
+size_t ckpt_payload(const lbm_solver* s) {
+  const size_t sb = (size_t)9 * s->plane * s->elem;
+  return 2 * sb + (s->d_snap ? sizeof(double) * 3 * (size_t)s->plane : 0);
+}
 
 int copy_out(lbm_solver* s, FILE* f, const void* dev, size_t bytes) {
   const size_t CH = (size_t)256 << 20;
@@ -1201,21 +1242,7 @@ int lbm_solver_save(lbm_solver* s, const char* path) {
     set_error(std::string("cannot write ") + path);
     return LBM_EIO;
   }
-  CkptHeader h;
-  std::memset(&h, 0, sizeof h);
-  std::memcpy(h.magic, "LBMCKPT1", 8);
-  h.abi = LBM_ABI_VERSION;
-  h.n = s->cs->n;
-  h.m = s->cs->m;
-  h.prec = s->prec;
-  h.coll = s->coll;
-  h.pitch = s->pitch;
-  h.plane = s->plane;
-  h.iter = s->iter;
-  h.cur = s->cur;
-  h.resid_kind = s->opt.resid_kind;
-  h.has_snap = s->d_snap ? 1 : 0;
-  std::memcpy(h.resid, s->resid, sizeof h.resid);
+  const CkptHeader h = ckpt_header(s);
   rc = std::fwrite(&h, sizeof h, 1, f) == 1 ? LBM_OK : LBM_EIO;
   const size_t sb = (size_t)9 * s->plane * s->elem;
   if (!rc) rc = copy_out(s, f, s->buf[0], sb);
@@ -1238,19 +1265,62 @@ int lbm_solver_load(lbm_solver* s, const char* path) {
   }
   CkptHeader h;
   int rc = std::fread(&h, sizeof h, 1, f) == 1 ? LBM_OK : LBM_EPARSE;
-  if (!rc && (std::memcmp(h.magic, "LBMCKPT1", 8) != 0 || h.n != s->cs->n || h.m != s->cs->m ||
-              h.prec != s->prec || h.pitch != s->pitch || h.plane != s->plane ||
-              h.has_snap != (s->d_snap ? 1 : 0))) {
-    set_error("checkpoint does not match this case / precision / residual kind");
-    rc = LBM_EINVAL;
+  if (rc) set_error(std::string(path) + ": not a checkpoint (short header)");
+  const CkptHeader want = ckpt_header(s);
+  if (!rc && std::memcmp(h.magic, want.magic, 8) != 0) {
+    set_error(std::string(path) + ": not a checkpoint of this format (magic LBMCKPT2)");
+    rc = LBM_EPARSE;
+  }
+  if (!rc) {
+    std::string why;
+    if (h.abi != want.abi) why += " abi";
+    if (h.n != want.n || h.m != want.m || h.pitch != want.pitch || h.plane != want.plane)
+      why += " lattice";
+    if (h.prec != want.prec) why += " precision";
+    if (h.coll != want.coll) why += " collision";
+    if (h.resid_kind != want.resid_kind || h.has_snap != want.has_snap) why += " residual-kind";
+    if (h.resid_kind == LBM_RESID_VELOCITY && h.resid_every != want.resid_every)
+      why += " residual-interval";
+    if (std::memcmp(&h.rho0, &want.rho0, sizeof h.rho0) != 0) why += " rho_ini";
+    if (s->prec == LBM_PREC_F16 && std::memcmp(&h.f16_scale, &want.f16_scale, sizeof(float)) != 0)
+      why += " f16_scale";
+    if (!why.empty()) {
+      set_error("checkpoint does not match this solver:" + why);
+      rc = LBM_EINVAL;
+    }
+  }
+  // the whole payload must be there before any device memory is touched: a
+  // short or over-long file leaves the solver exactly as it was
+  if (!rc) {
+    const long long need = (long long)sizeof h + (long long)ckpt_payload(s);
+    long long have = -1;
+    if (fseeko(f, 0, SEEK_END) == 0) have = (long long)ftello(f);
+    if (have != need) {
+      set_error(std::string(path) + ": checkpoint holds " + std::to_string(have) +
+                " bytes, this solver's state needs " + std::to_string(need));
+      rc = LBM_EPARSE;
+    } else if (fseeko(f, (off_t)sizeof h, SEEK_SET) != 0) {
+      set_error(std::string("cannot seek in ") + path);
+      rc = LBM_EIO;
+    }
+  }
+  if (rc) {
+    std::fclose(f);
+    return rc;
   }
   HC(hipStreamSynchronize(s->stream));
   const size_t sb = (size_t)9 * s->plane * s->elem;
-  if (!rc) rc = copy_in(s, f, s->buf[0], sb);
+  rc = copy_in(s, f, s->buf[0], sb);
   if (!rc) rc = copy_in(s, f, s->buf[1], sb);
   if (!rc && s->d_snap) rc = copy_in(s, f, s->d_snap, sizeof(double) * 3 * (size_t)s->plane);
   std::fclose(f);
-  if (rc) return rc;
+  if (rc) {
+    // a read failed after part of the state was overwritten: refuse to step
+    s->invalid = true;
+    set_error(std::string(lbm_last_error()) + " (solver state invalid until a checkpoint loads)");
+    return rc;
+  }
+  s->invalid = false;
   s->iter = h.iter;
   s->cur = h.cur;
   std::memcpy(s->resid, h.resid, sizeof h.resid);
@@ -1258,6 +1328,10 @@ int lbm_solver_load(lbm_solver* s, const char* path) {
   s->pending = 0;
   s->pending_iter.clear();
   s->pending_what.clear();
+  s->hist_iter.clear();
+  s->hist_val.clear();
+  s->force_iter.clear();
+  s->force_val.clear();
   s->diverged = false;
   return LBM_OK;
 }
@@ -1274,17 +1348,19 @@ int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs) {
   if (!rc) rc = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
   if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
   if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+  // best launch over every copy shape (vectors in flight per thread, load kind)
   double best = 0.0;
-  for (int r = 0; !rc && r < reps + 1; r++) {
-    rc = check_hip(hipEventRecord(e0, st), "record");
-    if (!rc) rc = check_hip(launch_copy(a, b, bytes, st), "copy");
-    if (!rc) rc = check_hip(hipEventRecord(e1, st), "record");
-    if (!rc) rc = check_hip(hipEventSynchronize(e1), "sync");
-    float ms = 0.f;
-    if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    const double g = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
-    if (!rc && r > 0 && g > best) best = g;  // first launch is a warm-up
-  }
+  for (int shape = 0; !rc && shape < COPY_SHAPES; shape++)
+    for (int r = 0; !rc && r < reps + 1; r++) {
+      rc = check_hip(hipEventRecord(e0, st), "record");
+      if (!rc) rc = check_hip(launch_copy_shape(a, b, bytes, shape, st), "copy");
+      if (!rc) rc = check_hip(hipEventRecord(e1, st), "record");
+      if (!rc) rc = check_hip(hipEventSynchronize(e1), "sync");
+      float ms = 0.f;
+      if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      const double g = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+      if (!rc && r > 0 && g > best) best = g;  // first launch of a shape is a warm-up
+    }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (st) (void)hipStreamDestroy(st);

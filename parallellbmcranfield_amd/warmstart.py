@@ -191,7 +191,11 @@ def channel_scales(inputs: np.ndarray, targets: np.ndarray) -> np.ndarray:
 
 def clip_density(inputs: np.ndarray, targets: np.ndarray, q: float) -> np.ndarray:
     """Targets with the density channel clipped to its [1 - q, q] quantiles
-    over the fluid points of the set (the other channels unchanged)."""
+    over the fluid points of the set (the other channels unchanged).
+    0.5 < q < 1: at q <= 0.5 the interval is empty (the channel would become
+    one constant) and at q >= 1 nothing is clipped."""
+    if not (0.5 < q < 1.0):
+        raise ValueError(f"density clip quantile must lie in (0.5, 1), got {q!r}")
     x = targets[..., 2][inputs[..., 0] > 0]
     lo, hi = np.quantile(x, 1.0 - q), np.quantile(x, q)
     out = np.array(targets, copy=True)
@@ -233,10 +237,10 @@ def train(net: Net, inputs: np.ndarray, targets: np.ndarray, epochs: int = 30,
     rho_asinh quantile of |x| over the fluid points (``to_net``), div_weight:
     weight of the predicted velocity's squared divergence in the loss
     (``Net.set_div_weight``; compressive seed errors cost the most),
-    rho_clip: if > 0, the density target is clipped to its [1 - rho_clip,
+    rho_clip: if not 0, the density target is clipped to its [1 - rho_clip,
     rho_clip] quantiles over the set's fluid points (``clip_density``), so
     the lid corners' pressure peaks leave the loss to the bulk field."""
-    if rho_clip > 0:
+    if rho_clip != 0:  # 0 is off; anything else must be a valid quantile
         targets = clip_density(inputs, targets, rho_clip)
     net.rho_s0 = None
     if rho_asinh > 0:
@@ -312,8 +316,8 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
                    neq: bool = True, density: bool = True, run_rest=None) -> dict:
     """Held-out comparison: the same cavities from rest and from the network's
     prediction, each as one batched run to the criterion.  Per instance:
-    steps and wall time to convergence (the warm start's time includes the
-    prediction), and the relative L2 difference of the two converged
+    steps and wall time to convergence (both include the solver's
+    construction, the warm start's also the prediction), and the relative L2 difference of the two converged
     velocity fields and of the seed against the rest start's converged
     velocity (the prediction error).  neq: seed feq plus the Chapman-Enskog non-equilibrium
     part of the predicted velocity gradients (LBM_INIT_FIELD_NEQ).  density:
@@ -326,6 +330,9 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
     seeds = {}
 
     def run(field):
+        # the clock starts before solver construction for both starts: a
+        # field start's construction (seed upload, and for INIT_FIELD_NEQ
+        # the host pass over the velocity gradients) is part of its cost
         t0 = time.perf_counter()
         pre = 0.0
         if field == "warm":
@@ -340,6 +347,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
             s = Solver(packed, opts, instances=inst)
         s.sync()
         t1 = time.perf_counter()
+        setup = t1 - t0 - pre  # solver construction
         while True:
             done = s.step(chunk)
             if done < chunk or s.iter >= opts.max_iterations:
@@ -349,15 +357,16 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         it, sec, _ = s.batch_status()
         rho, u, v = s.macro()
         s.close()
-        return dict(conv_iter=it, conv_s=sec + pre, wall_s=wall + pre, predict_s=pre, rho=rho, u=u,
-                    v=v)
+        return dict(conv_iter=it, conv_s=sec + pre + setup, wall_s=wall + pre + setup,
+                    predict_s=pre, setup_s=setup, rho=rho, u=u, v=v)
 
     rest = run_rest if run_rest is not None else run("rest")
     if log:
         log(f"rest start: {rest['wall_s']:.1f} s")
     warm = run("warm")
     if log:
-        log(f"warm start: {warm['wall_s']:.1f} s (prediction {warm['predict_s']:.2f} s)")
+        log(f"warm start: {warm['wall_s']:.1f} s (prediction {warm['predict_s']:.2f} s, "
+            f"solver set-up {warm['setup_s']:.2f} s; rest set-up {rest['setup_s']:.2f} s)")
     both = (rest["conv_iter"] > 0) & (warm["conv_iter"] > 0)
     fluid = packed.cells()["fluid"].reshape(packed.m, packed.n) == 1
 
@@ -405,7 +414,7 @@ def compare_starts(net: Net, specs, opts=None, chunk: int = 5000, log=None,
         field_rel_l2=diff, field_rel_l2_median=float(np.median(diff[both])) if both.any() else None,
         seed_rel_l2=pred, seed_rel_l2_median=float(np.median(pred[both])) if both.any() else None,
         heldout_mse=heldout,
-        predict_s=warm["predict_s"])
+        predict_s=warm["predict_s"], setup_rest_s=rest["setup_s"], setup_warm_s=warm["setup_s"])
 
 
 __all__ = ["GRID", "IN_CH", "DILATION", "RHO_WEIGHT", "WEIGHT_DECAY", "RHO_ASINH", "DIV_WEIGHT",

@@ -22,11 +22,23 @@ def ini_dict(values):
 
 def small_cases():
     names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
-    return [n for n in names if not n.startswith("coarse_")]
+    return [n for n in names if not n.startswith(("coarse_", "medium_"))]
 
 
 def shipped_cases():
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "coarse_*.npz")))
+
+
+def medium_cases():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "medium_*.npz")))
+
+
+SETUP_FIELDS = ("fluid", "corner", "stream", "id", "boundary", "boundary_id", "bc", "q", "x", "y")
+
+
+def sha(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
 def load(name):

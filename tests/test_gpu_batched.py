@@ -245,3 +245,64 @@ def test_neq_start_has_the_chapman_enskog_stress_and_converges_sooner():
     assert (steps[lbm.INIT_FIELD_NEQ] > 0).all()
     assert (steps[lbm.INIT_FIELD_NEQ] <= steps[lbm.INIT_FIELD]).all()
     assert (steps[lbm.INIT_FIELD_NEQ] < it0 / 2).all()
+
+
+# Uneven sizes, aspect ratios, Re and lid speeds; rectangles in the bulk, near
+# walls and near the lid corners.  (NX, NY, rectangles, viscosity, lid speed)
+ORACLE_SET = [
+    (48, 30, [(5, 5, 12, 9), (30, 18, 36, 25)], 0.02, 0.1),
+    (25, 53, [(3, 40, 20, 45)], 0.05, 0.07),
+    (70, 36, [], 0.008, 0.1),
+    (19, 19, [(8, 3, 11, 14)], 0.1, 0.05),
+    (33, 41, [(3, 3, 6, 6), (25, 30, 30, 36), (10, 20, 18, 24)], 0.015, 0.12),
+]
+
+
+def _oracle_params(kind, nu, lid):
+    from parallellbmcranfield_amd import sweep as SW
+    if kind == "mrt_sweep":  # the batched sweep's collision: MRT, Lallemand-Luo rates, 1/tau of the instance
+        return O.params(collision=3, uavg=lid, rho_ini=1.0, viscosity=nu, init_rest=1,
+                        resid_kind=3, mrt_textbook=1, mrt_s=SW.mrt_rates(nu))
+    if kind == "mrt_reference":  # MRTInitializer's rates and the reference's rho^2 equilibria
+        return O.params(collision=3, uavg=lid, rho_ini=1.0, viscosity=nu, init_rest=1,
+                        resid_kind=3)
+    return O.params(collision=1, uavg=lid, rho_ini=1.0, viscosity=nu, init_rest=1, resid_kind=3)
+
+
+@pytest.mark.parametrize("kind", ["mrt_sweep", "mrt_reference", "bgk"])
+def test_batched_instances_match_the_fp64_oracle(kind, tmp_path):
+    """Each instance of a packed lattice (FP32 storage, its own omega and lid
+    speed) against the FP64 oracle run alone on that instance's mesh, written
+    in the reference format by lbm_case_write and read back by the oracle's
+    own reader: velocity relative L2 < 2e-5 and density within 1e-5 after 400
+    steps, per instance.  Anchors: the loop body serial/Iterate.c:263-332,
+    BGKW / MRT serial/CellFunctions.c:352-365, 401-438."""
+    from parallellbmcranfield_amd import sweep as SW
+    print("batched vs oracle", kind, flush=True)
+    steps = 400
+    cases = [lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in ORACLE_SET]
+    packed = lbm.Case.pack(cases)
+    if kind == "mrt_sweep":
+        o = SW.sweep_options(crit=-1.0, resid_every=50)
+        o.use_stop_rule = 0
+    else:
+        o = batch_opts(lbm.PREC_F32, 3 if kind == "mrt_reference" else 1, resid_every=50)
+    s = lbm.Solver(packed, o, instances=[(nu, u) for *_, nu, u in ORACLE_SET])
+    assert s.step(steps) == steps
+    rho, u, v = s.macro()
+    for k, (nx, ny, r, nu, lid) in enumerate(ORACLE_SET):
+        nf, cf = str(tmp_path / f"n{k}.dat"), str(tmp_path / f"c{k}.dat")
+        cases[k].write(nf, cf)
+        ref = O.Oracle.from_files(nf, cf, _oracle_params(kind, nu, lid))
+        assert (ref.n, ref.m) == (cases[k].n, cases[k].m)
+        ref.step(steps)
+        ro, uo, vo = ref.macro()
+        fl = cases[k].cells()["fluid"] == 1
+        ub, vb, rb = crop(u, packed, k)[fl], crop(v, packed, k)[fl], crop(rho, packed, k)[fl]
+        err = np.sqrt(((ub - uo[fl]) ** 2 + (vb - vo[fl]) ** 2).sum() /
+                      (uo[fl] ** 2 + vo[fl] ** 2).sum())
+        drho = np.abs(rb - ro[fl]).max()
+        print(f"instance {k} ({nx}x{ny}, nu {nu}, lid {lid}): velocity rel L2 {err:.2e}, "
+              f"max |drho| {drho:.2e}", flush=True)
+        assert err < 2e-5, k
+        assert drho < 1e-5, k

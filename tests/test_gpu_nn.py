@@ -146,3 +146,96 @@ def test_divergence_penalty_gradients_match_oracle():
     with pytest.raises(lbm.LbmError):
         net.set_div_weight(-1.0)
     net.close()
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _adamw_ref(p, g, m, v, t, lr, wd, b1=0.9, b2=0.999, eps=1e-8, mag=0.0):
+    """One AdamW step (Loshchilov & Hutter) in float64: first and second moment
+    with bias corrections 1 - b^t, eps added to the root of the corrected
+    second moment, decay -lr * wd * p decoupled from the gradient and applied
+    to every parameter (biases included).  The hyperparameters enter as the
+    float32 numbers an FP32 optimiser holds (0.999 is 0.99900001287...; the
+    corrections 1 - b^t are formed in float32, where 1 - 0.999^2 keeps only
+    four digits).  Also returns a rounding bound of the step: `mag` carries
+    the sum of |terms| of the first moment (where b1 m and (1 - b1) g cancel,
+    float32 rounding is relative to the terms, not to their sum)."""
+    b1, b2, eps, lr, wd = (_f32(x) for x in (b1, b2, eps, lr, wd))
+    c1 = float(np.float32(1) - np.float32(b1) ** np.float32(t))
+    c2 = float(np.float32(1) - np.float32(b2) ** np.float32(t))
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mag = b1 * mag + (1 - b1) * np.abs(g)
+    mh, vh = m / c1, v / c2
+    den = np.sqrt(vh) + eps
+    step = lr * mh / den
+    f32 = 2.0 ** -24
+    # first moment to 4 roundings of its terms, the rest to 8 of the step
+    bound = lr * 4 * f32 * (mag / c1) / den + 8 * f32 * np.abs(step)
+    return p - step - lr * wd * p, m, v, mag, bound
+
+
+def _check_step(p_old, p_new, p_ref, bound, what):
+    d, d_ref = p_new - p_old, p_ref - p_old
+    # p rounds to float32 twice (the step, the decay): two ulps of p
+    ulp = np.spacing(np.abs(p_old).astype(np.float32)).astype(np.float64)
+    tol = bound + 2 * ulp + 1e-13
+    bad = np.abs(d - d_ref) > tol
+    worst = int(np.argmax(np.abs(d - d_ref) / tol))
+    assert not bad.any(), (what, int(bad.sum()), d[worst], d_ref[worst], tol[worst])
+
+
+def test_adam_steps_match_the_closed_form_update():
+    """lbm_net_train's optimiser step against the closed-form AdamW update in
+    numpy, from the gradient lbm_net_gradients returns for the same sample and
+    parameters.  Four steps over three samples whose gradients differ (so
+    beta1, beta2 and both bias corrections move the result; with one repeated
+    gradient Adam's step would not depend on them), then a fresh first step on
+    a target 1e-6 off the network's own prediction, where the gradients are of
+    the order of eps = 1e-8 and eps's place in the formula shows."""
+    print("adam closed form", flush=True)
+    H = W = 33
+    inp, tgt = _sample(3, H, W, 11)
+    tgt[1] = -2.0 * tgt[1] + 0.3
+    tgt[2] = np.roll(tgt[2], 7, axis=1) * 0.5
+    net = lbm.Net(H, W, batch=1, base=32, levels=3, seed=4)
+    lr, wd = 2e-3, 0.02
+    net.set_weight_decay(wd)
+    net.set_data(inp, tgt)
+    p0 = net.params()
+    p = p0.astype(np.float64)
+    m, v, mag = np.zeros_like(p), np.zeros_like(p), np.zeros_like(p)
+    for t, k in enumerate([0, 1, 2, 1], start=1):
+        _, g = net.gradients(inp[k:k + 1], tgt[k:k + 1])
+        p_ref, m, v, mag, bound = _adamw_ref(p, g.astype(np.float64), m, v, t, lr, wd, mag=mag)
+        net.train(np.array([k]), lr)
+        p_new = net.params().astype(np.float64)
+        _check_step(p, p_new, p_ref, bound, f"step {t}")
+        # the bound has teeth: beta2 = 0.99 instead of 0.999 would fail it
+        if t >= 2:
+            alt = _adamw_ref(p, g.astype(np.float64), m_prev, v_prev, t, lr, wd, b2=0.99,
+                             mag=mag_prev)[0]
+            assert (np.abs((p_new - p) - (alt - p)) > bound + 2 * np.spacing(
+                np.abs(p).astype(np.float32)).astype(np.float64)).mean() > 0.1
+        m_prev, v_prev, mag_prev = m, v, mag
+        p = p_new
+    # eps: a fresh Adam state (set_params resets it), gradients near 1e-8
+    net.set_params(p0)
+    y = net.predict(inp[:1])
+    near = (y + 1e-6 * np.random.default_rng(3).standard_normal(y.shape)).astype(np.float32)
+    net.set_data(inp[:1], near)
+    _, g = net.gradients(inp[:1], near)
+    gz = np.abs(g[g != 0])
+    print("median |g| near the prediction", float(np.median(gz)), flush=True)
+    assert np.median(gz) < 1e-6
+    p = p0.astype(np.float64)
+    p_ref, _, _, _, bound = _adamw_ref(p, g.astype(np.float64), 0.0, 0.0, 1, lr, wd)
+    net.train(np.array([0]), lr)
+    p_new = net.params().astype(np.float64)
+    _check_step(p, p_new, p_ref, bound, "eps step")
+    # eps on the root has teeth here: eps = 1e-6 would fail the bound
+    alt = _adamw_ref(p, g.astype(np.float64), 0.0, 0.0, 1, lr, wd, eps=1e-6)[0]
+    assert (np.abs((p_new - p) - (alt - p)) > bound).mean() > 0.1
+    net.close()

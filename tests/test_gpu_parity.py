@@ -14,7 +14,8 @@ import pytest
 import meshgen
 import oracle as O
 import parallellbmcranfield_amd as lbm
-from golden_util import ini_dict, load, residual_lines, shipped_cases, small_cases
+from golden_util import (SETUP_FIELDS, ini_dict, load, medium_cases, residual_lines, sha,
+                         shipped_cases, small_cases)
 
 pytestmark = pytest.mark.gpu
 
@@ -67,6 +68,28 @@ def test_fp64_bit_exact_on_coarse(name):
     assert np.array_equal(u, g["u"]) and np.array_equal(v, g["v"]) and np.array_equal(rho, g["rho"])
     f, _ = s.distributions()
     assert hashlib.sha256(np.ascontiguousarray(f).tobytes()).hexdigest() == str(g["f_sha256"])
+
+
+@pytest.mark.parametrize("name", medium_cases())
+def test_fp64_bit_exact_on_medium(name):
+    """The shipped medium mesh (255 x 256 nodes), 1000 steps: the regenerated
+    mesh has the reference's CellIni state, and the FP64 path ends on the
+    reference's u, v, rho and F bit for bit (checksums and three full rows)."""
+    print("case", name, flush=True)
+    g = load(name)
+    nodes, conn = meshgen.cavity(int(g["N"])).arrays()
+    case = lbm.Case.from_arrays(nodes, conn)
+    cells = case.cells()
+    for f in SETUP_FIELDS:
+        assert sha(cells[f]) == str(g[f"setup_{f}_sha256"]), f
+    case, s = run_gpu(nodes, conn, opts_from_golden(g), int(g["ini"][8]))
+    rho, u, v = s.macro()
+    n, m = int(g["n"]), int(g["m"])
+    for f, a in (("u", u), ("v", v), ("rho", rho)):
+        assert np.array_equal(a.reshape(m, n)[g["rows"]], g[f"{f}_rows"]), f
+        assert sha(a) == str(g[f"{f}_sha256"]), f
+    fd, _ = s.distributions()
+    assert sha(fd) == str(g["f_sha256"])
 
 
 # ------------------------------------------------------- switches vs the oracle
@@ -210,6 +233,24 @@ def test_run_writes_reference_results(name, tmp_path):
     (tmp_path / "SetUpData.ini").write_text("\n".join(str(v) for v in values) + "\n")
     rc = lbm.run(str(tmp_path / "SetUpData.ini"), str(tmp_path / "Mesh" / "D2node.dat"),
                  str(tmp_path / "Mesh" / "BCconnectors.dat"), str(tmp_path))
+    assert rc == lbm.OK
+    _compare_results(ref_dir, str(tmp_path / "Results"))
+
+
+@pytest.mark.parametrize("name", ["cav8_text_csv", "cav8_text_tec"])
+def test_iteration_signature_writes_reference_results(name, tmp_path, monkeypatch):
+    """lbm_iteration, the reference's Iteration() with its 18 arguments in the
+    order serial/main.c:69-86 passes them (serial/include/Iterate.h:21-27;
+    AutosaveAfter before AutosaveEvery, unlike the ini), writes Results/ in the
+    working directory as the reference program did."""
+    print("lbm_iteration", name, flush=True)
+    ref_dir, v = _text_case(name)
+    meshgen.cavity(8).write(str(tmp_path))
+    monkeypatch.chdir(tmp_path)
+    f, i = float, int
+    rc = lbm.Iteration("Mesh/D2node.dat", "Mesh/BCconnectors.dat", f(v[0]), f(v[1]), f(v[2]),
+                       f(v[3]), i(v[4]), i(v[5]), i(v[6]), i(v[7]), i(v[8]), i(v[10]), i(v[9]),
+                       i(v[11]), i(v[12]), f(v[13]), f(v[14]), i(v[15]))
     assert rc == lbm.OK
     _compare_results(ref_dir, str(tmp_path / "Results"))
 
@@ -404,30 +445,52 @@ def test_cylinder_2049x513_vortex_shedding():
     assert abs(cd32 - cd64) < 0.01 * abs(cd64)
 
 
-@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32])
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16])
 def test_checkpoint_restart_is_exact(prec, tmp_path):
     """save at t=40, keep stepping to 90; a new solver loaded from the file and
     stepped 50 more reaches the same state bit for bit (velocity residual
-    snapshot included)."""
+    snapshot included).  A solver whose lattice, rho_ini, FP16 scale,
+    collision or residual interval differs refuses the file, and so does any
+    solver given a truncated copy; a refused load leaves the solver as it was."""
+    print("checkpoint", prec, flush=True)
     nodes, conn = meshgen.cavity(40).arrays()
-    opts = lbm.make_options(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.03,
-                            init=lbm.INIT_REST, precision=prec, resid_kind=lbm.RESID_VELOCITY,
-                            resid_every=10, use_stop_rule=0)
+    kw = dict(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.03,
+              init=lbm.INIT_REST, precision=prec, resid_kind=lbm.RESID_VELOCITY,
+              resid_every=10, use_stop_rule=0)
+    opts = lbm.make_options(**kw)
     case = lbm.Case.from_arrays(nodes, conn)
+    ck = str(tmp_path / "ck.bin")
     a = lbm.Solver(case, opts)
     a.step(40)
-    a.save(str(tmp_path / "ck.bin"))
+    a.save(ck)
     a.step(50)
     b = lbm.Solver(case, opts)
-    b.load(str(tmp_path / "ck.bin"))
+    b.load(ck)
     assert b.iter == 40
     b.step(50)
     for x, y in zip(a.macro(), b.macro()):
         assert np.array_equal(x, y)
     assert np.array_equal(a.residuals(), b.residuals())
-    c = lbm.Solver(lbm.Case.cavity(41), opts)
     with pytest.raises(lbm.LbmError):
-        c.load(str(tmp_path / "ck.bin"))
+        lbm.Solver(lbm.Case.cavity(41), opts).load(ck)
+    bad = [dict(kw, rho_ini=1.5), dict(kw, collision=lbm.BGKW), dict(kw, resid_every=20)]
+    if prec == lbm.PREC_F16:
+        bad.append(dict(kw, f16_scale=2048.0))
+    for k in bad:
+        with pytest.raises(lbm.LbmError, match="does not match"):
+            lbm.Solver(case, lbm.make_options(**k)).load(ck)
+    # truncated file: refused before the device state is touched
+    raw = open(ck, "rb").read()
+    short = str(tmp_path / "short.bin")
+    open(short, "wb").write(raw[:-4096])
+    c = lbm.Solver(case, opts)
+    c.step(40)
+    with pytest.raises(lbm.LbmError):
+        c.load(short)
+    assert c.iter == 40
+    c.step(50)
+    for x, y in zip(a.macro(), c.macro()):
+        assert np.array_equal(x, y)
 
 
 def test_parallel_writer_matches_oracle_bytes(tmp_path):

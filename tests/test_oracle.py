@@ -19,7 +19,8 @@ import pytest
 
 import meshgen
 import oracle as O
-from golden_util import ini_dict, load, residual_lines, shipped_cases, small_cases
+from golden_util import (SETUP_FIELDS, ini_dict, load, medium_cases, residual_lines, sha,
+                         shipped_cases, small_cases)
 
 FIELDS_EXACT = ["fluid", "corner", "stream", "id", "boundary", "boundary_id", "bc", "q", "x",
                 "y", "uo", "vo", "u", "v", "rho", "f", "metaf", "drag", "lift"]
@@ -62,17 +63,39 @@ def test_oracle_matches_reference_on_coarse(name):
     assert hashlib.sha256(np.ascontiguousarray(cells["f"]).tobytes()).hexdigest() == str(g["f_sha256"])
 
 
-def test_shipped_mesh_regenerated():
-    """meshgen reproduces cases/coarse: same cell state as the shipped files (when present)."""
-    path = "/root/reference/cases/coarse/Mesh"
+@pytest.mark.parametrize("name,N", [("coarse", 125), ("medium", 253)])
+def test_shipped_mesh_regenerated(name, N):
+    """meshgen reproduces cases/coarse and cases/medium: same cell state as the
+    shipped files (when present)."""
+    path = f"/root/reference/cases/{name}/Mesh"
     if not os.path.isdir(path):
         pytest.skip("reference checkout not present")
     p = O.params()
     a = O.Oracle.from_files(path + "/D2node.dat", path + "/BCconnectors.dat", p).cells()
-    nodes, conn = meshgen.cavity(125).arrays()
+    nodes, conn = meshgen.cavity(N).arrays()
     b = O.Oracle(nodes, conn, p).cells()
     for f in a.dtype.names:
         assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("name", medium_cases())
+def test_oracle_matches_reference_on_medium(name):
+    """The shipped medium mesh, 1000 steps: the oracle on the regenerated mesh
+    starts from the reference's CellIni state (checksums) and ends on its
+    u, v, rho and F (checksums and three full rows)."""
+    g = load(name)
+    nodes, conn = meshgen.cavity(int(g["N"])).arrays()
+    o = O.Oracle(nodes, conn, O.params(**ini_dict(g["ini"])))
+    cells = o.cells()
+    for f in SETUP_FIELDS:
+        assert sha(cells[f]) == str(g[f"setup_{f}_sha256"]), f
+    o.step(int(g["ini"][8]))
+    cells = o.cells()
+    n, m = int(g["n"]), int(g["m"])
+    for f in ("u", "v", "rho"):
+        assert np.array_equal(cells[f].reshape(m, n)[g["rows"]], g[f"{f}_rows"]), f
+    for f in ("u", "v", "rho", "f"):
+        assert sha(cells[f]) == str(g[f"{f}_sha256"]), f
 
 
 def test_read_ini_shipped_layout(tmp_path):

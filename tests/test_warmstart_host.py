@@ -3,6 +3,7 @@ kernel launches)."""
 import math
 
 import numpy as np
+import pytest
 
 from parallellbmcranfield_amd import warmstart as WS
 
@@ -172,3 +173,15 @@ def test_density_clip_bounds_only_the_density_channel():
     inside = (t[..., 2] >= lo) & (t[..., 2] <= hi)
     assert np.array_equal(c[..., 2][inside], t[..., 2][inside])
     assert t[0, 0, 0, 2] == 50.0  # the input set is left as it was
+
+
+@pytest.mark.parametrize("q", [0.5, 0.3, 0.0001, 1.0, 1.5, -0.9])
+def test_density_clip_rejects_quantiles_outside_half_to_one(q):
+    """q <= 0.5 would collapse the density channel to one value, q >= 1
+    would clip nothing: both are refused, by clip_density and by train()."""
+    inp = np.ones((2, 5, 5, WS.IN_CH), np.float32)
+    t = np.zeros((2, 5, 5, 3), np.float32)
+    with pytest.raises(ValueError):
+        WS.clip_density(inp, t, q)
+    with pytest.raises(ValueError):
+        WS.train(None, inp, t, epochs=1, rho_clip=q)
