@@ -258,6 +258,25 @@ int lbm_solver_get_f(lbm_solver* s, double* f, double* metaf);
 int lbm_solver_write_results(lbm_solver* s, const char* path, int format, int final_layout,
                              int row0_bug);
 int lbm_solver_timing(const lbm_solver* s, lbm_timing* out);
+/* Replaces ComputeResiduals (serial/ComputeResiduals.c:6-51, declared at
+ * serial/include/ComputeResiduals.h:6-7) for a lattice state held in device
+ * memory, reduced in FP64.  f, metaf: DEVICE arrays of n*m*9 doubles,
+ * cell-major as above (Cells[idx].F[k], .METAF[k]); boundary_id (n*m ints),
+ * drag and lift (n*m doubles): DEVICE arrays, any may be NULL (no drag /
+ * lift terms); drag_lift: the ComputeDragLift id whose cells' DragF / LiftF
+ * are summed; out: DEVICE array of 4 doubles receiving Residuals[0..3]
+ * (sqrt(L2), sqrt(L2 / (m n)), drag, lift).  kind LBM_RESID_REFERENCE sums
+ * as the reference's loop is written (the first m cells, n times each, in
+ * its order: the reference's values bit for bit); LBM_RESID_FULL sums every
+ * cell once (fixed-order partial sums per 8192 values, then a fixed-order
+ * total: the same result on every run).  Work is queued on `stream` (a
+ * hipStream_t; NULL = the null stream); out is valid once the stream gets
+ * there.  A NaN in out[0] is the reference's "DIVERGENCE!" (it printed and
+ * called exit(1); ComputeResiduals.c:47-51).  LBM_RESID_FULL holds a
+ * stream-ordered allocation of 32 bytes per 8192 values for the call. */
+int lbm_compute_residuals(const double* f, const double* metaf, const int* boundary_id,
+                          const double* drag, const double* lift, int n, int m, int drag_lift,
+                          int kind, double* out, void* stream);
 /* Bytes of device memory held by the solver. */
 size_t lbm_solver_device_bytes(const lbm_solver* s);
 /* Binary checkpoint of the solver state (both distribution buffers, the

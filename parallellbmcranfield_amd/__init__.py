@@ -43,7 +43,8 @@ EXPORTS = [
     "lbm_solver_residual_history", "lbm_solver_force_history", "lbm_solver_get_macro",
     "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
-    "lbm_solver_fused", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs", "lbm_run",
+    "lbm_solver_fused", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs",
+    "lbm_compute_residuals", "lbm_run",
     "lbm_run_checkpointed", "lbm_iteration",
     "lbm_net_create", "lbm_net_destroy", "lbm_net_num_params", "lbm_net_get_params",
     "lbm_net_set_params", "lbm_net_predict", "lbm_net_gradients", "lbm_net_set_data",
@@ -153,6 +154,7 @@ def lib() -> C.CDLL:
         "lbm_solver_device_bytes": (C.c_size_t, [P]),
         "lbm_device_copy_gbs": (I, [C.c_size_t, I, C.POINTER(D)]),
         "lbm_solver_fused": (I, [P]),
+        "lbm_compute_residuals": (I, [P, P, P, P, P, I, I, I, I, P, P]),
         "lbm_solver_save": (I, [P, C.c_char_p]),
         "lbm_solver_load": (I, [P, C.c_char_p]),
         "lbm_run_checkpointed": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
@@ -552,6 +554,16 @@ def Iteration(NodeDataFile, BCconnectorDataFile, Uavg, Vavg, rho_ini, Viscosity,
                                OutletProfile, Iterations, AutosaveAfter, AutosaveEvery,
                                postproc_prog, CalculateDragLift, ConvergenceCritVeloc,
                                ConvergenceCritRho, ReadFormerData)
+
+
+def compute_residuals(f: int, metaf: int, n: int, m: int, out: int, kind: int = RESID_FULL,
+                      boundary_id: int | None = None, drag: int | None = None,
+                      lift: int | None = None, drag_lift: int = 0, stream: int | None = None):
+    """ComputeResiduals (serial/ComputeResiduals.c:6-51) over device arrays
+    (lbm_compute_residuals): f, metaf, boundary_id, drag, lift and out are
+    device addresses (ints); out receives the four Residuals on the stream."""
+    _check(lib().lbm_compute_residuals(f, metaf, boundary_id, drag, lift, n, m, drag_lift, kind,
+                                       out, stream))
 
 
 def device_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:

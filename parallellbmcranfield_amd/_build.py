@@ -36,6 +36,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math
 UNITS = [
     ("kernels_f64.hip", "hip", ["-ffp-contract=off"]),
     ("kernels_lp.hip", "hip", []),
+    ("residuals.hip", "hip", ["-ffp-contract=off"]),
     ("nn_kernels.hip", "hip", []),
     ("case_io.cpp", "host", []),
     ("solver.cpp", "host", []),

@@ -48,6 +48,16 @@ hipError_t launch_inst_resid(const double* snap, int plane, int pitch, const int
                              const uint8_t* active, int ninst, double* out, hipStream_t st);
 hipError_t launch_ref_residual(const double* cres, int m, int n, double* out, hipStream_t st);
 
+// ComputeResiduals over device arrays (residuals.hip).  LBM_RESID_FULL sums
+// 8192 flat values per partial row of RESID_ROW doubles in `part`
+// (residual_partials(n, m) rows); LBM_RESID_REFERENCE needs no `part`.
+constexpr int RESID_ROW = 4;
+long residual_partials(int n, int m);
+int check_residual_args(const double* f, const double* metaf, int n, int m, int kind, double* out);
+hipError_t launch_residuals(const double* f, const double* metaf, const int* bid,
+                            const double* drag, const double* lift, int n, int m, int drag_lift,
+                            int kind, double* out, double* part, hipStream_t st);
+
 // Plain device copy of `bytes` (multiple of 16); launch_copy_shape picks one
 // of COPY_SHAPES launch shapes (vectors in flight per thread, load kind).
 constexpr int COPY_SHAPES = 7;

@@ -226,3 +226,14 @@ def test_ini_reader(tmp_path):
     (tmp_path / "bad.ini").write_text("0.1 0.2\n")
     with pytest.raises(lbm.LbmError):
         lbm.read_ini(str(tmp_path / "bad.ini"))
+
+
+def test_compute_residuals_rejects_bad_arguments_before_any_device_work():
+    L = lbm.lib()
+    for args in [(None, 8, 8, 10, 10, 0, lbm.RESID_FULL, 8),          # no f
+                 (8, 8, 8, 10, 0, 0, lbm.RESID_FULL, 8),              # m = 0
+                 (8, 8, 8, 10, 10, 0, lbm.RESID_VELOCITY, 8),         # not a sweep kind
+                 (8, 8, 8, 10, 10, 0, lbm.RESID_FULL, None)]:         # no out
+        f, metaf, bid, n, m, dl, kind, out = args
+        rc = L.lbm_compute_residuals(f, metaf, None, None, None, n, m, dl, kind, out, None)
+        assert rc == lbm.EINVAL, args
