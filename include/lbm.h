@@ -277,6 +277,21 @@ int lbm_solver_timing(const lbm_solver* s, lbm_timing* out);
 int lbm_compute_residuals(const double* f, const double* metaf, const int* boundary_id,
                           const double* drag, const double* lift, int n, int m, int drag_lift,
                           int kind, double* out, void* stream);
+/* lbm_compute_residuals that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The partial sums of LBM_RESID_FULL go to a fixed 1 MiB array
+ * in the library's device image (one per GPU, held until the process ends)
+ * instead of a per-call allocation, so LBM_RESID_FULL takes at most
+ * LBM_NOALLOC_MAX_CELLS cells (n*m) and refuses larger lattices with
+ * LBM_EINVAL.  Same arguments, kernels and results as lbm_compute_residuals,
+ * bit for bit.  All calls on one GPU share its array: calls that overlap on
+ * one GPU (other streams, other host threads) race and must be serialised
+ * by the caller. */
+#define LBM_NOALLOC_MAX_CELLS 29826161L
+int lbm_compute_residuals_noalloc(const double* f, const double* metaf, const int* boundary_id,
+                                  const double* drag, const double* lift, int n, int m,
+                                  int drag_lift, int kind, double* out, void* stream);
 /* Bytes of device memory held by the solver. */
 size_t lbm_solver_device_bytes(const lbm_solver* s);
 /* Binary checkpoint of the solver state (both distribution buffers, the

@@ -44,7 +44,7 @@ EXPORTS = [
     "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
     "lbm_solver_fused", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs",
-    "lbm_compute_residuals", "lbm_run",
+    "lbm_compute_residuals", "lbm_compute_residuals_noalloc", "lbm_run",
     "lbm_run_checkpointed", "lbm_iteration",
     "lbm_net_create", "lbm_net_destroy", "lbm_net_num_params", "lbm_net_get_params",
     "lbm_net_set_params", "lbm_net_predict", "lbm_net_gradients", "lbm_net_set_data",
@@ -155,6 +155,7 @@ def lib() -> C.CDLL:
         "lbm_device_copy_gbs": (I, [C.c_size_t, I, C.POINTER(D)]),
         "lbm_solver_fused": (I, [P]),
         "lbm_compute_residuals": (I, [P, P, P, P, P, I, I, I, I, P, P]),
+        "lbm_compute_residuals_noalloc": (I, [P, P, P, P, P, I, I, I, I, P, P]),
         "lbm_solver_save": (I, [P, C.c_char_p]),
         "lbm_solver_load": (I, [P, C.c_char_p]),
         "lbm_run_checkpointed": (I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,

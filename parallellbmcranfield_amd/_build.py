@@ -37,6 +37,7 @@ UNITS = [
     ("kernels_f64.hip", "hip", ["-ffp-contract=off"]),
     ("kernels_lp.hip", "hip", []),
     ("residuals.hip", "hip", ["-ffp-contract=off"]),
+    ("residuals_noalloc.hip", "hip", ["-ffp-contract=off"]),
     ("nn_kernels.hip", "hip", []),
     ("case_io.cpp", "host", []),
     ("solver.cpp", "host", []),
