@@ -52,7 +52,14 @@ enum { LBM_FORMAT_CSV = 1, LBM_FORMAT_TECPLOT = 2 };
 enum {
   LBM_PREC_F64 = 0, /* FP64 f, FP64 arithmetic in the reference's evaluation order */
   LBM_PREC_F32 = 1, /* FP32 storage of f - w_k*rho0, FP32 arithmetic */
-  LBM_PREC_F16 = 2  /* FP16 storage of (f - w_k*rho0)*scale, FP32 arithmetic */
+  LBM_PREC_F16 = 2, /* FP16 storage of (f - w_k*rho0)*scale, FP32 arithmetic */
+  LBM_PREC_F16_SR = 3 /* the same with stochastic rounding of the stored values: the FP16
+                         neighbour above with probability (distance to the one below) /
+                         (their spacing), from a hash of the value's place and the step
+                         number (reproducible run to run); where per-step changes fall
+                         below half an FP16 spacing (viscous, slowly changing flows) the
+                         rounding errors then add up as a random walk instead of
+                         coherently */
 };
 /* Initial state. */
 enum {
@@ -316,6 +323,15 @@ int lbm_solver_fused(const lbm_solver* s);
  * practical HBM ceiling the step kernels are compared with.  No counterpart
  * in the reference. */
 int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs);
+/* Bandwidth of the step kernel's memory traffic without its arithmetic, on
+ * the padded layout of an n x m lattice: every cell of the lattice's rows
+ * pulls its nine values from its upstream neighbours and stores them (same
+ * planes, offsets, tiles and XCD order as the step), elem_bytes 4 (FP32) or
+ * 8 (FP64).  *gbs: bytes moved (2 * 9 * elem_bytes per cell) over the average
+ * of `reps` launches after a warm-up; *us (may be NULL): that average in
+ * microseconds.  The practical ceiling of the step's access pattern.  No
+ * counterpart in the reference. */
+int lbm_device_stream_probe(int n, int m, int elem_bytes, int reps, double* gbs, double* us);
 
 /* ---------------------------------------------------------------- driver */
 /* The whole reference program, serial/main.c:20-93 + Iteration

@@ -24,7 +24,7 @@ CLI_PATH = os.path.join(_PKG, "bin", "lbm_solve")
 OK, EIO, EPARSE, DIVERGED, EHIP, EINVAL, ENOMEM = range(7)
 OUTLET_VELOCITY, OUTLET_OPEN2, OUTLET_OPEN1, OUTLET_PRESSURE = 1, 2, 3, 4
 BGKW, TRT, MRT = 1, 2, 3
-PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
+PREC_F64, PREC_F32, PREC_F16, PREC_F16_SR = 0, 1, 2, 3
 INIT_REFERENCE, INIT_REST, INIT_FIELD, INIT_FIELD_NEQ = 0, 1, 2, 3
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
@@ -44,7 +44,8 @@ EXPORTS = [
     "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
     "lbm_solver_fused", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs",
-    "lbm_compute_residuals", "lbm_compute_residuals_noalloc", "lbm_run",
+    "lbm_compute_residuals", "lbm_compute_residuals_noalloc", "lbm_device_stream_probe",
+    "lbm_run",
     "lbm_run_checkpointed", "lbm_iteration",
     "lbm_net_create", "lbm_net_destroy", "lbm_net_num_params", "lbm_net_get_params",
     "lbm_net_set_params", "lbm_net_predict", "lbm_net_gradients", "lbm_net_set_data",
@@ -153,6 +154,7 @@ def lib() -> C.CDLL:
         "lbm_solver_timing": (I, [P, C.POINTER(Timing)]),
         "lbm_solver_device_bytes": (C.c_size_t, [P]),
         "lbm_device_copy_gbs": (I, [C.c_size_t, I, C.POINTER(D)]),
+        "lbm_device_stream_probe": (I, [I, I, I, I, C.POINTER(D), C.POINTER(D)]),
         "lbm_solver_fused": (I, [P]),
         "lbm_compute_residuals": (I, [P, P, P, P, P, I, I, I, I, P, P]),
         "lbm_compute_residuals_noalloc": (I, [P, P, P, P, P, I, I, I, I, P, P]),
@@ -555,6 +557,14 @@ def Iteration(NodeDataFile, BCconnectorDataFile, Uavg, Vavg, rho_ini, Viscosity,
                                OutletProfile, Iterations, AutosaveAfter, AutosaveEvery,
                                postproc_prog, CalculateDragLift, ConvergenceCritVeloc,
                                ConvergenceCritRho, ReadFormerData)
+
+
+def device_stream_probe(n: int, m: int, elem_bytes: int = 4, reps: int = 20):
+    """(GB/s, average us) of the step's loads and stores without arithmetic on
+    an n x m lattice's layout (lbm_device_stream_probe)."""
+    g, us = C.c_double(), C.c_double()
+    _check(lib().lbm_device_stream_probe(n, m, elem_bytes, reps, C.byref(g), C.byref(us)))
+    return g.value, us.value
 
 
 def compute_residuals(f: int, metaf: int, n: int, m: int, out: int, kind: int = RESID_FULL,

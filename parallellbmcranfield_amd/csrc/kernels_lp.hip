@@ -1,5 +1,6 @@
 // kernels_lp.hip — FP32 and FP16 storage instantiations (deviation form,
-// FP32 arithmetic; the compiler may contract multiply-adds here).
+// FP32 arithmetic; the compiler may contract multiply-adds here).  prec 3 is
+// FP16 storage with stochastic rounding.
 #include "lbm_kernels.inl"
 #include "lbm_launch.hpp"
 #include "lbm_pass_impl.inl"
@@ -19,6 +20,12 @@ hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& 
       case 2: return pass_impl<StF16, 2>(mode, pass, L);
       case 3: return pass_impl<StF16, 3>(mode, pass, L);
     }
+  } else if (prec == 3) {
+    switch (coll) {
+      case 1: return pass_impl<StF16S, 1>(mode, pass, L);
+      case 2: return pass_impl<StF16S, 2>(mode, pass, L);
+      case 3: return pass_impl<StF16S, 3>(mode, pass, L);
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -36,6 +43,12 @@ hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L) {
       case 2: return init_impl<StF16, 2>(f0_kind, L);
       case 3: return init_impl<StF16, 3>(f0_kind, L);
     }
+  } else if (prec == 3) {
+    switch (coll) {
+      case 1: return init_impl<StF16S, 1>(f0_kind, L);
+      case 2: return init_impl<StF16S, 2>(f0_kind, L);
+      case 3: return init_impl<StF16S, 3>(f0_kind, L);
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -43,6 +56,7 @@ hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L) {
 hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf) {
   if (prec == 1) return fill_impl<StF32>(L, buf);
   if (prec == 2) return fill_impl<StF16>(L, buf);
+  if (prec == 3) return fill_impl<StF16S>(L, buf);
   return hipErrorInvalidValue;
 }
 
@@ -112,3 +126,48 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st)
 }
 
 }  // namespace lbmdev
+
+// ---------------------------------------------------------------- pull-stream probe
+// The step kernel's memory traffic without its arithmetic: each cell of
+// storage rows 1..m loads its nine values from the upstream neighbours (the
+// step's offsets, planes, 256-cell tiles and grouped XCD order) and stores
+// them with non-temporal stores into the other buffer.  Its bandwidth is the
+// practical ceiling of the step's access pattern on the device.  Every pull
+// stays inside the nine planes: rows 0 and m+1 are halo, and a plane holds
+// at least (m + 2) * pitch elements.
+namespace lbmdev {
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pull_probe(const T* __restrict__ src,
+                                                    T* __restrict__ dst, int plane, int pitch,
+                                                    int begin, int end, unsigned G) {
+  unsigned b = blockIdx.x;
+  if (G > 0) b = xcd_group_block(b, G);
+  const int idx = begin + (int)(b * 256 + threadIdx.x);
+  if (idx >= end) return;
+  T v[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) v[k] = ld(src + k * plane + idx - CX[k] - CY[k] * pitch);
+#pragma unroll
+  for (int k = 0; k < 9; k++) __builtin_nontemporal_store(v[k], dst + k * plane + idx);
+}
+
+hipError_t launch_pull_probe(int elem_bytes, const void* src, void* dst, int plane, int pitch,
+                             int begin, int end, int G, hipStream_t st) {
+  const unsigned g = G > 0 ? (unsigned)G : 0u;
+  const long align = 8L * (g > 0 ? g : 1);
+  long nb = (long)(end - begin + 255) / 256;
+  nb = (nb + align - 1) / align * align;  // the grouped order is a permutation of whole runs
+  if (elem_bytes == 4)
+    k_pull_probe<float><<<(unsigned)nb, 256, 0, st>>>((const float*)src, (float*)dst, plane,
+                                                      pitch, begin, end, g);
+  else if (elem_bytes == 8)
+    k_pull_probe<double><<<(unsigned)nb, 256, 0, st>>>((const double*)src, (double*)dst, plane,
+                                                       pitch, begin, end, g);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace lbmdev
+

@@ -61,29 +61,128 @@ __device__ inline T ld(const T* p) {
     return *p;
 }
 
+// store() writes a step's result; store_rn() writes fixed values (the solid
+// value, set-up fills) rounded to nearest in every storage kind.
 struct StF64 {
   using T = double;
   using R = double;  // arithmetic type
   static constexpr bool DEV = false;
+  static constexpr bool SR = false;
   __device__ static R load(const T* p, const KParams&) { return ld(p); }
-  __device__ static void store(T* p, R v, const KParams&) { __builtin_nontemporal_store(v, p); }
+  __device__ static void store(T* p, R v, const KParams&, uint32_t) { __builtin_nontemporal_store(v, p); }
+  __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
+  __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) store(dst + k * kp.plane + idx, v[k], kp, 0u);
+  }
 };
 struct StF32 {
   using T = float;
   using R = float;
   static constexpr bool DEV = true;
+  static constexpr bool SR = false;
   __device__ static R load(const T* p, const KParams&) { return ld(p); }
-  __device__ static void store(T* p, R v, const KParams&) { __builtin_nontemporal_store(v, p); }
+  __device__ static void store(T* p, R v, const KParams&, uint32_t) { __builtin_nontemporal_store(v, p); }
+  __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
+  __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) store(dst + k * kp.plane + idx, v[k], kp, 0u);
+  }
 };
 struct StF16 {
   using T = _Float16;
   using R = float;
   static constexpr bool DEV = true;
+  static constexpr bool SR = false;
   __device__ static R load(const T* p, const KParams& kp) { return (float)ld(p) * kp.inv_scale_f; }
-  __device__ static void store(T* p, R v, const KParams& kp) {
+  __device__ static void store(T* p, R v, const KParams& kp, uint32_t) {
     __builtin_nontemporal_store((_Float16)(v * kp.scale_f), p);
   }
+  __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
+  __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) store(dst + k * kp.plane + idx, v[k], kp, 0u);
+  }
 };
+
+// Stochastic rounding to FP16.  Round to nearest (StF16) stores every value
+// with an error that depends only on the value, so where a population's
+// per-step change is below half an FP16 step it is lost the same way step
+// after step, and the errors add up coherently; with the neighbour chosen at
+// random in proportion to the distance to each (expected value exact) they
+// add up as a random walk instead.  The random words hash the value's place
+// in the buffer and the step number, so a run repeats bit for bit.
+//
+// 32-bit integer hash: the "lowbias32" finaliser (xor-shift / multiply
+// constants from C. Wellons' hash-prospector search, public domain).
+__device__ inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+// The random words of a cell's nine stores in a step: one hash of the cell
+// pair (cells 2q and 2q + 1 of a plane share a 4-byte word) and the step
+// number, then one xorshift32 step (Marsaglia 2003) per direction, so the
+// paired kernel and the one-cell paths draw the same words.  The second
+// cell of a pair takes the word rotated by 16 bits, so the two draw on
+// different bits whichever of the word's bits the conversion reads.
+__device__ inline uint32_t sr_pair_bits(int idx, uint32_t step) {
+  return mix32((uint32_t)(idx >> 1) ^ (step * 0x9e3779b9u));
+}
+__device__ inline uint32_t xorshift32(uint32_t x) {
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+__device__ inline uint32_t rot16(uint32_t x) { return (x >> 16) | (x << 16); }
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+// v rounded to one of its two FP16 neighbours at random, the upper with
+// probability (distance to the lower) / (their spacing): gfx950's
+// v_cvt_sr_f16_f32 adds random bits below FP16's precision before it
+// truncates, with r as the random word; `second` picks the half it writes.
+__device__ inline half2_t f16_stochastic_into(half2_t x, float v, uint32_t r, bool second) {
+  return second ? __builtin_amdgcn_cvt_sr_f16_f32(x, v, r, true)
+                : __builtin_amdgcn_cvt_sr_f16_f32(x, v, r, false);
+}
+struct StF16S : StF16 {
+  static constexpr bool SR = true;
+  __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp,
+                                    uint32_t step) {
+    uint32_t r = sr_pair_bits(idx, step);
+    const bool second = (idx & 1) != 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      half2_t x = {(_Float16)0, (_Float16)0};
+      x = f16_stochastic_into(x, v[k] * kp.scale_f, second ? rot16(r) : r, false);
+      __builtin_nontemporal_store(x[0], dst + k * kp.plane + idx);
+      r = xorshift32(r);
+    }
+  }
+  __device__ static void store_rn(T* p, R v, const KParams& kp) { StF16::store(p, v, kp, 0u); }
+};
+
+// The step number a launch stores with (stochastic-rounding storage; 0
+// otherwise), read once at the start of the kernel and handed to the stores:
+// read inside each store it would be re-read after every store to memory of
+// the same type.  A step's launches (MODE 0 / 1) also hand it on to the next
+// step from their first workgroup.
+template <class S>
+__device__ inline uint32_t step_number(const KParams& p) {
+  if constexpr (S::SR)
+    return p.seed[p.seed_slot];
+  else
+    return 0u;
+}
+template <class S>
+__device__ inline void advance_seed(const KParams& p, uint32_t step) {
+  if constexpr (S::SR) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.seed[p.seed_slot ^ 1] = step + 1u;
+  }
+}
 
 // ---------------------------------------------------------------- collision
 // Reference-order FP64 kernels (absolute f).

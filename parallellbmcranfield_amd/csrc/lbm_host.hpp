@@ -84,6 +84,7 @@ struct lbm_solver {
   double* d_cres = nullptr;   // [m][12]
   double* d_part = nullptr;   // [nparts][4]
   double* d_hist = nullptr;   // [HCAP][4]
+  uint32_t* d_seed = nullptr; // FP16 stochastic rounding: step number, 2 slots
   int nparts = 0;
   int nb_bulk = 0, nb_bnd = 0, nb_out = 0;
   bool fused = true;          // bulk + boundary-list nodes in one launch

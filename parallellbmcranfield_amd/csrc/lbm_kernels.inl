@@ -134,7 +134,7 @@ __device__ inline void apply_bc(typename S::R F[9], const BNode& nd, const typen
 template <class S, int COLL, int MODE, bool BND>
 __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
                                    typename S::T* dst, const KParams& p, const KOut& o, int idx,
-                                   const BNode* nd, double (&acc)[NP], int inst = -1) {
+                                   const BNode* nd, double (&acc)[NP], int inst, uint32_t step) {
   using R = typename S::R;
   R Rh, U, V, dr = 0;
   if constexpr (S::DEV) {
@@ -234,8 +234,7 @@ __device__ inline void finish_node(typename S::R F[9], const typename S::T* src,
       collide_dev<COLL>(F, dr, Rh, U, V, coll_par(p, inst), out);
     else
       collide_f64<COLL>(F, Rh, U, V, p, out);
-#pragma unroll
-    for (int k = 0; k < 9; k++) S::store(dst + k * p.plane + idx, out[k], p);
+    S::store_cell(dst, idx, out, p, step);
   }
 }
 
@@ -294,6 +293,8 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
                                              const uint8_t* __restrict__ tflag, KParams p,
                                              KOut o) {
   using R = typename S::R;
+  const uint32_t step = step_number<S>(p);
+  if constexpr (MODE != 2) advance_seed<S>(p, step);
   unsigned b = blockIdx.x;
   if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x);
   else if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(BS) void k_bulk(const typename S::T* __restrict__ s
   if (active) {
     R F[9];
     pull_cell<S>(F, src, p, idx, c);
-    finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst);
+    finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst, step);
   }
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
@@ -336,6 +337,8 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
                                                   const BNode* __restrict__ bn, int nbn, KParams p,
                                                   KOut o) {
   using R = typename S::R;
+  const uint32_t step = step_number<S>(p);
+  if constexpr (MODE != 2) advance_seed<S>(p, step);
   const int i = blockIdx.x * 256 + threadIdx.x;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < nbn && !(bn[i].flags & BN_OUTLET) && bnode_live(p, bn[i])) {
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
 #pragma unroll
       for (int k = 0; k < 9; k++) side[k] = F[k];
     }
-    finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst);
+    finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst, step);
   }
   if constexpr (MODE == 1) block_sum<256>(acc, o.part, p.nb_bulk + blockIdx.x);
 }
@@ -364,6 +367,8 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
                                              const BNode* __restrict__ bn, int nbn, KParams p,
                                              KOut o) {
   using R = typename S::R;
+  const uint32_t step = step_number<S>(p);
+  if constexpr (MODE != 2) advance_seed<S>(p, step);
   const int nbb = p.nb_bnd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if ((int)blockIdx.x < nbb) {
@@ -378,7 +383,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
 #pragma unroll
         for (int k = 0; k < 9; k++) side[k] = F[k];
       }
-      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst);
+      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst, step);
     }
   } else {
     unsigned b = blockIdx.x - nbb;
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
     if (active) {
       R F[9];
       pull_cell<S>(F, src, p, idx, c);
-      finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst);
+      finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst, step);
     }
   }
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
@@ -427,10 +432,21 @@ struct Pair<StF16> {
   __device__ static float hi(W w, const KParams& p) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)) * p.inv_scale_f;
   }
-  __device__ static W pack(float a, float b, const KParams& p) {
+  __device__ static W pack(float a, float b, uint32_t, const KParams& p) {
     const uint16_t l = __builtin_bit_cast(uint16_t, (_Float16)(a * p.scale_f));
     const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)(b * p.scale_f));
     return (uint32_t)l | ((uint32_t)h << 16);
+  }
+};
+// Stochastic rounding: the pair's random word r of direction k (the same
+// word StF16S::store_cell draws for each of the two cells alone).
+template <>
+struct Pair<StF16S> : Pair<StF16> {
+  __device__ static W pack(float a, float b, uint32_t r, const KParams& p) {
+    half2_t x = {(_Float16)0, (_Float16)0};
+    x = f16_stochastic_into(x, a * p.scale_f, r, false);
+    x = f16_stochastic_into(x, b * p.scale_f, rot16(r), true);
+    return __builtin_bit_cast(W, x);
   }
 };
 
@@ -444,6 +460,8 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
   using R = float;
   using PW = Pair<S>;
   using W = typename PW::W;
+  const uint32_t step = step_number<S>(p);
+  advance_seed<S>(p, step);
   const int nbb = p.nb_bnd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if ((int)blockIdx.x < nbb) {
@@ -458,7 +476,7 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
 #pragma unroll
         for (int k = 0; k < 9; k++) side[k] = F[k];
       }
-      finish_node<S, COLL, 0, true>(F, src, dst, p, o, nd.idx, &nd, acc, -1);
+      finish_node<S, COLL, 0, true>(F, src, dst, p, o, nd.idx, &nd, acc, -1, step);
     }
     return;
   }
@@ -490,9 +508,12 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
       collide_dev<COLL>(F1, dr, Rh, U, V, coll_par(p, -1), out1);
     }
     W* dw = reinterpret_cast<W*>(dst);
+    uint32_t r = S::SR ? sr_pair_bits(c0, step) : 0u;
 #pragma unroll
-    for (int k = 0; k < 9; k++)
-      __builtin_nontemporal_store(PW::pack(out0[k], out1[k], p), dw + ((k * p.plane + c0) >> 1));
+    for (int k = 0; k < 9; k++) {
+      __builtin_nontemporal_store(PW::pack(out0[k], out1[k], r, p), dw + ((k * p.plane + c0) >> 1));
+      if constexpr (S::SR) r = xorshift32(r);
+    }
     return;
   }
   for (int h = 0; h < 2; h++) {  // per-cell path of k_step for each tile of the pair
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
     if (c != 1 && c != 4) continue;
     R F[9];
     pull_cell<S>(F, src, p, idx, c);
-    finish_node<S, COLL, 0, false>(F, src, dst, p, o, idx, nullptr, acc, -1);
+    finish_node<S, COLL, 0, false>(F, src, dst, p, o, idx, nullptr, acc, -1, step);
   }
 }
 
@@ -520,6 +541,8 @@ __global__ __launch_bounds__(64) void k_outlet(const typename S::T* __restrict__
                                                const int* __restrict__ row_start, int nrows,
                                                KParams p, KOut o) {
   using R = typename S::R;
+  const uint32_t step = step_number<S>(p);
+  if constexpr (MODE != 2) advance_seed<S>(p, step);
   const int r = blockIdx.x * 64 + threadIdx.x;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (r < nrows) {
@@ -543,7 +566,7 @@ __global__ __launch_bounds__(64) void k_outlet(const typename S::T* __restrict__
       R* mine = side + (size_t)nd.side * 9;
 #pragma unroll
       for (int k = 0; k < 9; k++) mine[k] = F[k];
-      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc);
+      finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, -1, step);
     }
   }
   if constexpr (MODE == 1)
@@ -557,11 +580,14 @@ __global__ __launch_bounds__(256) void k_init(typename S::T* __restrict__ dst,
                                               const uint8_t* __restrict__ cls, int f0_kind,
                                               KParams p, KOut o) {
   using R = typename S::R;
+  const uint32_t step = step_number<S>(p);
+  advance_seed<S>(p, step);
   const int idx = p.bulk_begin + (int)(blockIdx.x * 256 + threadIdx.x);
   if (idx >= p.bulk_end) return;
   if (cls[idx] == 0) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) S::store(dst + k * p.plane + idx, S::DEV ? (R)(-W64[k] * p.rho0) : (R)0, p);
+    for (int k = 0; k < 9; k++)
+      S::store_rn(dst + k * p.plane + idx, S::DEV ? (R)(-W64[k] * p.rho0) : (R)0, p);
     return;
   }
   const double rho = o.rho[idx], u = o.u[idx], v = o.v[idx];
@@ -601,8 +627,7 @@ __global__ __launch_bounds__(256) void k_init(typename S::T* __restrict__ dst,
   } else {
     collide_f64<COLL>(F0, rho, u, v, p, out);
   }
-#pragma unroll
-  for (int k = 0; k < 9; k++) S::store(dst + k * p.plane + idx, out[k], p);
+  S::store_cell(dst, idx, out, p, step);
 }
 
 // Fill a storage buffer with the solid value everywhere (halo included).
@@ -612,7 +637,8 @@ __global__ void k_fill_solid(typename S::T* __restrict__ buf, KParams p) {
   if (i >= p.plane) return;
   using R = typename S::R;
 #pragma unroll
-  for (int k = 0; k < 9; k++) S::store(buf + k * p.plane + i, S::DEV ? (R)(-W64[k] * p.rho0) : (R)0, p);
+  for (int k = 0; k < 9; k++)
+    S::store_rn(buf + k * p.plane + i, S::DEV ? (R)(-W64[k] * p.rho0) : (R)0, p);
 }
 
 }  // namespace lbmdev

@@ -48,6 +48,13 @@ struct KParams {
   const uint32_t* wmask;        // class-4 cells: StreamLattice bits 0-8, wall links 16-24
   const uint32_t* tile_list;    // bulk tiles to run, in launch order (null: all);
                                 // 0xffffffff pads the list to the XCD grouping
+  // FP16 storage with stochastic rounding: the step number lives on the
+  // device in two slots, so replayed graphs advance it.  A step reads
+  // seed[seed_slot]; its first workgroup writes seed[seed_slot ^ 1] = that + 1
+  // for the next step (no one reads that slot during the launch).  The random
+  // words of a stored value hash its cell and the step number.
+  uint32_t* seed;
+  int seed_slot;
 };
 
 // Boundary-node record (host builds, kernel reads).  One per fluid node that

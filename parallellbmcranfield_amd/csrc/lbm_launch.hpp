@@ -63,5 +63,9 @@ hipError_t launch_residuals(const double* f, const double* metaf, const int* bid
 constexpr int COPY_SHAPES = 7;
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 hipError_t launch_copy_shape(const void* src, void* dst, size_t bytes, int shape, hipStream_t st);
+// The step's loads and stores without arithmetic over an n x m lattice's
+// padded layout (plane, pitch; cells [begin, end)), elem_bytes 4 or 8.
+hipError_t launch_pull_probe(int elem_bytes, const void* src, void* dst, int plane, int pitch,
+                             int begin, int end, int G, hipStream_t st);
 
 }  // namespace lbmdev

@@ -30,7 +30,7 @@ static hipError_t fused_launch(const Launch& L) {
   using T = typename S::T;
   const T* src = (const T*)L.src;
   T* dst = (T*)L.dst;
-  if constexpr (std::is_same_v<S, StF16> && MODE == 0) {
+  if constexpr ((std::is_same_v<S, StF16> || std::is_same_v<S, StF16S>) && MODE == 0) {
     if (L.pairs) {  // paired-cell plain step (nb_bulk even, grouping by G / 2 pairs)
       const int grid2 = L.kp.nb_bnd + L.nb_bulk / 2;
       if (grid2 <= 0) return hipSuccess;

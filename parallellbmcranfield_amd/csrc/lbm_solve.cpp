@@ -15,7 +15,7 @@
 static void usage() {
   std::fprintf(stderr,
                "usage: lbm_solve [--ini FILE] [--nodes FILE] [--bc FILE] [--out DIR]\n"
-               "                 [--precision f64|f32|f16] [--init reference|rest]\n"
+               "                 [--precision f64|f32|f16|f16sr] [--init reference|rest]\n"
                "                 [--resid reference|full|velocity|none] [--resid-every K]\n"
                "                 [--trt-lambda X] [--trt-fp64] [--mrt-textbook]\n"
                "                 [--mrt-rates s0,s1,...,s8] [--restart FILE]\n"
@@ -80,6 +80,7 @@ int main(int argc, char** argv) {
     lbm_options_from_ini(&ini, &opt);
     if (precision == "f32") opt.precision = LBM_PREC_F32;
     else if (precision == "f16") opt.precision = LBM_PREC_F16;
+    else if (precision == "f16sr") opt.precision = LBM_PREC_F16_SR;
     else if (!precision.empty() && precision != "f64") { usage(); return 2; }
     if (init == "rest") opt.init = LBM_INIT_REST;
     else if (!init.empty() && init != "reference") { usage(); return 2; }

@@ -130,6 +130,12 @@ int lbm_net_create(int H, int W, int batch, int base, int levels, int in_channel
       ((long)base << (levels - 1)) > 2048 ||
       in_channels < 1 || in_channels > CIN0 || dilation < 1 || dilation > 64)
     return LBM_EINVAL;
+  // the kernels index activations with 32-bit offsets: the largest tensor
+  // (batch x H x W x max(base, padded input channels)) must stay below 2^31
+  if ((long long)batch * H * W * (base > CIN0 ? base : CIN0) >= (1LL << 31)) {
+    set_error("network tensors must hold fewer than 2^31 elements (batch x H x W x channels)");
+    return LBM_EINVAL;
+  }
   *out = nullptr;
   lbm_net* n = new lbm_net();
   n->H = H;

@@ -88,7 +88,7 @@ Launch make_launch(lbm_solver* s, int src_buf) {
   // FP16, fused, single lattice of >= 16384 tiles (at 1025^2, 4.4 k tiles,
   // half as many workgroups under-fill the chip: 18.4 -> 21.2 us), even XCD
   // grouping (the pair grouping must stay a permutation): paired-cell steps
-  L.pairs = s->prec == 2 && s->nb_bulk >= 16384 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
+  L.pairs = (s->prec == LBM_PREC_F16 || s->prec == LBM_PREC_F16_SR) && s->nb_bulk >= 16384 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
             s->kp.xcd_group % 2 == 0 && s->nb_bulk % (8 * s->kp.xcd_group) == 0 &&
             !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
   L.out.rho = s->d_rho;
@@ -98,6 +98,9 @@ Launch make_launch(lbm_solver* s, int src_buf) {
   L.out.snap = s->d_snap;
   L.out.side = s->d_side;
   L.stream = s->stream;
+  // stochastic rounding: the step reads the step number from the slot of its
+  // source buffer
+  L.kp.seed_slot = src_buf;
   return L;
 }
 
@@ -229,8 +232,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     set_error("collision model must be 1 (BGKW), 2 (TRT) or 3 (MRT)");
     return LBM_EINVAL;
   }
-  if (o->precision < 0 || o->precision > 2) {
-    set_error("precision must be LBM_PREC_F64/F32/F16");
+  if (o->precision < 0 || o->precision > 3) {
+    set_error("precision must be LBM_PREC_F64/F32/F16/F16_SR");
     return LBM_EINVAL;
   }
   if (o->resid_kind < 0 || o->resid_kind > 3) {
@@ -255,7 +258,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (ninst > 0) s->opt.resid_kind = LBM_RESID_VELOCITY;
   s->prec = o->precision;
   s->coll = o->collision;
-  s->elem = s->prec == 0 ? 8 : (s->prec == 1 ? 4 : 2);
+  s->elem = s->prec == 0 ? 8 : (s->prec == 1 ? 4 : 2);  // both FP16 kinds: 2
   s->pitch = ((n + 2) + 63) / 64 * 64;
   s->rows = m + 2;
   const long long plane = ((long long)s->pitch * s->rows + 255) / 256 * 256;
@@ -533,6 +536,11 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (!rc) rc = dev_alloc(s, (void**)&s->d_v, sizeof(double) * (size_t)s->plane);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_part, sizeof(double) * 6 * (size_t)s->nparts);
   if (!rc) rc = dev_alloc(s, (void**)&s->d_hist, sizeof(double) * HW * HCAP);
+  if (!rc && s->prec == LBM_PREC_F16_SR) {  // step number of stochastic rounding, 2 slots
+    rc = dev_alloc(s, (void**)&s->d_seed, 2 * sizeof(uint32_t));
+    if (!rc) rc = check_hip(hipMemset(s->d_seed, 0, 2 * sizeof(uint32_t)), "clear step number");
+    kp.seed = s->d_seed;
+  }
   if (!rc && s->opt.resid_kind == LBM_RESID_REFERENCE)
     rc = dev_alloc(s, (void**)&s->d_cres, sizeof(double) * 12 * (size_t)m);
   if (!rc && s->opt.resid_kind == LBM_RESID_VELOCITY)
@@ -776,7 +784,7 @@ void lbm_solver_destroy(lbm_solver* s) {
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
                   s->d_part, s->d_hist, s->d_inst_par, s->d_tile_inst, s->d_node_inst,
                   s->d_inst_active, s->d_inst_rect, s->d_inst_res, s->d_tile_list,
-                  s->d_wmask};
+                  s->d_wmask, s->d_seed};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -1282,7 +1290,7 @@ int lbm_solver_load(lbm_solver* s, const char* path) {
     if (h.resid_kind == LBM_RESID_VELOCITY && h.resid_every != want.resid_every)
       why += " residual-interval";
     if (std::memcmp(&h.rho0, &want.rho0, sizeof h.rho0) != 0) why += " rho_ini";
-    if (s->prec == LBM_PREC_F16 && std::memcmp(&h.f16_scale, &want.f16_scale, sizeof(float)) != 0)
+    if (s->elem == 2 && std::memcmp(&h.f16_scale, &want.f16_scale, sizeof(float)) != 0)
       why += " f16_scale";
     if (!why.empty()) {
       set_error("checkpoint does not match this solver:" + why);
@@ -1319,6 +1327,10 @@ int lbm_solver_load(lbm_solver* s, const char* path) {
     s->invalid = true;
     set_error(std::string(lbm_last_error()) + " (solver state invalid until a checkpoint loads)");
     return rc;
+  }
+  if (s->d_seed) {  // the next step (from buffer cur) is step iter + 1
+    const uint32_t next = (uint32_t)h.iter + 1u;
+    HC(hipMemcpy(s->d_seed + h.cur, &next, sizeof next, hipMemcpyHostToDevice));
   }
   s->invalid = false;
   s->iter = h.iter;
@@ -1368,6 +1380,50 @@ int lbm_device_copy_gbs(size_t bytes, int reps, double* gbs) {
   if (b) (void)hipFree(b);
   *gbs = best;
   return rc;
+}
+
+int lbm_device_stream_probe(int n, int m, int elem_bytes, int reps, double* gbs, double* us) {
+  if (!gbs || n < 1 || m < 1 || (elem_bytes != 4 && elem_bytes != 8) || reps < 1)
+    return LBM_EINVAL;
+  // the solver's padded layout (solver_create): pitch, halo rows, planes
+  const int pitch = ((n + 2) + 63) / 64 * 64;
+  const long long plane = ((long long)pitch * (m + 2) + 255) / 256 * 256;
+  if (plane * 9 >= (1LL << 31)) {
+    set_error("lattice too large for 32-bit plane indexing");
+    return LBM_EINVAL;
+  }
+  const size_t bytes = (size_t)9 * (size_t)plane * (size_t)elem_bytes + 256;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = check_hip(hipMalloc(&a, bytes), "hipMalloc");
+  if (!rc) rc = check_hip(hipMalloc(&b, bytes), "hipMalloc");
+  if (!rc) rc = check_hip(hipMemset(a, 0, bytes), "hipMemset");
+  if (!rc) rc = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+  if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
+  if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+  const int begin = pitch, end = (m + 1) * pitch;
+  double sum = 0.0;
+  for (int r = 0; !rc && r < reps + 1; r++) {
+    rc = check_hip(hipEventRecord(e0, st), "record");
+    if (!rc) rc = check_hip(launch_pull_probe(elem_bytes, (r & 1) ? b : a, (r & 1) ? a : b,
+                                              (int)plane, pitch, begin, end, 4, st), "probe");
+    if (!rc) rc = check_hip(hipEventRecord(e1, st), "record");
+    if (!rc) rc = check_hip(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (!rc && r > 0) sum += ms;  // the first launch is a warm-up
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (rc) return rc;
+  const double avg_s = sum / reps * 1e-3;
+  *gbs = 2.0 * 9.0 * elem_bytes * (double)(end - begin) / avg_s / 1e9;
+  if (us) *us = avg_s * 1e6;
+  return LBM_OK;
 }
 
 }  // extern "C"

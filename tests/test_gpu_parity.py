@@ -349,7 +349,7 @@ def test_cavity_129_bgk_fp64_1000_steps(init_rest):
         assert np.array_equal(a, b), name
 
 
-@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32])
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16_SR])
 @pytest.mark.parametrize("geom", ["cavity", "channel_open_outlet"])
 def test_graph_replay_matches_eager_launches(prec, geom):
     """Plain steps replay a captured two-step HIP graph; every replay, with
@@ -445,7 +445,7 @@ def test_cylinder_2049x513_vortex_shedding():
     assert abs(cd32 - cd64) < 0.01 * abs(cd64)
 
 
-@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16])
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16, lbm.PREC_F16_SR])
 def test_checkpoint_restart_is_exact(prec, tmp_path):
     """save at t=40, keep stepping to 90; a new solver loaded from the file and
     stepped 50 more reaches the same state bit for bit (velocity residual
@@ -474,8 +474,9 @@ def test_checkpoint_restart_is_exact(prec, tmp_path):
     with pytest.raises(lbm.LbmError):
         lbm.Solver(lbm.Case.cavity(41), opts).load(ck)
     bad = [dict(kw, rho_ini=1.5), dict(kw, collision=lbm.BGKW), dict(kw, resid_every=20)]
-    if prec == lbm.PREC_F16:
+    if prec in (lbm.PREC_F16, lbm.PREC_F16_SR):
         bad.append(dict(kw, f16_scale=2048.0))
+        bad.append(dict(kw, precision=lbm.PREC_F16 + lbm.PREC_F16_SR - prec))
     for k in bad:
         with pytest.raises(lbm.LbmError, match="does not match"):
             lbm.Solver(case, lbm.make_options(**k)).load(ck)
@@ -508,15 +509,17 @@ def test_parallel_writer_matches_oracle_bytes(tmp_path):
         assert (tmp_path / "o.csv").read_bytes() == (tmp_path / "g.csv").read_bytes(), (final, bug)
 
 
-@pytest.mark.parametrize("coll", [lbm.BGKW, lbm.TRT, lbm.MRT])
-def test_paired_cell_kernel_is_bit_identical(coll):
+@pytest.mark.parametrize("coll,prec", [(lbm.BGKW, lbm.PREC_F16), (lbm.TRT, lbm.PREC_F16),
+                                       (lbm.MRT, lbm.PREC_F16), (lbm.MRT, lbm.PREC_F16_SR),
+                                       (lbm.BGKW, lbm.PREC_F16_SR)])
+def test_paired_cell_kernel_is_bit_identical(coll, prec):
     """FP16 plain steps run two cells per thread with 32-bit loads and stores;
     the per-cell kernel (LBM_TUNE_NO_PAIRS) gives the same bits, on a cavity
     and on a channel whose cylinder makes mixed tiles, with residual and
-    observe steps in between.  The paired kernel runs on lattices of at least
-    16384 tiles."""
-    prec = lbm.PREC_F16
-    print("pairs", coll, flush=True)
+    observe steps in between (with stochastic rounding too: both draw the
+    same random bits for a value).  The paired kernel runs on lattices of at
+    least 16384 tiles."""
+    print("pairs", coll, prec, flush=True)
     # lattices of >= 16384 tiles of 256 cells (the kernel's threshold)
     for make in (lambda: lbm.Case.cavity(2050),
                  lambda: lbm.Case.channel(4000, 1050, 1200.3, 530.1, 60.5)):
@@ -559,3 +562,60 @@ def test_in_tile_walls_equal_boundary_list(prec, coll):
             assert np.array_equal(a, b)
         assert np.array_equal(out[0][1], out[1][1])
         assert np.allclose(out[0][2], out[1][2], rtol=1e-12, atol=0)
+
+
+def test_stochastic_rounding_is_unbiased_and_reproducible():
+    """FP16 storage with stochastic rounding, end to end through the kernels:
+    a uniform field (rho 1.02, u 0.03, v 0.01) gives every node the same
+    first post-collision values (stored by the set-up collision), so the
+    stored FP16 values of one direction over ~60 000 interior nodes are draws
+    of one rounding: each lies on one of the two FP16 neighbours of the FP64
+    value and their mean is that value within 4 standard errors (round to
+    nearest stores one neighbour everywhere).  One step later (the step
+    kernel's draws, on values that carry the first draws' noise) the mean is
+    still the FP64 value within a bound for that noise.  Runs repeat bit for
+    bit.  That the step number advances once per step on the device (graph
+    replays included) is pinned by the checkpoint test: a restart writes it
+    from the iteration count and must continue bit for bit."""
+    print("stochastic rounding", flush=True)
+    N = 256
+    case = lbm.Case.cavity(N, lid_type=1)
+    nc = case.n * case.m
+    field = (np.full(nc, 1.02), np.full(nc, 0.03), np.full(nc, 0.01))
+    kw = dict(collision=lbm.BGKW, uavg=0.0, rho_ini=1.0, viscosity=0.1, resid_kind=lbm.RESID_NONE)
+    W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    j, i = np.divmod(np.arange(nc), case.n)
+    inner = (i >= 5) & (i <= N - 4) & (j >= 5) & (j <= N - 4)
+    for n_steps in (1, 2):
+        ref = lbm.Solver(case, lbm.make_options(precision=lbm.PREC_F64, **kw), field=field)
+        ref.step(n_steps)
+        exact = ref.distributions()[1]
+        runs = []
+        for prec in (lbm.PREC_F16_SR, lbm.PREC_F16_SR, lbm.PREC_F16):
+            s = lbm.Solver(case, lbm.make_options(precision=prec, **kw), field=field)
+            s.step(n_steps)
+            runs.append(s.distributions()[1])
+        assert np.array_equal(runs[0], runs[1])
+        for k in range(9):
+            x = exact[inner, k]
+            assert np.all(x == x[0])
+            scaled = (x[0] - W[k]) * 4096.0
+            lo = np.float16(scaled)
+            if float(lo) > scaled:
+                lo = np.nextafter(lo, np.float16(-np.inf))
+            hi = np.nextafter(lo, np.float16(np.inf))
+            gap = float(hi) - float(lo)
+            sr = (runs[0][inner, k] - W[k]) * 4096.0
+            if n_steps == 1:
+                assert np.all((sr == float(lo)) | (sr == float(hi))), k
+                rn = (runs[2][inner, k] - W[k]) * 4096.0
+                assert np.all(rn == rn[0]), k
+                p = (scaled - float(lo)) / gap
+                se = gap * np.sqrt(max(p * (1 - p), 1e-12) / sr.size)
+            else:
+                # each value carries the first draws' noise of the nine
+                # populations it was made from (at most one spacing each)
+                # besides its own draw; neighbours share pulled values
+                se = 3 * gap * np.sqrt(9 / sr.size)
+            print(k, n_steps, "mean - exact", sr.mean() - scaled, "bound", 4 * se, flush=True)
+            assert abs(sr.mean() - scaled) <= 4 * se + 1e-9, (k, n_steps)
