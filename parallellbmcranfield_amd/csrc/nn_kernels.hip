@@ -42,16 +42,45 @@ __device__ inline bf16x8 zero8() {
 }
 
 // 8 channels [ci, ci+8) of the virtual input at (b, y, x); zeros outside.
+// Offsets in 32-bit arithmetic: every tensor holds fewer than 2^31 elements
+// (lbm_net_create checks), and a 32-bit offset from a uniform base is one
+// address register per load.
 __device__ inline bf16x8 load8(const Src& s, int b, int y, int x, int ci) {
   if (y < 0 || y >= s.H || x < 0 || x >= s.W) return zero8();
   if (ci < s.C0) {
-    const size_t off = (((size_t)b * s.H + y) * s.W + x) * s.C0 + ci;
+    const unsigned off = ((unsigned)(b * s.H + y) * (unsigned)s.W + (unsigned)x) * (unsigned)s.C0 +
+                         (unsigned)ci;
     return *reinterpret_cast<const bf16x8*>(s.in0 + off);
   }
   const int Hc = (s.H + 1) >> 1, Wc = (s.W + 1) >> 1;
-  const size_t off = (((size_t)b * Hc + (y >> 1)) * Wc + (x >> 1)) * s.C1 + (ci - s.C0);
+  const unsigned off = ((unsigned)(b * Hc + (y >> 1)) * (unsigned)Wc + (unsigned)(x >> 1)) *
+                           (unsigned)s.C1 +
+                       (unsigned)(ci - s.C0);
   return *reinterpret_cast<const bf16x8*>(s.in1 + off);
 }
+
+// (b, y, x) of pixel p in the flattened (b, y, x) order, and the step that
+// moves it n pixels on (n <= W or a few row wraps): pixel coordinates advance
+// by additions instead of a division per fetch.
+struct PixelCursor {
+  int x, y, b;
+  __device__ void at(int p, int W, int H) {
+    const int q = p / W;
+    x = p - q * W;
+    b = q / H;
+    y = q - b * H;
+  }
+  __device__ void advance(int n, int W, int H) {
+    x += n;
+    while (x >= W) {
+      x -= W;
+      if (++y == H) {
+        y = 0;
+        b++;
+      }
+    }
+  }
+};
 
 __device__ inline float load1(const Src& s, int b, int y, int x, int ci) {
   if (y < 0 || y >= s.H || x < 0 || x >= s.W) return 0.0f;
@@ -112,11 +141,37 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__
   // K position of the next load: tap lt, channels lci .. lci + 31
   int lt = 0, lci = 0;
   bf16x8 wreg = zero8();
-  auto load = [&](bf16x8 (&A)[TM]) {
+  // per tap: whether each pixel's neighbour is inside, and the offsets of its
+  // channel kb * 8 in the two halves of the virtual input (set up once per
+  // tap, at its first K step; 32-bit, as load8)
+  bool tok[TM];
+  unsigned toff0[TM], toff1[TM];
+  const int Hc = (s.H + 1) >> 1, Wc = (s.W + 1) >> 1;
+  auto tap_setup = [&]() {
     const int dy = (lt / 3 - 1) * s.dil, dx = (lt % 3 - 1) * s.dil;
 #pragma unroll
-    for (int g = 0; g < TM; g++)
-      A[g] = pv[g] ? load8(s, pbb[g], py[g] + dy, px[g] + dx, lci + kb * 8) : zero8();
+    for (int g = 0; g < TM; g++) {
+      const int y = py[g] + dy, x = px[g] + dx;
+      tok[g] = pv[g] && y >= 0 && y < s.H && x >= 0 && x < s.W;
+      const int yy = tok[g] ? y : 0, xx = tok[g] ? x : 0;
+      toff0[g] = ((unsigned)(pbb[g] * s.H + yy) * (unsigned)s.W + (unsigned)xx) * (unsigned)s.C0 +
+                 (unsigned)(kb * 8);
+      toff1[g] = ((unsigned)(pbb[g] * Hc + (yy >> 1)) * (unsigned)Wc + (unsigned)(xx >> 1)) *
+                     (unsigned)s.C1 +
+                 (unsigned)(kb * 8);
+    }
+  };
+  auto load = [&](bf16x8 (&A)[TM]) {
+    if (lci == 0) tap_setup();
+#pragma unroll
+    for (int g = 0; g < TM; g++) {
+      if (!tok[g])
+        A[g] = zero8();
+      else if (lci < s.C0)
+        A[g] = *reinterpret_cast<const bf16x8*>(s.in0 + (toff0[g] + (unsigned)lci));
+      else
+        A[g] = *reinterpret_cast<const bf16x8*>(s.in1 + (toff1[g] + (unsigned)(lci - s.C0)));
+    }
     if (wl) wreg = *reinterpret_cast<const bf16x8*>(wsrc + (size_t)lt * Cin + lci);
     lci += 32;
     if (lci == Cin) {
@@ -218,23 +273,20 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   // loader lanes: pixels 4 lq .. 4 lq + 3 of the group x 8-channel group lc
   const int lq = lane >> 2, lc = (lane & 3) * 8;
   bf16x8 va[4], vb[4];
-  auto fetch = [&](int it) {
+  PixelCursor cur;  // the lane's first pixel of the next group it fetches
+  cur.at((g0 + wave) * WG_PX + lq * 4, s.W, s.H);
+  auto fetch = [&](int it) {  // called for it = 0, 1, 2, ... in turn
     const int gi = g0 + it * 4 + wave;
     const bool live = it < nit && gi < g1;
     const int p0 = gi * WG_PX + lq * 4;
-    int x = 0, y = 0, b = 0;
-    if (live) {
-      const int q = p0 / s.W;
-      x = p0 - q * s.W;
-      b = q / s.H;
-      y = q - b * s.H;
-    }
+    int x = cur.x, y = cur.y, b = cur.b;
+    cur.advance(4 * WG_PX, s.W, s.H);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       va[j] = zero8();
       vb[j] = zero8();
       if (live && p0 + j < P) {
-        va[j] = *reinterpret_cast<const bf16x8*>(dout + (size_t)(p0 + j) * Cout + cob + lc);
+        va[j] = *reinterpret_cast<const bf16x8*>(dout + ((unsigned)(p0 + j) * (unsigned)Cout + cob + lc));
         vb[j] = load8(s, b, y + dy, x + dx, cib + lc);
       }
       if (++x == s.W) {  // next pixel in (b, y, x) order
@@ -251,13 +303,19 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
   for (int g = 0; g < 2; g++)
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[g][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // a half-wave's store writes rows lc + e (lc = 0, 8, 16, 24) at 8
+  // consecutive 8-byte slots; rows 16 apart share banks, so rows 16-31 take
+  // their slots XOR 8 (the other half of the row): conflict-free.  Operand
+  // reads cover 16 aligned rows (one swizzle per read).
+  auto swz = [](int row) { return ((row >> 4) & 1) * 32; };  // in pixels
   fetch(0);
   for (int it = 0; it < nit; it++) {
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-      *reinterpret_cast<bf16x4*>(&lds[0][wave][lc + e][lq * 4]) =
+      const int row = lc + e;
+      *reinterpret_cast<bf16x4*>(&lds[0][wave][row][(lq * 4) ^ swz(row)]) =
           bf16x4{va[0][e], va[1][e], va[2][e], va[3][e]};
-      *reinterpret_cast<bf16x4*>(&lds[1][wave][lc + e][lq * 4]) =
+      *reinterpret_cast<bf16x4*>(&lds[1][wave][row][(lq * 4) ^ swz(row)]) =
           bf16x4{vb[0][e], vb[1][e], vb[2][e], vb[3][e]};
     }
     wave_sync();
@@ -266,11 +324,15 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
     for (int ks = 0; ks < WG_PX / 32; ks++) {
       bf16x8 a[2], bm[2];
 #pragma unroll
-      for (int g = 0; g < 2; g++)
-        a[g] = *reinterpret_cast<const bf16x8*>(&lds[0][wave][g * 16 + r][ks * 32 + kb * 8]);
+      for (int g = 0; g < 2; g++) {
+        const int row = g * 16 + r;
+        a[g] = *reinterpret_cast<const bf16x8*>(&lds[0][wave][row][(ks * 32 + kb * 8) ^ swz(row)]);
+      }
 #pragma unroll
-      for (int h = 0; h < 2; h++)
-        bm[h] = *reinterpret_cast<const bf16x8*>(&lds[1][wave][h * 16 + r][ks * 32 + kb * 8]);
+      for (int h = 0; h < 2; h++) {
+        const int row = h * 16 + r;
+        bm[h] = *reinterpret_cast<const bf16x8*>(&lds[1][wave][row][(ks * 32 + kb * 8) ^ swz(row)]);
+      }
 #pragma unroll
       for (int g = 0; g < 2; g++)
 #pragma unroll
@@ -324,22 +386,19 @@ __global__ __launch_bounds__(256) void k_wgrad64(Src s, const bf16* __restrict__
   const int which = threadIdx.x >> 7, li = threadIdx.x & 127;
   const int lq = li >> 3, lc = (li & 7) * 8;
   bf16x8 v[4];
-  auto fetch = [&](int it) {
+  PixelCursor cur;  // the lane's first pixel of the next group it fetches
+  cur.at(g0 * WG_PX + lq * 4, s.W, s.H);
+  auto fetch = [&](int it) {  // called for it = 0, 1, 2, ... in turn
     const bool live = it < nit;
     const int p0 = (g0 + it) * WG_PX + lq * 4;
-    int x = 0, y = 0, b = 0;
-    if (live) {
-      const int q = p0 / s.W;
-      x = p0 - q * s.W;
-      b = q / s.H;
-      y = q - b * s.H;
-    }
+    int x = cur.x, y = cur.y, b = cur.b;
+    cur.advance(WG_PX, s.W, s.H);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       v[j] = zero8();
       if (live && p0 + j < P) {
         if (which == 0)
-          v[j] = *reinterpret_cast<const bf16x8*>(dout + (size_t)(p0 + j) * Cout + cob + lc);
+          v[j] = *reinterpret_cast<const bf16x8*>(dout + ((unsigned)(p0 + j) * (unsigned)Cout + cob + lc));
         else
           v[j] = load8(s, b, y + dy, x + dx, cib + lc);
       }
@@ -352,11 +411,20 @@ __global__ __launch_bounds__(256) void k_wgrad64(Src s, const bf16* __restrict__
       }
     }
   };
+  // LDS rows of 64 pixels = 16 slots of 4 pixels (8 bytes).  A half-wave's
+  // store writes rows lc + e (lc = 0, 8, .., 56) at 4 consecutive slots; the
+  // 144-byte rows put rows 16 apart on the same banks, so a row takes its
+  // slots XOR 4 ((row >> 4) & 3): the eight rows then cover all 64 banks.
+  // Operand reads cover 16 aligned rows, one swizzle for the whole read,
+  // which keeps each 16-byte pair of slots together.
+  auto swz = [](int row) { return ((row >> 4) & 3) * 16; };  // in pixels
   auto store = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 8; e++)
-      *reinterpret_cast<bf16x4*>(&lds[buf][which][lc + e][lq * 4]) =
+    for (int e = 0; e < 8; e++) {
+      const int row = lc + e;
+      *reinterpret_cast<bf16x4*>(&lds[buf][which][row][(lq * 4) ^ swz(row)]) =
           bf16x4{v[0][e], v[1][e], v[2][e], v[3][e]};
+    }
   };
   const int arow = (wave & 1) * 32, brow = (wave >> 1) * 32;
   f32x4 acc[2][2];
@@ -374,11 +442,15 @@ __global__ __launch_bounds__(256) void k_wgrad64(Src s, const bf16* __restrict__
     for (int ks = 0; ks < WG_PX / 32; ks++) {
       bf16x8 a[2], bm[2];
 #pragma unroll
-      for (int g = 0; g < 2; g++)
-        a[g] = *reinterpret_cast<const bf16x8*>(&lds[buf][0][arow + g * 16 + r][ks * 32 + kb * 8]);
+      for (int g = 0; g < 2; g++) {
+        const int row = arow + g * 16 + r;
+        a[g] = *reinterpret_cast<const bf16x8*>(&lds[buf][0][row][(ks * 32 + kb * 8) ^ swz(row)]);
+      }
 #pragma unroll
-      for (int h = 0; h < 2; h++)
-        bm[h] = *reinterpret_cast<const bf16x8*>(&lds[buf][1][brow + h * 16 + r][ks * 32 + kb * 8]);
+      for (int h = 0; h < 2; h++) {
+        const int row = brow + h * 16 + r;
+        bm[h] = *reinterpret_cast<const bf16x8*>(&lds[buf][1][row][(ks * 32 + kb * 8) ^ swz(row)]);
+      }
 #pragma unroll
       for (int g = 0; g < 2; g++)
 #pragma unroll
