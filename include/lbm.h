@@ -114,6 +114,8 @@ typedef struct lbm_options {
   double outlet_rho;       /* LBM_OUTLET_PRESSURE density; 0 = rho_ini */
   int    force_id;         /* momentum-exchange force on the wall links of this boundary id (0 = off) */
   int    force_every;      /* force sampled every K steps (lbm_solver_force_history) */
+  int    two_step_rows;    /* output rows per two-step tile (> 0 also turns two-step launches on
+                              at any lattice size); 0 = tuned default */
 } lbm_options;
 
 /* Launch-shape switches (results are identical; only speed changes).
@@ -127,9 +129,16 @@ typedef struct lbm_options {
  * (one cell per thread, 16-bit loads and stores).
  * LBM_TUNE_NO_TILE_WALLS: every boundary node in the boundary list (by
  * default nodes whose only rule is straight-wall bounce-back run in their
- * bulk tile). */
+ * bulk tile).
+ * LBM_TUNE_NO_TWO_STEP: every step its own launch.  By default two plain
+ * steps in a row run as one pass over the state where that pays (FP64 and
+ * FP32 storage, lattices of at least 2^22 cells, two thirds of them away
+ * from the boundary-list nodes; not batched): interior and in-tile wall
+ * nodes read METAF_t once and write METAF_{t+2} once, the nodes next to
+ * boundary-list nodes take two single steps.  two_step_rows > 0 turns them
+ * on for any lattice and storage but stochastic rounding. */
 enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2, LBM_TUNE_NO_GRAPH = 4,
-       LBM_TUNE_NO_PAIRS = 8, LBM_TUNE_NO_TILE_WALLS = 16 };
+       LBM_TUNE_NO_PAIRS = 8, LBM_TUNE_NO_TILE_WALLS = 16, LBM_TUNE_NO_TWO_STEP = 32 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;
@@ -240,7 +249,10 @@ void lbm_solver_destroy(lbm_solver* s);
 int lbm_solver_step(lbm_solver* s, int nsteps, int* done);
 /* Same, timing the step kernel with HIP events on the solver's stream around
  * every `sample_every`-th launch (1 = every launch).  total_ms: stream time of
- * the nsteps; kernel_ms: average duration of one sampled step-kernel launch. */
+ * the nsteps; kernel_ms: average duration of one sampled step-kernel launch.
+ * With two-step launches on (lbm_solver_two_step_info) the sampled kernel is
+ * the two-step kernel, one launch per two steps, and single steps are not
+ * sampled. */
 int lbm_solver_step_timed(lbm_solver* s, int nsteps, int sample_every, double* total_ms,
                           double* kernel_ms);
 int lbm_solver_sync(lbm_solver* s);
@@ -314,8 +326,16 @@ int lbm_solver_save(lbm_solver* s, const char* path);
 int lbm_solver_load(lbm_solver* s, const char* path);
 /* Kernel layout (for profiling): bit 0 set if interior and boundary-list
  * nodes share one launch per step (else separate launches); bit 1 set if
- * plain steps run the paired-cell FP16 kernel. */
+ * plain steps run the paired-cell FP16 kernel; bit 2 set if pairs of plain
+ * steps run as two-step launches. */
 int lbm_solver_fused(const lbm_solver* s);
+/* Two-step launches of this solver: *cells = nodes the two-step kernel
+ * advances by two steps per launch (0: two-step launches off), *tiles = its
+ * tiles (one wave each), *rows = output rows per tile, *band = nodes that
+ * take two single steps instead.  Any pointer may be NULL.  No counterpart
+ * in the reference. */
+int lbm_solver_two_step_info(const lbm_solver* s, long long* cells, int* tiles, int* rows,
+                             long long* band);
 
 /* Bandwidth of a plain device-to-device copy kernel over a buffer of `bytes`
  * (read + write counted), best of `reps` launches of each of several launch

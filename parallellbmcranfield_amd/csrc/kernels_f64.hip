@@ -16,6 +16,8 @@ hipError_t launch_pass_f64(int coll, int mode, int pass, const Launch& L) {
   return hipErrorInvalidValue;
 }
 
+int two_step_occupancy_f64(int coll) { return two_step_occupancy_coll<StF64>(coll); }
+
 hipError_t launch_init_f64(int coll, int f0_kind, const Launch& L) {
   switch (coll) {
     case 1: return init_impl<StF64, 1>(f0_kind, L);

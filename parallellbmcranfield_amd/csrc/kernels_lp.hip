@@ -30,6 +30,12 @@ hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& 
   return hipErrorInvalidValue;
 }
 
+int two_step_occupancy_lp(int prec, int coll) {
+  if (prec == 1) return two_step_occupancy_coll<StF32>(coll);
+  if (prec == 2) return two_step_occupancy_coll<StF16>(coll);
+  return 0;
+}
+
 hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L) {
   if (prec == 1) {
     switch (coll) {

@@ -69,6 +69,7 @@ struct StF64 {
   static constexpr bool DEV = false;
   static constexpr bool SR = false;
   __device__ static R load(const T* p, const KParams&) { return ld(p); }
+  __device__ static R round_trip(R v, const KParams&) { return v; }
   __device__ static void store(T* p, R v, const KParams&, uint32_t) { __builtin_nontemporal_store(v, p); }
   __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
   __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
@@ -82,6 +83,7 @@ struct StF32 {
   static constexpr bool DEV = true;
   static constexpr bool SR = false;
   __device__ static R load(const T* p, const KParams&) { return ld(p); }
+  __device__ static R round_trip(R v, const KParams&) { return v; }
   __device__ static void store(T* p, R v, const KParams&, uint32_t) { __builtin_nontemporal_store(v, p); }
   __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
   __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
@@ -95,6 +97,10 @@ struct StF16 {
   static constexpr bool DEV = true;
   static constexpr bool SR = false;
   __device__ static R load(const T* p, const KParams& kp) { return (float)ld(p) * kp.inv_scale_f; }
+  // what load() returns for a value store() wrote
+  __device__ static R round_trip(R v, const KParams& kp) {
+    return (float)(_Float16)(v * kp.scale_f) * kp.inv_scale_f;
+  }
   __device__ static void store(T* p, R v, const KParams& kp, uint32_t) {
     __builtin_nontemporal_store((_Float16)(v * kp.scale_f), p);
   }

@@ -90,6 +90,22 @@ struct lbm_solver {
   bool fused = true;          // bulk + boundary-list nodes in one launch
   bool use_graph = true;      // replay captured two-step graphs for plain steps
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by starting buffer parity
+  // two plain steps per launch: k_two_steps over its tiles, the band (every
+  // other node) two single steps through buf3
+  bool two = false;
+  int4* d_tiles = nullptr;    // [ntiles] (x0, y0, rows, 0), XCD-ordered, rows 0 = padding
+  uint8_t* d_cls_two = nullptr;  // classes, bit 7 set on the nodes the tiles advance
+  int ntiles = 0, ntiles_real = 0, two_rows = 0;
+  long long two_cells = 0, band_cells = 0;
+  void* buf3 = nullptr;       // METAF_{t+1} of the band and its ring
+  uint8_t* d_cls_b[2] = {nullptr, nullptr};    // classes of the band's first / second step
+  uint8_t* d_tflag_b[2] = {nullptr, nullptr};
+  uint32_t* d_list_b[2] = {nullptr, nullptr};  // bulk tiles of each band step (padded)
+  int nlist_b[2] = {0, 0};
+  hipGraphExec_t gexec2[2] = {nullptr, nullptr};  // two-step launches by source parity
+  // the last launch was a two-step one: buf[cur^1] holds the state two steps
+  // back, not the one step back that observe passes and checkpoints read
+  bool prev_stale = false;
   lbmdev::KParams kp{};
   // host state
   int iter = 0;

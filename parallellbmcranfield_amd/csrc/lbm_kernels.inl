@@ -410,6 +410,235 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
   if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
 }
 
+// ---------------------------------------------------------------- two steps per launch
+// Macros and collision of a bulk node (class 1 or 4) exactly as finish_node
+// forms them for MODE 0 outside the boundary list.
+template <class S, int COLL>
+__device__ inline void collide_bulk(const typename S::R F[9], const KParams& p,
+                                    typename S::R out[9]) {
+  using R = typename S::R;
+  R Rh, U, V, dr = 0;
+  if constexpr (S::DEV) {
+    macro_dev(F, p.rho0_f, dr, Rh, U, V);
+    collide_dev<COLL>(F, dr, Rh, U, V, coll_par(p, -1), out);
+  } else {
+    macro_f64(F, Rh, U, V);
+    collide_f64<COLL>(F, Rh, U, V, p, out);
+  }
+}
+
+// WallBC on the links a class-4 node's mask marks (bits 16-24), in the
+// order of pull_cell.
+template <class R>
+__device__ inline void wall_bounce(R F[9], uint32_t w) {
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    if ((w >> (16 + k)) & 1) F[OPP[k]] = F[k];
+}
+
+// Pull loads of one cell for step t+1, issued ahead of their use and
+// without branches (so the compiler counts them exactly and the wave waits
+// for each row's loads only when it forms that row): class 1 pulls all nine
+// from the upstream neighbours, class 4 those its stream mask w marks (its
+// own value otherwise), other classes load their own words (valid addresses,
+// values unused).
+template <class S>
+__device__ inline void issue_pulls(typename S::R F[9], const typename S::T* src,
+                                   const KParams& p, int idx, int c, uint32_t w) {
+  const uint32_t m = c == 1 ? 0x1ffu : (c == 4 ? w : 0u);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int off = ((m >> k) & 1) ? -CX[k] - CY[k] * p.pitch : 0;
+    F[k] = S::load(src + k * p.plane + idx + off, p);
+  }
+}
+
+// Value of v in lane j (j the same in every lane).
+__device__ inline float lane_value(float v, int j) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+__device__ inline double lane_value(double v, int j) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), j);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                (unsigned)lo));
+}
+
+// Step t+1 of one cell from its pulls (bulk classes 1 and 4; zeros for the
+// others, which no advanced node reads), rounded as the store would.
+template <class S, int COLL, bool WALLS>
+__device__ inline void form_t1(typename S::R kn[9], typename S::R F[9], int c, uint32_t w,
+                               const KParams& p) {
+  using R = typename S::R;
+  if (c == 1 || c == 4) {
+    if (WALLS && c == 4) wall_bounce(F, w);
+    R o[9];
+    collide_bulk<S, COLL>(F, p, o);
+#pragma unroll
+    for (int k = 0; k < 9; k++) kn[k] = S::round_trip(o[k], p);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) kn[k] = 0;
+  }
+}
+
+// Rows of the side-column batch: 32 cells of each side column, two lanes per
+// row; a batch serves the step-t+2 rows base+1 .. base+30.
+constexpr int SIDE_ROWS = 32;
+
+// One tile of k_two_steps; WALLS: the tile holds class-4 nodes (their stream
+// and wall masks are fetched with the class bytes).
+template <class S, int COLL, bool WALLS>
+__device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
+                                      typename S::T* __restrict__ dst,
+                                      const uint8_t* __restrict__ cls, const int4 tl,
+                                      const KParams& p) {
+  using R = typename S::R;
+  const int lane = threadIdx.x;
+  const int x = tl.x + lane;
+  const bool xin = x < p.pitch;
+  const int xc = xin ? x : p.pitch - 1;  // loads of lanes past the row stay in it
+  const int nit = tl.z + 2;              // step-t+1 rows y0-1 .. y0+rows
+  const int r0 = tl.y - 1, rlast = r0 + nit - 1;
+  auto cls_at = [&](int r) -> int { return xin ? (int)cls[r * p.pitch + xc] : 0; };
+  auto mask_at = [&](int r) -> uint32_t { return WALLS ? p.wmask[r * p.pitch + xc] : 0u; };
+  // Side columns x0-1 and x0+64: lane j of a batch forms the cell of column
+  // (j odd ? x0+64 : x0-1) in row base + j/2 at t+1 (zeros outside the grid
+  // or past row rlast).
+  R sd[9];
+  auto side_batch = [&](int base) {
+    const int xs = (lane & 1) ? tl.x + 64 : tl.x - 1;
+    const int row = base + (lane >> 1);
+    const bool ok = xs >= 0 && xs < p.pitch && row <= rlast;
+    const int idx = (ok ? row : r0) * p.pitch + (ok ? xs : xc);
+    const int c = ok ? (int)cls[idx] & 0x7f : 0;
+    const uint32_t w = (WALLS && ok) ? p.wmask[idx] : 0u;
+    R F[9];
+    issue_pulls<S>(F, src, p, idx, c, w);
+    form_t1<S, COLL, WALLS>(sd, F, c, w, p);
+  };
+  int sbase = r0;
+  R kd[3] = {0, 0, 0};                     // row rr-1 at t+1: directions 2, 5, 6
+  R kc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // row rr at t+1
+  int cc = 0;                              // class byte of (x, rr)
+  uint32_t wc = 0;                         // masks of (x, rr)
+  // Software pipeline: while row r is formed, the class bytes (and masks)
+  // of rows r+1, r+2 and the pulls of row r+1 are in flight.
+  int cb1 = cls_at(r0), cb2 = cls_at(min(r0 + 1, rlast));
+  uint32_t w1 = mask_at(r0), w2 = mask_at(min(r0 + 1, rlast));
+  R Fq[9];
+  issue_pulls<S>(Fq, src, p, r0 * p.pitch + xc, cb1 & 0x7f, w1);
+  // one copy of the body: unrolled copies only grow the code
+#pragma unroll 1
+  for (int it = 0; it < nit; it++) {
+    const int r = r0 + it;  // row of step t+1
+    const int cb = cb1;
+    const uint32_t w = w1;
+    R F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = Fq[k];
+    cb1 = cb2;
+    w1 = w2;
+    const int rn = min(r + 1, rlast), rn2 = min(r + 2, rlast);
+    cb2 = cls_at(rn2);
+    w2 = mask_at(rn2);
+    issue_pulls<S>(Fq, src, p, rn * p.pitch + xc, cb1 & 0x7f, w1);
+    R kn[9];  // row r at t+1
+    form_t1<S, COLL, WALLS>(kn, F, cb & 0x7f, w, p);
+    if (it >= 2) {
+      const int rr = r - 1;  // row of step t+2
+      if ((it - 2) % (SIDE_ROWS - 2) == 0) {
+        sbase = rr - 1;
+        side_batch(sbase);
+      }
+      const int jl = 2 * (rr - sbase);  // side lanes of row rr (left; right = +1)
+      // F[k](x) = f_k(x - c_k) at t+1; lanes 0 and 63 take the side columns
+      R nb[9];
+      nb[0] = kc[0];
+      nb[1] = __shfl_up(kc[1], 1, 64);
+      nb[2] = kd[0];
+      nb[3] = __shfl_down(kc[3], 1, 64);
+      nb[4] = kn[4];
+      nb[5] = __shfl_up(kd[1], 1, 64);
+      nb[6] = __shfl_down(kd[2], 1, 64);
+      nb[7] = __shfl_down(kn[7], 1, 64);
+      nb[8] = __shfl_up(kn[8], 1, 64);
+      const R l1 = lane_value(sd[1], jl), l5 = lane_value(sd[5], jl - 2);
+      const R l8 = lane_value(sd[8], jl + 2), r3 = lane_value(sd[3], jl + 1);
+      const R r6 = lane_value(sd[6], jl - 1), r7 = lane_value(sd[7], jl + 3);
+      if (lane == 0) {
+        nb[1] = l1;
+        nb[5] = l5;
+        nb[8] = l8;
+      }
+      if (lane == 63) {
+        nb[3] = r3;
+        nb[6] = r6;
+        nb[7] = r7;
+      }
+      if (xin && (cc & 0x80)) {
+        R G[9];
+        if (WALLS && (cc & 0x7f) == 4) {
+          // StreamingStep mask, then WallBC, as pull_cell
+#pragma unroll
+          for (int k = 0; k < 9; k++) G[k] = ((wc >> k) & 1) ? nb[k] : kc[k];
+          wall_bounce(G, wc);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 9; k++) G[k] = nb[k];
+        }
+        R o[9];
+        collide_bulk<S, COLL>(G, p, o);
+        S::store_cell(dst, rr * p.pitch + x, o, p, 0u);
+      }
+    }
+    kd[0] = kc[2];
+    kd[1] = kc[5];
+    kd[2] = kc[6];
+#pragma unroll
+    for (int k = 0; k < 9; k++) kc[k] = kn[k];
+    cc = cb;
+    wc = w;
+  }
+}
+
+// Two plain steps of a rectangle of the lattice in one pass over the state:
+// METAF_t is read once and METAF_{t+2} written once, where two single steps
+// read and write the whole state twice.  One wave per tile (x0, y0, rows,
+// walls): lane l works on padded column x0 + l (x0 a multiple of 64, so every
+// row of a tile is whole cache lines: stores that share a line with another
+// tile's lose half the bandwidth); the outputs are rows y0 .. y0 + rows - 1.
+// The wave marches up the rows.  For row r it forms step t+1 of its 64
+// columns (pull from METAF_t, collide, round as the store would) and keeps
+// the values in registers; with rows r-2, r-1, r in hand it forms step t+2
+// of row r-1, pulling the vertical directions from its own kept rows and the
+// diagonal and horizontal ones from lanes l-1 / l+1 by lane shuffles, and
+// stores it.  The two side columns x0-1 and x0+64 are formed at t+1 in
+// batches of 32 rows, two lanes per row, and lanes 0 and 63 read them by
+// readlane; rows y0-1 and y0+rows are formed at t+1 and not stored.
+// `walls` is set when any lane of the tile holds a class-4 node.
+// cls carries bit 7 on the nodes the tiles advance: interior (class 1) or
+// in-tile wall (class 4) nodes whose fluid neighbours are all of class 1 or
+// 4 too, so that every t+1 value they pull is formed here without a
+// boundary-list record.  Lanes of other classes form nothing (zeros that no
+// advanced node reads).  The remaining nodes take two single steps through a
+// third buffer in separate launches.  The arithmetic per cell is k_step's
+// (pull_cell, collide_bulk, the store's rounding), so the result equals two
+// single steps bit for bit.
+template <class S, int COLL>
+__global__ __launch_bounds__(64) void k_two_steps(const typename S::T* __restrict__ src,
+                                                  typename S::T* __restrict__ dst,
+                                                  const uint8_t* __restrict__ cls,
+                                                  const int4* __restrict__ tiles, KParams p) {
+  const int4 tl = tiles[blockIdx.x];  // x0, y0, rows, walls
+  if (tl.z <= 0) return;
+  if (tl.w)
+    two_steps_tile<S, COLL, true>(src, dst, cls, tl, p);
+  else
+    two_steps_tile<S, COLL, false>(src, dst, cls, tl, p);
+}
+
 // Plain steps with FP16 storage: the bulk workgroups take PAIRS of tiles
 // (2 x 256 consecutive cells) and each thread two neighbouring cells, so the
 // loads and stores of all-interior pairs move one 32-bit word per two cells:

@@ -19,12 +19,16 @@ struct Launch {
   int nb_bulk = 0;             // bulk grid (multiple of 8)
   int block = 256;             // bulk workgroup size
   bool pairs = false;          // FP16 plain steps: paired-cell kernel (k_step_pair)
+  const int4* tiles = nullptr; // two-step tiles (x0, y0, rows, 0), one wave each
+  int ntiles = 0;
   KParams kp{};
   KOut out{};
   hipStream_t stream = nullptr;
 };
 
-enum Pass { PASS_BULK = 0, PASS_BOUNDARY = 1, PASS_OUTLET = 2, PASS_FUSED = 3 };
+// PASS_TWO: two plain steps of the two-step tiles (MODE_STEP, not with
+// stochastic rounding).
+enum Pass { PASS_BULK = 0, PASS_BOUNDARY = 1, PASS_OUTLET = 2, PASS_FUSED = 3, PASS_TWO = 4 };
 enum Mode { MODE_STEP = 0, MODE_RESID = 1, MODE_OBSERVE = 2 };
 
 // prec: 0 F64, 1 F32, 2 F16.  coll: 1 BGKW, 2 TRT, 3 MRT.
@@ -39,6 +43,9 @@ hipError_t launch_fill_solid_f64(const Launch& L, void* buf);
 hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& L);
 hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L);
 hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf);
+// Resident waves of the two-step kernel per CU (0: no such kernel).
+int two_step_occupancy_f64(int coll);
+int two_step_occupancy_lp(int prec, int coll);
 
 // Deterministic FP64 sum of nparts x 6 partials -> out[6].
 hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st);

@@ -62,6 +62,13 @@ static hipError_t pass_mode(int pass, const Launch& L) {
       return hipGetLastError();
     case PASS_FUSED:
       return fused_launch<S, COLL, MODE>(L);
+    case PASS_TWO:
+      if constexpr (MODE == 0 && !S::SR) {
+        if (L.ntiles <= 0) return hipSuccess;
+        k_two_steps<S, COLL><<<L.ntiles, 64, 0, L.stream>>>(src, dst, L.cls, L.tiles, L.kp);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
     case PASS_OUTLET:
       if (L.norows <= 0) return hipSuccess;
       k_outlet<S, COLL, MODE><<<(L.norows + 63) / 64, 64, 0, L.stream>>>(
@@ -79,6 +86,31 @@ static hipError_t pass_impl(int mode, int pass, const Launch& L) {
     case MODE_OBSERVE: return pass_mode<S, COLL, 2>(pass, L);
   }
   return hipErrorInvalidValue;
+}
+
+// Resident waves of k_two_steps per CU (one wave per workgroup); 0 where the
+// kernel does not exist.
+template <class S, int COLL>
+static int two_step_occupancy_impl() {
+  if constexpr (S::SR) {
+    return 0;
+  } else {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_two_steps<S, COLL>, 64, 0) !=
+        hipSuccess)
+      return 0;
+    return nb;
+  }
+}
+
+template <class S>
+static int two_step_occupancy_coll(int coll) {
+  switch (coll) {
+    case 1: return two_step_occupancy_impl<S, 1>();
+    case 2: return two_step_occupancy_impl<S, 2>();
+    case 3: return two_step_occupancy_impl<S, 3>();
+  }
+  return 0;
 }
 
 template <class S, int COLL>

@@ -31,6 +31,8 @@ using namespace lbmhost;
     if (_rc != LBM_OK) return _rc;             \
   } while (0)
 
+static int launch_step_passes(lbm_solver* s, const Launch& L);
+
 namespace {
 
 constexpr int HCAP = 1024;  // device history slots
@@ -171,9 +173,28 @@ int flush_history(lbm_solver* s) {
   return s->diverged ? LBM_DIVERGED : LBM_OK;
 }
 
+// After a two-step launch the state one step back (METAF_t, what observe
+// passes pull from) existed only inside the launch: form it again from the
+// state two steps back with one plain step into buf3 (the same arithmetic, so
+// the same bits) and put it in buf[cur^1].
+int refresh_previous(lbm_solver* s) {
+  if (!s->prev_stale) return LBM_OK;
+  Launch L = make_launch(s, s->cur ^ 1);
+  L.kp.res_kind = RK_NONE;
+  L.dst = s->buf3;
+  int rc = launch_step_passes(s, L);
+  if (rc) return rc;
+  HC(hipMemcpyAsync(s->buf[s->cur ^ 1], s->buf3, (size_t)9 * s->plane * s->elem,
+                    hipMemcpyDeviceToDevice, s->stream));
+  s->prev_stale = false;
+  return LBM_OK;
+}
+
 int observe(lbm_solver* s, bool with_f) {
   if (s->iter == 0) return LBM_OK;
   if (s->observed == s->iter && (!with_f || s->observed_f == s->iter)) return LBM_OK;
+  int rc0 = refresh_previous(s);
+  if (rc0) return rc0;
   if (with_f && !s->d_f) {
     int rc = dev_alloc(s, (void**)&s->d_f, sizeof(double) * 9 * (size_t)s->plane);
     if (rc) return rc;
@@ -207,6 +228,149 @@ void depad(const lbm_solver* s, const T* padded, T* out, long plane_off) {
 
 extern "C" {
 static int rebuild_tile_list(lbm_solver* s);
+
+// ---- two plain steps per launch
+// Default output rows per two-step tile: as few tiles as fill the GPU once
+// (every CU holding as many waves of the kernel as fit), so no tile waits
+// for a second round, but at least TWO_STEP_MIN_ROWS (the two ring rows
+// cost a step-1 row each).
+constexpr int TWO_STEP_MIN_ROWS = 8;
+static int two_step_rows(const lbm_solver* s, int nstrip) {
+  const int per_cu = s->prec == 0 ? two_step_occupancy_f64(s->coll)
+                                  : two_step_occupancy_lp(s->prec, s->coll);
+  hipDeviceProp_t prop;
+  int cus = 0;
+  if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) cus = prop.multiProcessorCount;
+  const long long waves = (long long)std::max(per_cu, 1) * std::max(cus, 1);
+  const long long segs = std::max(1LL, waves / std::max(nstrip, 1));
+  const int m = s->cs->m;
+  return std::max(TWO_STEP_MIN_ROWS, (int)((m + segs - 1) / segs));
+}
+
+// Tiles of k_two_steps and the band's launches.  A tile is a strip of 64
+// padded columns (64 s .. 64 s + 63) and a run of at most `rows` output rows.
+// A node is the tile's to advance (bit 7 of its class byte) when it is an
+// interior or in-tile wall node (class 1 or 4) and so are all its fluid
+// neighbours (class 0 = no node): the t+1 values it pulls can then be formed
+// in the tile without a boundary-list record.  The band (every other node:
+// boundary-list and outlet nodes and their neighbours) takes two single
+// steps: the first advances the band and its one-cell ring from buf[cur]
+// into buf3 (the ring's values at t+1 are what the band's second step
+// pulls), the second the band from buf3 into buf[cur^1].  Tiles are sorted
+// by row, then column, and dealt to the 8 XCDs in contiguous runs (launch
+// slot w goes to XCD w % 8), so the tiles that share edge lines meet in one
+// L2.
+static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS, int align,
+                          long long active) {
+  const lbm_options* o = &s->opt;
+  const int n = s->cs->n, m = s->cs->m, pitch = s->pitch;
+  const int R = o->two_step_rows > 0 ? o->two_step_rows : two_step_rows(s, n / 64 + 1);
+  auto ok = [](uint8_t c) { return c == 0 || c == 1 || c == 4; };
+  auto bulk = [](uint8_t c) { return c == 1 || c == 4; };
+  std::vector<uint8_t> cls2(cls);
+  long long cells = 0;
+  for (int r = 1; r <= m; r++)
+    for (int x = 1; x <= n; x++) {
+      const size_t idx = (size_t)r * pitch + x;
+      if (!bulk(cls[idx])) continue;
+      bool g = true;
+      for (int dy = -1; dy <= 1 && g; dy++)
+        for (int dx = -1; dx <= 1 && g; dx++) g = ok(cls[idx + (long)dy * pitch + dx]);
+      if (g) {
+        cls2[idx] |= 0x80;
+        cells++;
+      }
+    }
+  std::vector<int4> tiles;
+  for (int x0 = 0; x0 <= n; x0 += 64) {
+    const int o1 = std::min(x0 + 64, pitch);  // one past the last output column
+    for (int y = 1; y <= m; y += R) {
+      const int y1 = std::min(y + R - 1, m);
+      bool any = false, walls = false;
+      for (int r = y; r <= y1 && !any; r++)
+        for (int x = x0; x < o1 && !any; x++) any = cls2[(size_t)r * pitch + x] & 0x80;
+      // class-4 nodes anywhere in columns x0-1 .. x0+64 of rows y-1 .. y1+1
+      for (int r = y - 1; r <= y1 + 1 && any && !walls; r++)
+        for (int x = std::max(x0 - 1, 0); x <= std::min(x0 + 64, pitch - 1) && !walls; x++)
+          walls = cls[(size_t)r * pitch + x] == 4;
+      if (any) tiles.push_back(make_int4(x0, y, y1 - y + 1, walls ? 1 : 0));
+    }
+  }
+  std::vector<uint8_t> covered((size_t)s->plane, 0);
+  for (size_t i = 0; i < cls2.size(); i++) covered[i] = (cls2[i] & 0x80) != 0;
+  // By default on FP64 / FP32 lattices of at least 2^22 cells (at 1025^2 a
+  // step is on its latency floor and the band's two extra launches cost
+  // more than the halved traffic saves; FP16's paired-cell single step moves
+  // its bytes faster than the one-cell two-step kernel).
+  const bool want = o->two_step_rows > 0 ||
+                    ((s->prec == LBM_PREC_F64 || s->prec == LBM_PREC_F32) &&
+                     (long long)n * m >= (1LL << 22) && 3 * cells >= 2 * active);
+  if (tiles.empty() || !want) return LBM_OK;
+  std::stable_sort(tiles.begin(), tiles.end(), [](const int4& a, const int4& b) {
+    return a.y != b.y ? a.y < b.y : a.x < b.x;
+  });
+  const int nt = (int)tiles.size();
+  const int q = (nt + 7) / 8;  // tiles per XCD
+  std::vector<int4> order((size_t)8 * q, make_int4(0, 0, 0, 0));
+  for (int x = 0; x < 8; x++)
+    for (int k = 0; k < q; k++)
+      if (x * q + k < nt) order[(size_t)8 * k + x] = tiles[(size_t)x * q + k];
+  // the band, and the band with its ring
+  std::vector<uint8_t> band((size_t)s->plane, 0), ring((size_t)s->plane, 0);
+  long long nband = 0;
+  for (int r = 1; r <= m; r++)
+    for (int x = 1; x <= n; x++) {
+      const size_t idx = (size_t)r * pitch + x;
+      if (cls[idx] == 0 || covered[idx]) continue;
+      band[idx] = 1;
+      nband++;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) ring[idx + (long)dy * pitch + dx] = 1;
+    }
+  std::vector<uint8_t> cb[2];
+  std::vector<uint8_t> tf[2];
+  std::vector<uint32_t> list[2];
+  for (int b = 0; b < 2; b++) {
+    const std::vector<uint8_t>& in = b == 0 ? ring : band;
+    cb[b].assign((size_t)s->plane, 0);
+    for (size_t i = 0; i < cb[b].size(); i++)
+      if (in[i] && bulk(cls[i])) cb[b][i] = cls[i];
+    tf[b].assign((size_t)s->nb_bulk, 0);
+    for (int t = 0; t < s->nb_bulk; t++) {
+      for (int e = 0; e < BS; e++) {
+        const long idx = (long)s->kp.bulk_begin + (long)t * BS + e;
+        if (idx < s->kp.bulk_end && cb[b][idx]) {
+          tf[b][t] = 2;
+          break;
+        }
+      }
+      if (tf[b][t]) list[b].push_back((uint32_t)t);
+    }
+    while (list[b].empty() || list[b].size() % align) list[b].push_back(0xffffffffu);
+  }
+  int rc = dev_alloc(s, (void**)&s->d_tiles, sizeof(int4) * order.size());
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_cls_two, cls2.size());
+  if (!rc) rc = check_hip(hipMemcpy(s->d_cls_two, cls2.data(), cls2.size(), hipMemcpyHostToDevice), "upload two-step classes");
+  if (!rc) rc = dev_alloc(s, &s->buf3, (size_t)9 * s->plane * s->elem + 256);
+  for (int b = 0; b < 2 && !rc; b++) {
+    rc = dev_alloc(s, (void**)&s->d_cls_b[b], cb[b].size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_tflag_b[b], tf[b].size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_list_b[b], sizeof(uint32_t) * list[b].size());
+    if (!rc) rc = check_hip(hipMemcpy(s->d_cls_b[b], cb[b].data(), cb[b].size(), hipMemcpyHostToDevice), "upload band classes");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_tflag_b[b], tf[b].data(), tf[b].size(), hipMemcpyHostToDevice), "upload band tile flags");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_list_b[b], list[b].data(), sizeof(uint32_t) * list[b].size(), hipMemcpyHostToDevice), "upload band tiles");
+    s->nlist_b[b] = (int)list[b].size();
+  }
+  if (!rc) rc = check_hip(hipMemcpy(s->d_tiles, order.data(), sizeof(int4) * order.size(), hipMemcpyHostToDevice), "upload two-step tiles");
+  if (rc) return rc;
+  s->two = true;
+  s->two_rows = R;
+  s->ntiles = (int)order.size();
+  s->ntiles_real = nt;
+  s->two_cells = cells;
+  s->band_cells = nband;
+  return LBM_OK;
+}
 }
 
 static int solver_create(const lbm_case* c, const lbm_options* o, const double* frho,
@@ -683,6 +847,17 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (ninst > 0) {
     rc = rebuild_tile_list(s);
     if (rc) {
+      if (d_grad) (void)hipFree(d_grad);
+      lbm_solver_destroy(s);
+      return rc;
+    }
+  }
+  if (ninst == 0 && s->prec != LBM_PREC_F16_SR && !(o->tune_flags & LBM_TUNE_NO_TWO_STEP)) {
+    long long active = 0;
+    for (uint8_t cv : cls) active += cv != 0;
+    rc = setup_two_step(s, cls, BS, align, active);
+    if (rc) {
+      if (d_grad) (void)hipFree(d_grad);
       lbm_solver_destroy(s);
       return rc;
     }
@@ -692,6 +867,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   L.out.grad = d_grad;
   hipError_t e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[0]) : launch_fill_solid_lp(s->prec, L, s->buf[0]);
   if (e == hipSuccess) e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[1]) : launch_fill_solid_lp(s->prec, L, s->buf[1]);
+  if (e == hipSuccess && s->buf3)
+    e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf3) : launch_fill_solid_lp(s->prec, L, s->buf3);
   if (e == hipSuccess) e = s->prec == 0 ? launch_init_f64(s->coll, s->f0_kind, L) : launch_init_lp(s->prec, s->coll, s->f0_kind, L);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (d_grad) (void)hipFree(d_grad);
@@ -712,6 +889,11 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
 static int launch_plain_step(lbm_solver* s, int src_buf) {
   Launch L = make_launch(s, src_buf);
   L.kp.res_kind = RK_NONE;
+  return launch_step_passes(s, L);
+}
+
+// One step of the launches in L (bulk tiles + boundary list, then outlets).
+static int launch_step_passes(lbm_solver* s, const Launch& L) {
   if (s->fused) {
     HC(pass(s, MODE_STEP, PASS_FUSED, L));
   } else {
@@ -720,6 +902,55 @@ static int launch_plain_step(lbm_solver* s, int src_buf) {
   }
   HC(pass(s, MODE_STEP, PASS_OUTLET, L));
   return LBM_OK;
+}
+
+// Two plain steps from buf[src_buf] into buf[src_buf ^ 1]: the two-step
+// tiles (between events e0 and e1 when given), then the band's two single
+// steps through buf3.  The tiles read only buf[src_buf] and write only their
+// own nodes, so the order of the three launches is free.
+static int launch_two_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
+                            hipEvent_t e1 = nullptr) {
+  Launch L = make_launch(s, src_buf);
+  L.kp.res_kind = RK_NONE;
+  L.pairs = false;
+  L.tiles = s->d_tiles;
+  L.ntiles = s->ntiles;
+  {
+    Launch T = L;
+    T.cls = s->d_cls_two;
+    if (e0) HC(hipEventRecord(e0, s->stream));
+    HC(pass(s, MODE_STEP, PASS_TWO, T));
+  }
+  if (e1) HC(hipEventRecord(e1, s->stream));
+  for (int b = 0; b < 2; b++) {
+    Launch B = L;
+    B.src = b == 0 ? s->buf[src_buf] : s->buf3;
+    B.dst = b == 0 ? s->buf3 : s->buf[src_buf ^ 1];
+    B.cls = s->d_cls_b[b];
+    B.tflag = s->d_tflag_b[b];
+    B.kp.tile_list = s->d_list_b[b];
+    B.nb_bulk = s->nlist_b[b];
+    int rc = launch_step_passes(s, B);
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+// The launches of launch_two_steps(parity) as a graph: after a replay the
+// state is in buf[parity ^ 1] two iterations later.
+static int build_graph2(lbm_solver* s, int parity) {
+  hipGraph_t g = nullptr;
+  HC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_two_steps(s, parity);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc != LBM_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HC(e);
+  e = hipGraphInstantiate(&s->gexec2[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return check_hip(e, "hipGraphInstantiate");
 }
 
 // Capture steps (parity, parity^1): after replay the state is back in
@@ -784,11 +1015,14 @@ void lbm_solver_destroy(lbm_solver* s) {
                   s->d_rho, s->d_u, s->d_v, s->d_f, s->d_metaf, s->d_snap, s->d_cres,
                   s->d_part, s->d_hist, s->d_inst_par, s->d_tile_inst, s->d_node_inst,
                   s->d_inst_active, s->d_inst_rect, s->d_inst_res, s->d_tile_list,
-                  s->d_wmask, s->d_seed};
+                  s->d_wmask, s->d_seed, s->d_tiles, s->d_cls_two, s->buf3, s->d_cls_b[0], s->d_cls_b[1],
+                  s->d_tflag_b[0], s->d_tflag_b[1], s->d_list_b[0], s->d_list_b[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   for (hipGraphExec_t g : s->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (hipGraphExec_t g : s->gexec2)
     if (g) (void)hipGraphExecDestroy(g);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -887,8 +1121,10 @@ static int batch_eval(lbm_solver* s) {
   return LBM_OK;
 }
 
+// evs: timing events, two per sample; rec receives the event-pair index of
+// every sample taken.
 static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_t>* evs,
-                     int sample_every = 1) {
+                     int sample_every = 1, std::vector<size_t>* rec = nullptr) {
   if (s->ninst && s->t_first_step < 0) s->t_first_step = wall_seconds();
   if (done) *done = 0;
   if (s->invalid) {
@@ -924,11 +1160,37 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     };
     const bool feval = forces_at(s->iter);
     const bool eval = reval || feval;
+    const bool plain_pair = t + 1 < nsteps && !eval && !evals_at(s->iter + 1) &&
+                            !forces_at(s->iter + 1) && !stop_can_trigger &&
+                            (!o.use_stop_rule || s->iter + 2 <= o.max_iterations);
+    const bool sample_pair = evs && (t % sample_every == 0 || (t + 1) % sample_every == 0);
+    // Two plain steps in a row with two-step launches: one pass over the state.
+    if (s->two && plain_pair) {
+      if (sample_pair) {
+        const size_t ei = 2 * (size_t)((t % sample_every == 0 ? t : t + 1) / sample_every);
+        int rc = launch_two_steps(s, s->cur, (*evs)[ei], (*evs)[ei + 1]);
+        if (rc) return rc;
+        if (rec) rec->push_back(ei);
+      } else if (s->use_graph) {
+        if (!s->gexec2[s->cur]) {
+          int rc = build_graph2(s, s->cur);
+          if (rc) return rc;
+        }
+        HC(hipGraphLaunch(s->gexec2[s->cur], s->stream));
+      } else {
+        int rc = launch_two_steps(s, s->cur);
+        if (rc) return rc;
+      }
+      s->cur ^= 1;
+      s->iter += 2;
+      ran += 2;
+      t += 1;
+      s->prev_stale = true;
+      continue;
+    }
+    s->prev_stale = false;  // the launches below leave the previous state in buf[cur^1]
     // Two plain steps in a row: replay the captured graph.
-    if (s->use_graph && t + 1 < nsteps && !eval && !evals_at(s->iter + 1) &&
-        !forces_at(s->iter + 1) && !stop_can_trigger &&
-        (!o.use_stop_rule || s->iter + 2 <= o.max_iterations) &&
-        !(evs && (t % sample_every == 0 || (t + 1) % sample_every == 0))) {
+    if (s->use_graph && plain_pair && !sample_pair) {
       if (!s->gexec[s->cur]) {
         int rc = build_graph(s, s->cur);
         if (rc) return rc;
@@ -945,9 +1207,11 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     L.kp.res_kind = rpart ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
     L.kp.force_on = feval ? 1 : 0;
     if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
-    const bool timed = evs && (t % sample_every == 0);
+    // with two-step launches only those are sampled
+    const bool timed = evs && !s->two && (t % sample_every == 0);
     const size_t ei = 2 * (size_t)(t / (sample_every > 0 ? sample_every : 1));
     if (timed) HC(hipEventRecord((*evs)[ei], s->stream));
+    if (timed && rec) rec->push_back(ei);
     if (s->fused) {
       HC(pass(s, mode, PASS_FUSED, L));  // bulk tiles + boundary-list nodes, one launch
       if (timed) HC(hipEventRecord((*evs)[ei + 1], s->stream));
@@ -1010,16 +1274,16 @@ int lbm_solver_step_timed(lbm_solver* s, int nsteps, int sample_every, double* t
   hipEvent_t e0 = s->ev[2 * nsamp], e1 = s->ev[2 * nsamp + 1];
   HC(hipEventRecord(e0, s->stream));
   int done = 0;
-  int rc = step_impl(s, nsteps, &done, &evs, sample_every);
+  std::vector<size_t> rec;
+  int rc = step_impl(s, nsteps, &done, &evs, sample_every, &rec);
   HC(hipEventRecord(e1, s->stream));
   HC(hipEventSynchronize(e1));
   float ms = 0.f;
   HC(hipEventElapsedTime(&ms, e0, e1));
   double bsum = 0.0;
   int nb = 0;
-  for (int t = 0; t < done; t += sample_every) {
+  for (size_t ei : rec) {
     float b = 0.f;
-    const size_t ei = 2 * (size_t)(t / sample_every);
     HC(hipEventElapsedTime(&b, evs[ei], evs[ei + 1]));
     bsum += b;
     nb++;
@@ -1158,7 +1422,18 @@ size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 
 
 int lbm_solver_fused(const lbm_solver* s) {
   if (!s) return 0;
-  return (s->fused ? 1 : 0) | (make_launch(const_cast<lbm_solver*>(s), 0).pairs ? 2 : 0);
+  return (s->fused ? 1 : 0) | (make_launch(const_cast<lbm_solver*>(s), 0).pairs ? 2 : 0) |
+         (s->two ? 4 : 0);
+}
+
+int lbm_solver_two_step_info(const lbm_solver* s, long long* cells, int* tiles, int* rows,
+                             long long* band) {
+  if (!s) return LBM_EINVAL;
+  if (cells) *cells = s->two ? s->two_cells : 0;
+  if (tiles) *tiles = s->two ? s->ntiles_real : 0;
+  if (rows) *rows = s->two ? s->two_rows : 0;
+  if (band) *band = s->two ? s->band_cells : 0;
+  return LBM_OK;
 }
 
 // ---------------------------------------------------------------- checkpoint
@@ -1244,6 +1519,8 @@ int lbm_solver_save(lbm_solver* s, const char* path) {
   }
   int rc = flush_history(s);
   if (rc && rc != LBM_DIVERGED) return rc;
+  rc = refresh_previous(s);  // both buffers as single steps leave them
+  if (rc) return rc;
   HC(hipStreamSynchronize(s->stream));
   FILE* f = std::fopen(path, "wb");
   if (!f) {
@@ -1335,6 +1612,7 @@ int lbm_solver_load(lbm_solver* s, const char* path) {
   s->invalid = false;
   s->iter = h.iter;
   s->cur = h.cur;
+  s->prev_stale = false;
   std::memcpy(s->resid, h.resid, sizeof h.resid);
   s->observed = s->observed_f = -1;
   s->pending = 0;

@@ -137,28 +137,36 @@ def run_single(spec, opts=None, chunk: int = 5000, fields: bool = False) -> dict
 
 
 def autotune(case: Case, opts, instances=None, steps: int = 200,
-             blocks=(256, 512), groups=(1, 2, 4, 8)) -> tuple[dict, list]:
-    """Time `steps` plain steps for each block size / XCD tile-group choice
-    on this lattice and return the fastest setting (and every timing).
-    Block 256 runs the fused step kernel; larger blocks split bulk and
-    boundary launches."""
+             blocks=(256, 512, 1024), groups=(1, 2, 4, 8, 16)) -> tuple[dict, list]:
+    """Time `steps` plain steps for each launch shape on this lattice and
+    return the fastest (and every timing).  A bulk tile is `block_size`
+    consecutive cells of a storage row (one workgroup); `xcd_group` runs of
+    that many adjacent tiles go to one XCD (its L2 serves the +-1 pulls the
+    tiles share); FP16 storage also tries one cell per thread against the
+    paired-cell kernel.  Block 256 runs the fused step kernel, larger blocks
+    split bulk and boundary launches.  Results are identical for every shape."""
+    from . import PREC_F16, PREC_F16_SR, TUNE_NO_PAIRS
+    fp16 = opts.precision in (PREC_F16, PREC_F16_SR)
     results = []
     for bs in blocks:
-        for g in (groups if bs == 256 else (1,)):
-            o = type(opts).from_buffer_copy(opts)
-            o.block_size, o.xcd_group = bs, g
-            o.use_stop_rule = 0
-            o.resid_kind = 3
-            o.crit_v = -1.0  # no instance stops while timing
-            s = Solver(case, o, instances=instances) if instances else Solver(case, o)
-            s.step(20)
-            s.sync()
-            t0 = time.perf_counter()
-            s.step(steps)
-            s.sync()
-            dt = (time.perf_counter() - t0) / steps
-            results.append(dict(block_size=bs, xcd_group=g, ms_per_step=dt * 1e3))
-            s.close()
+        for g in groups:
+            for pairs in ((True, False) if fp16 and bs == 256 else (True,)):
+                o = type(opts).from_buffer_copy(opts)
+                o.block_size, o.xcd_group = bs, g
+                o.tune_flags = (opts.tune_flags & ~TUNE_NO_PAIRS) | (0 if pairs else TUNE_NO_PAIRS)
+                o.use_stop_rule = 0
+                o.resid_kind = 3
+                o.crit_v = -1.0  # no instance stops while timing
+                s = Solver(case, o, instances=instances) if instances else Solver(case, o)
+                s.step(20)
+                s.sync()
+                t0 = time.perf_counter()
+                s.step(steps)
+                s.sync()
+                dt = (time.perf_counter() - t0) / steps
+                results.append(dict(block_size=bs, xcd_group=g, pairs=pairs and fp16 and bs == 256,
+                                    ms_per_step=dt * 1e3))
+                s.close()
     best = min(results, key=lambda r: r["ms_per_step"])
     return best, results
 

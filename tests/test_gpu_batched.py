@@ -306,3 +306,27 @@ def test_batched_instances_match_the_fp64_oracle(kind, tmp_path):
               f"max |drho| {drho:.2e}", flush=True)
         assert err < 2e-5, k
         assert drho < 1e-5, k
+
+
+@pytest.mark.parametrize("prec", [lbm.PREC_F32, lbm.PREC_F16])
+def test_batched_launch_shapes_are_bit_identical(prec):
+    """The sweep's tuner times tile lengths 256 / 512 / 1024 and XCD runs of
+    1..16 tiles on the packed lattice: every shape gives the same fields bit
+    for bit, and the same per-instance stop iterations."""
+    print("batched shapes", prec, flush=True)
+    cases = [lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in SET]
+    packed = lbm.Case.pack(cases)
+    inst = [(nu, u) for *_, nu, u in SET]
+    out = []
+    for bs, g in ((256, 4), (256, 1), (512, 2), (1024, 16)):
+        s = lbm.Solver(packed, batch_opts(prec, resid_every=10, crit_v=1e-3, use_stop_rule=1,
+                                          block_size=bs, xcd_group=g), instances=inst)
+        s.step(600)
+        it, _, _ = s.batch_status()
+        out.append((s.macro(), it, (bs, g)))
+        s.close()
+    assert (out[0][1] > 0).any(), "no instance stopped: the criterion is not exercised"
+    for fields, it, shape in out[1:]:
+        assert np.array_equal(it, out[0][1]), shape
+        for a, b in zip(out[0][0], fields):
+            assert np.array_equal(a, b), shape

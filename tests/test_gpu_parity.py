@@ -92,6 +92,29 @@ def test_fp64_bit_exact_on_medium(name):
     assert sha(fd) == str(g["f_sha256"])
 
 
+@pytest.mark.parametrize("name", medium_cases())
+def test_two_step_launches_bit_exact_on_medium(name):
+    """The same 1000 steps with every pair of plain steps as one two-step
+    launch (16-row tiles; residual evaluation off, so no step is left single
+    but the last of an odd count): the reference's u, v, rho and F bit for
+    bit."""
+    print("two-step medium", name, flush=True)
+    g = load(name)
+    nodes, conn = meshgen.cavity(int(g["N"])).arrays()
+    case, s = run_gpu(nodes, conn, opts_from_golden(g, resid_kind=lbm.RESID_NONE,
+                                                    two_step_rows=16), int(g["ini"][8]))
+    info = s.two_step_info()
+    assert info["cells"] > 0.9 * case.info.num_fluid, info
+    assert info["cells"] + info["band"] == case.info.num_fluid, info
+    rho, u, v = s.macro()
+    n, m = int(g["n"]), int(g["m"])
+    for f, a in (("u", u), ("v", v), ("rho", rho)):
+        assert np.array_equal(a.reshape(m, n)[g["rows"]], g[f"{f}_rows"]), f
+        assert sha(a) == str(g[f"{f}_sha256"]), f
+    fd, _ = s.distributions()
+    assert sha(fd) == str(g["f_sha256"])
+
+
 # ------------------------------------------------------- switches vs the oracle
 SWITCH_CASES = {
     # name: (mesh factory, oracle params, steps)
@@ -619,3 +642,115 @@ def test_stochastic_rounding_is_unbiased_and_reproducible():
                 se = 3 * gap * np.sqrt(9 / sr.size)
             print(k, n_steps, "mean - exact", sr.mean() - scaled, "bound", 4 * se, flush=True)
             assert abs(sr.mean() - scaled) <= 4 * se + 1e-9, (k, n_steps)
+
+
+# (block_size, xcd_group, tune_flags); the first is the default shape
+LAUNCH_SHAPES = [(256, 4, 0), (256, 1, 0), (256, 16, 0), (512, 2, 0), (1024, 8, 0),
+                 (256, 4, lbm.TUNE_SPLIT_BOUNDARY), (256, 4, lbm.TUNE_XCD_REMAP),
+                 (512, 1, lbm.TUNE_XCD_REMAP)]
+
+
+@pytest.mark.parametrize("prec,coll", [(lbm.PREC_F64, lbm.BGKW), (lbm.PREC_F32, lbm.MRT),
+                                       (lbm.PREC_F16_SR, lbm.TRT)])
+def test_launch_shapes_are_bit_identical(prec, coll):
+    """Every launch shape the tuner picks from (tile length 256 / 512 / 1024
+    cells, runs of 1..16 tiles per XCD, the hardware's round-robin order,
+    fused or split boundary launches) advances the same cells by the same
+    arithmetic: same fields and distributions bit for bit, on a cavity and on
+    a channel whose cylinder makes mixed tiles, through residual and observe
+    steps.  The residuals sum per tile, so their order follows the tile
+    length: equal to 1e-12."""
+    print("launch shapes", prec, coll, flush=True)
+    for case in (lbm.Case.cavity(150), lbm.Case.channel(300, 90, 80.2, 45.3, 12.5)):
+        out = []
+        for bs, g, flags in LAUNCH_SHAPES:
+            o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
+                                 precision=prec, init=lbm.INIT_REST, curved=1,
+                                 resid_kind=lbm.RESID_FULL, resid_every=5, trt_lambda=0.25,
+                                 block_size=bs, xcd_group=g, tune_flags=flags)
+            s = lbm.Solver(case, o)
+            s.step(41)
+            f, mf = s.distributions()
+            out.append((s.macro(), f, s.residuals()))
+            s.close()
+        for k, (bs, g, flags) in enumerate(LAUNCH_SHAPES[1:], 1):
+            for a, b in zip(out[0][0], out[k][0]):
+                assert np.array_equal(a, b), (bs, g, flags)
+            assert np.array_equal(out[0][1], out[k][1]), (bs, g, flags)
+            assert np.allclose(out[0][2], out[k][2], rtol=1e-12, atol=0), (bs, g, flags)
+
+
+TWO_STEP_GEOMS = {
+    # name: (case factory, options)
+    "cavity": (lambda: lbm.Case.cavity(150), dict(curved=2)),
+    "cylinder_bouzidi_open_outlet": (lambda: lbm.Case.channel(300, 90, 80.2, 45.3, 12.5),
+                                     dict(curved=1, inlet_profile=1, outlet_profile=2)),
+    "cylinder_walls_pressure_outlet": (lambda: lbm.Case.channel(300, 90, 80.2, 45.3, 12.5),
+                                       dict(curved=2, inlet_profile=1, outlet_profile=4)),
+    "obstacle_velocity_outlet": (lambda: lbm.Case.channel(200, 70, 60.0, 30.0, 9.0),
+                                 dict(curved=2, inlet_profile=2, outlet_profile=1)),
+}
+
+
+@pytest.mark.parametrize("geom", sorted(TWO_STEP_GEOMS))
+@pytest.mark.parametrize("prec,coll", [(lbm.PREC_F64, lbm.BGKW), (lbm.PREC_F64, lbm.MRT),
+                                       (lbm.PREC_F32, lbm.MRT), (lbm.PREC_F32, lbm.TRT),
+                                       (lbm.PREC_F16, lbm.BGKW), (lbm.PREC_F16, lbm.MRT)])
+def test_two_step_launches_are_bit_identical(prec, coll, geom):
+    """Pairs of plain steps as two-step launches (tiles of 1, 5 and 16 rows)
+    against every step its own launch: same fields, distributions and
+    residual history bit for bit, with residual steps in between (every 7th,
+    so pairs and single steps alternate), eager and graph-replayed."""
+    print("two-step", prec, coll, geom, flush=True)
+    make, extra = TWO_STEP_GEOMS[geom]
+    case = make()
+    runs = []
+    for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (16, 0), (5, 0), (1, 0),
+                        (5, lbm.TUNE_NO_GRAPH)):
+        o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
+                             precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_FULL,
+                             resid_every=7, trt_lambda=0.25, two_step_rows=rows,
+                             tune_flags=flags, **extra)
+        s = lbm.Solver(case, o)
+        assert (s.two_step_info()["cells"] > 0) == (flags & lbm.TUNE_NO_TWO_STEP == 0)
+        for chunk in (1, 20, 2, 38):
+            s.step(chunk)
+        f, mf = s.distributions()
+        its, vals = s.residual_history()
+        runs.append((s.macro(), f, mf, its, vals, (rows, flags)))
+        s.close()
+    base = runs[0]
+    for r in runs[1:]:
+        for a, b in zip(base[0], r[0]):
+            assert np.array_equal(a, b), r[5]
+        assert np.array_equal(base[1], r[1]) and np.array_equal(base[2], r[2]), r[5]
+        assert np.array_equal(base[3], r[3]) and np.array_equal(base[4], r[4]), r[5]
+
+
+@pytest.mark.parametrize("prec,default_on", [(lbm.PREC_F32, True), (lbm.PREC_F64, True),
+                                             (lbm.PREC_F16, False)])
+def test_two_step_default_on_2049(prec, default_on):
+    """At 2049^2 two-step launches are on by default for FP32 and FP64 (FP16
+    keeps its paired-cell single step unless asked); 301 plain steps with
+    them (default tiles, or 30-row tiles for FP16) equal 301 single steps bit
+    for bit."""
+    print("two-step 2049", prec, flush=True)
+    case = lbm.Case.cavity(2049)
+    out = []
+    for flags in (0, lbm.TUNE_NO_TWO_STEP):
+        o = lbm.make_options(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.1,
+                             precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE,
+                             tune_flags=flags)
+        s = lbm.Solver(case, o)
+        assert (s.two_step_info()["cells"] > 0) == (flags == 0 and default_on)
+        if flags == 0 and not default_on:
+            s.close()
+            o.two_step_rows = 30
+            s = lbm.Solver(case, o)
+        info = s.two_step_info()
+        assert (info["cells"] > 0.99 * case.info.num_fluid) == (flags == 0), info
+        s.step(301)
+        out.append(s.macro())
+        s.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
