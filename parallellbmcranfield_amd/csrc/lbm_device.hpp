@@ -61,6 +61,17 @@ __device__ inline T ld(const T* p) {
     return *p;
 }
 
+// Word i of direction plane k of a state buffer.  The plane's base is the
+// same in every lane (scalar registers) and the in-plane byte offset fits in
+// 32 bits (a plane holds fewer than 2^31 / 9 words), so the access takes the
+// scalar-base plus 32-bit vector-offset form, without 64-bit address
+// arithmetic per lane.
+template <class T>
+__device__ inline T* plane_word(T* base, int plane, int k, int i) {
+  T* pk = base + (size_t)k * (size_t)plane;
+  return (T*)((char*)pk + (uint32_t)i * (uint32_t)sizeof(T));
+}
+
 // store() writes a step's result; store_rn() writes fixed values (the solid
 // value, set-up fills) rounded to nearest in every storage kind.
 struct StF64 {
@@ -74,7 +85,7 @@ struct StF64 {
   __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
   __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) store(dst + k * kp.plane + idx, v[k], kp, 0u);
+    for (int k = 0; k < 9; k++) store(plane_word(dst, kp.plane, k, idx), v[k], kp, 0u);
   }
 };
 struct StF32 {
@@ -88,7 +99,7 @@ struct StF32 {
   __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
   __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) store(dst + k * kp.plane + idx, v[k], kp, 0u);
+    for (int k = 0; k < 9; k++) store(plane_word(dst, kp.plane, k, idx), v[k], kp, 0u);
   }
 };
 struct StF16 {
@@ -107,7 +118,7 @@ struct StF16 {
   __device__ static void store_rn(T* p, R v, const KParams& kp) { store(p, v, kp, 0u); }
   __device__ static void store_cell(T* dst, int idx, const R (&v)[9], const KParams& kp, uint32_t) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) store(dst + k * kp.plane + idx, v[k], kp, 0u);
+    for (int k = 0; k < 9; k++) store(plane_word(dst, kp.plane, k, idx), v[k], kp, 0u);
   }
 };
 
@@ -164,7 +175,7 @@ struct StF16S : StF16 {
     for (int k = 0; k < 9; k++) {
       half2_t x = {(_Float16)0, (_Float16)0};
       x = f16_stochastic_into(x, v[k] * kp.scale_f, second ? rot16(r) : r, false);
-      __builtin_nontemporal_store(x[0], dst + k * kp.plane + idx);
+      __builtin_nontemporal_store(x[0], plane_word(dst, kp.plane, k, idx));
       r = xorshift32(r);
     }
   }

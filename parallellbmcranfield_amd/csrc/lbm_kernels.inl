@@ -449,7 +449,7 @@ __device__ inline void issue_pulls(typename S::R F[9], const typename S::T* src,
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     const int off = ((m >> k) & 1) ? -CX[k] - CY[k] * p.pitch : 0;
-    F[k] = S::load(src + k * p.plane + idx + off, p);
+    F[k] = S::load(plane_word(src, p.plane, k, idx + off), p);
   }
 }
 
@@ -501,56 +501,68 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
   const int xc = xin ? x : p.pitch - 1;  // loads of lanes past the row stay in it
   const int nit = tl.z + 2;              // step-t+1 rows y0-1 .. y0+rows
   const int r0 = tl.y - 1, rlast = r0 + nit - 1;
-  auto cls_at = [&](int r) -> int { return xin ? (int)cls[r * p.pitch + xc] : 0; };
-  auto mask_at = [&](int r) -> uint32_t { return WALLS ? p.wmask[r * p.pitch + xc] : 0u; };
+  auto cls_at = [&](int r) -> int { return xin ? (int)cls[(uint32_t)(r * p.pitch + xc)] : 0; };
+  auto mask_at = [&](int r) -> uint32_t {
+    return WALLS ? p.wmask[(uint32_t)(r * p.pitch + xc)] : 0u;
+  };
   // Side columns x0-1 and x0+64: lane j of a batch forms the cell of column
   // (j odd ? x0+64 : x0-1) in row base + j/2 at t+1 (zeros outside the grid
-  // or past row rlast).
+  // or past row rlast).  A batch's class bytes are fetched two rows before
+  // its use and its pulls one row before, ahead of that row's own loads, so
+  // forming it waits on nothing else in flight.
   R sd[9];
-  auto side_batch = [&](int base) {
+  R sq[9];
+  int sc = 0;
+  uint32_t sw = 0;
+  int sidx = 0;
+  auto side_classes = [&](int base) {
     const int xs = (lane & 1) ? tl.x + 64 : tl.x - 1;
     const int row = base + (lane >> 1);
     const bool ok = xs >= 0 && xs < p.pitch && row <= rlast;
-    const int idx = (ok ? row : r0) * p.pitch + (ok ? xs : xc);
-    const int c = ok ? (int)cls[idx] & 0x7f : 0;
-    const uint32_t w = (WALLS && ok) ? p.wmask[idx] : 0u;
-    R F[9];
-    issue_pulls<S>(F, src, p, idx, c, w);
-    form_t1<S, COLL, WALLS>(sd, F, c, w, p);
+    sidx = (ok ? row : r0) * p.pitch + (ok ? xs : xc);
+    sc = ok ? (int)cls[(uint32_t)sidx] & 0x7f : 0;
+    sw = (WALLS && ok) ? p.wmask[(uint32_t)sidx] : 0u;
   };
+  auto side_pulls = [&]() { issue_pulls<S>(sq, src, p, sidx, sc, sw); };
+  auto side_form = [&]() { form_t1<S, COLL, WALLS>(sd, sq, sc, sw, p); };
   int sbase = r0;
   R kd[3] = {0, 0, 0};                     // row rr-1 at t+1: directions 2, 5, 6
   R kc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // row rr at t+1
   int cc = 0;                              // class byte of (x, rr)
   uint32_t wc = 0;                         // masks of (x, rr)
-  // Software pipeline: while row r is formed, the class bytes (and masks)
-  // of rows r+1, r+2 and the pulls of row r+1 are in flight.
-  int cb1 = cls_at(r0), cb2 = cls_at(min(r0 + 1, rlast));
-  uint32_t w1 = mask_at(r0), w2 = mask_at(min(r0 + 1, rlast));
-  R Fq[9];
-  issue_pulls<S>(Fq, src, p, r0 * p.pitch + xc, cb1 & 0x7f, w1);
-  // one copy of the body: unrolled copies only grow the code
-#pragma unroll 1
-  for (int it = 0; it < nit; it++) {
+  // Software pipeline: while row r is formed, the pulls of rows r+1 and r+2
+  // (two buffers taking turns) and the class bytes (and masks) of rows up
+  // to r+3 are in flight.
+  int c0 = cls_at(r0), c1 = cls_at(min(r0 + 1, rlast)), c2 = cls_at(min(r0 + 2, rlast));
+  uint32_t w0 = mask_at(r0), w1 = mask_at(min(r0 + 1, rlast)), w2 = mask_at(min(r0 + 2, rlast));
+  R FA[9], FB[9];
+  issue_pulls<S>(FA, src, p, r0 * p.pitch + xc, c0 & 0x7f, w0);
+  issue_pulls<S>(FB, src, p, min(r0 + 1, rlast) * p.pitch + xc, c1 & 0x7f, w1);
+  // Row r = r0 + it from the buffer Fb that holds its pulls; Fb is refilled
+  // with the pulls of row r+2.
+  auto row = [&](int it, R (&Fb)[9]) {
     const int r = r0 + it;  // row of step t+1
-    const int cb = cb1;
-    const uint32_t w = w1;
-    R F[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) F[k] = Fq[k];
-    cb1 = cb2;
+    const int cb = c0;
+    const uint32_t w = w0;
+    // side batch b serves step-t+2 rows from it = 2 + 30 b: classes at
+    // it = 30 b, pulls at it = 1 + 30 b
+    if (it % (SIDE_ROWS - 2) == 0) side_classes(r);
+    if (it >= 1 && (it - 1) % (SIDE_ROWS - 2) == 0) side_pulls();
+    c0 = c1;
+    w0 = w1;
+    c1 = c2;
     w1 = w2;
-    const int rn = min(r + 1, rlast), rn2 = min(r + 2, rlast);
-    cb2 = cls_at(rn2);
-    w2 = mask_at(rn2);
-    issue_pulls<S>(Fq, src, p, rn * p.pitch + xc, cb1 & 0x7f, w1);
+    const int rn2 = min(r + 2, rlast), rn3 = min(r + 3, rlast);
+    c2 = cls_at(rn3);
+    w2 = mask_at(rn3);
     R kn[9];  // row r at t+1
-    form_t1<S, COLL, WALLS>(kn, F, cb & 0x7f, w, p);
+    form_t1<S, COLL, WALLS>(kn, Fb, cb & 0x7f, w, p);
+    issue_pulls<S>(Fb, src, p, rn2 * p.pitch + xc, c1 & 0x7f, w1);
     if (it >= 2) {
       const int rr = r - 1;  // row of step t+2
       if ((it - 2) % (SIDE_ROWS - 2) == 0) {
         sbase = rr - 1;
-        side_batch(sbase);
+        side_form();
       }
       const int jl = 2 * (rr - sbase);  // side lanes of row rr (left; right = +1)
       // F[k](x) = f_k(x - c_k) at t+1; lanes 0 and 63 take the side columns
@@ -600,6 +612,12 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     for (int k = 0; k < 9; k++) kc[k] = kn[k];
     cc = cb;
     wc = w;
+  };
+  // one copy of the two-row body: unrolled copies only grow the code
+#pragma unroll 1
+  for (int it = 0; it < nit; it += 2) {
+    row(it, FA);
+    if (it + 1 < nit) row(it + 1, FB);
   }
 }
 

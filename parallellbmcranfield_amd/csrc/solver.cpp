@@ -907,7 +907,9 @@ static int launch_step_passes(lbm_solver* s, const Launch& L) {
 // Two plain steps from buf[src_buf] into buf[src_buf ^ 1]: the two-step
 // tiles (between events e0 and e1 when given), then the band's two single
 // steps through buf3.  The tiles read only buf[src_buf] and write only their
-// own nodes, so the order of the three launches is free.
+// own nodes, so the order of the three launches is free.  (Running the band
+// on a second stream beside the tiles measured no faster: its workgroups
+// then share the CUs with the tiles.)
 static int launch_two_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
                             hipEvent_t e1 = nullptr) {
   Launch L = make_launch(s, src_buf);
@@ -920,8 +922,8 @@ static int launch_two_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
     T.cls = s->d_cls_two;
     if (e0) HC(hipEventRecord(e0, s->stream));
     HC(pass(s, MODE_STEP, PASS_TWO, T));
+    if (e1) HC(hipEventRecord(e1, s->stream));
   }
-  if (e1) HC(hipEventRecord(e1, s->stream));
   for (int b = 0; b < 2; b++) {
     Launch B = L;
     B.src = b == 0 ? s->buf[src_buf] : s->buf3;
