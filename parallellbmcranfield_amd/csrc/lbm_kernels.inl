@@ -483,10 +483,6 @@ __device__ inline void form_t1(typename S::R kn[9], typename S::R F[9], int c, u
   }
 }
 
-// Rows of the side-column batch: 32 cells of each side column, two lanes per
-// row; a batch serves the step-t+2 rows base+1 .. base+30.
-constexpr int SIDE_ROWS = 32;
-
 // One tile of k_two_steps; WALLS: the tile holds class-4 nodes (their stream
 // and wall masks are fetched with the class bytes).
 template <class S, int COLL, bool WALLS>
@@ -495,8 +491,8 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
                                       const uint8_t* __restrict__ cls, const int4 tl,
                                       const KParams& p) {
   using R = typename S::R;
-  const int lane = threadIdx.x;
-  const int x = tl.x + lane;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = tl.x + 64 * wave + lane;
   const bool xin = x < p.pitch;
   const int xc = xin ? x : p.pitch - 1;  // loads of lanes past the row stay in it
   const int nit = tl.z + 2;              // step-t+1 rows y0-1 .. y0+rows
@@ -505,31 +501,39 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
   auto mask_at = [&](int r) -> uint32_t {
     return WALLS ? p.wmask[(uint32_t)(r * p.pitch + xc)] : 0u;
   };
-  // Side columns x0-1 and x0+64: lane j of a batch forms the cell of column
-  // (j odd ? x0+64 : x0-1) in row base + j/2 at t+1 (zeros outside the grid
-  // or past row rlast).  A batch's class bytes are fetched two rows before
-  // its use and its pulls one row before, ahead of that row's own loads, so
-  // forming it waits on nothing else in flight.
+  // Edges between the waves: each row, lane 0 of every wave publishes the
+  // t+1 values its left neighbour pulls (directions 3, 6, 7) and lane 63 the
+  // ones its right neighbour pulls (1, 5, 8); two slots taking turns, one
+  // barrier per row.
+  __shared__ R edge[2][TWO_WAVES][2][3];
+  // The workgroup's outer columns x0-1 (wave 0) and x0+256 (last wave): lane
+  // j of a batch forms the cell of row base + j at t+1 (zeros outside the
+  // grid or past row rlast); a batch serves the step-t+2 rows base+1 ..
+  // base+62.  Its class bytes are fetched two rows before its use and its
+  // pulls one row before, ahead of that row's own loads, so forming it waits
+  // on nothing else in flight.
+  const bool side_owner = wave == 0 || wave == TWO_WAVES - 1;
+  const int xs = wave == 0 ? tl.x - 1 : tl.x + TWO_COLS;
   R sd[9];
   R sq[9];
   int sc = 0;
   uint32_t sw = 0;
   int sidx = 0;
   auto side_classes = [&](int base) {
-    const int xs = (lane & 1) ? tl.x + 64 : tl.x - 1;
-    const int row = base + (lane >> 1);
+    const int row = base + lane;
     const bool ok = xs >= 0 && xs < p.pitch && row <= rlast;
     sidx = (ok ? row : r0) * p.pitch + (ok ? xs : xc);
     sc = ok ? (int)cls[(uint32_t)sidx] & 0x7f : 0;
     sw = (WALLS && ok) ? p.wmask[(uint32_t)sidx] : 0u;
   };
-  auto side_pulls = [&]() { issue_pulls<S>(sq, src, p, sidx, sc, sw); };
-  auto side_form = [&]() { form_t1<S, COLL, WALLS>(sd, sq, sc, sw, p); };
   int sbase = r0;
   R kd[3] = {0, 0, 0};                     // row rr-1 at t+1: directions 2, 5, 6
   R kc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // row rr at t+1
   int cc = 0;                              // class byte of (x, rr)
   uint32_t wc = 0;                         // masks of (x, rr)
+  // neighbour columns' t+1 values: left (1, 5, 8) and right (3, 6, 7) of
+  // rows rr (1 / 3), rr-1 (5 / 6) and rr+1 (8 / 7)
+  R el1 = 0, el5 = 0, el5m = 0, er3 = 0, er6 = 0, er6m = 0;
   // Software pipeline: while row r is formed, the pulls of rows r+1 and r+2
   // (two buffers taking turns) and the class bytes (and masks) of rows up
   // to r+3 are in flight.
@@ -544,10 +548,12 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     const int r = r0 + it;  // row of step t+1
     const int cb = c0;
     const uint32_t w = w0;
-    // side batch b serves step-t+2 rows from it = 2 + 30 b: classes at
-    // it = 30 b, pulls at it = 1 + 30 b
-    if (it % (SIDE_ROWS - 2) == 0) side_classes(r);
-    if (it >= 1 && (it - 1) % (SIDE_ROWS - 2) == 0) side_pulls();
+    if (side_owner) {
+      // batch b serves step-t+2 rows from it = 2 + 62 b: classes at
+      // it = 62 b, pulls at it = 1 + 62 b
+      if (it % 62 == 0) side_classes(r);
+      if (it >= 1 && (it - 1) % 62 == 0) issue_pulls<S>(sq, src, p, sidx, sc, sw);
+    }
     c0 = c1;
     w0 = w1;
     c1 = c2;
@@ -558,14 +564,30 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     R kn[9];  // row r at t+1
     form_t1<S, COLL, WALLS>(kn, Fb, cb & 0x7f, w, p);
     issue_pulls<S>(Fb, src, p, rn2 * p.pitch + xc, c1 & 0x7f, w1);
+    // publish the edge lanes' row-r values, then take the neighbours'
+    const int slot = it & 1;
+    if (lane == 0) {
+      edge[slot][wave][0][0] = kn[3];
+      edge[slot][wave][0][1] = kn[6];
+      edge[slot][wave][0][2] = kn[7];
+    }
+    if (lane == 63) {
+      edge[slot][wave][1][0] = kn[1];
+      edge[slot][wave][1][1] = kn[5];
+      edge[slot][wave][1][2] = kn[8];
+    }
+    __syncthreads();
+    const int wl = wave > 0 ? wave - 1 : 0, wr = wave < TWO_WAVES - 1 ? wave + 1 : wave;
+    const R nl1 = edge[slot][wl][1][0], nl5 = edge[slot][wl][1][1], nl8 = edge[slot][wl][1][2];
+    const R nr3 = edge[slot][wr][0][0], nr6 = edge[slot][wr][0][1], nr7 = edge[slot][wr][0][2];
     if (it >= 2) {
       const int rr = r - 1;  // row of step t+2
-      if ((it - 2) % (SIDE_ROWS - 2) == 0) {
+      if (side_owner && (it - 2) % 62 == 0) {
         sbase = rr - 1;
-        side_form();
+        form_t1<S, COLL, WALLS>(sd, sq, sc, sw, p);
       }
-      const int jl = 2 * (rr - sbase);  // side lanes of row rr (left; right = +1)
-      // F[k](x) = f_k(x - c_k) at t+1; lanes 0 and 63 take the side columns
+      // F[k](x) = f_k(x - c_k) at t+1; lanes 0 and 63 take the neighbour
+      // columns: the next wave's edge lanes, or the batch at the outer edges
       R nb[9];
       nb[0] = kc[0];
       nb[1] = __shfl_up(kc[1], 1, 64);
@@ -576,9 +598,23 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
       nb[6] = __shfl_down(kd[2], 1, 64);
       nb[7] = __shfl_down(kn[7], 1, 64);
       nb[8] = __shfl_up(kn[8], 1, 64);
-      const R l1 = lane_value(sd[1], jl), l5 = lane_value(sd[5], jl - 2);
-      const R l8 = lane_value(sd[8], jl + 2), r3 = lane_value(sd[3], jl + 1);
-      const R r6 = lane_value(sd[6], jl - 1), r7 = lane_value(sd[7], jl + 3);
+      R l1 = el1, l5 = el5m, l8 = nl8, r3 = er3, r6 = er6m, r7 = nr7;
+      if (side_owner) {
+        const int j = rr - sbase;  // batch lane of row rr
+        const R s1 = lane_value(sd[1], j), s5 = lane_value(sd[5], j - 1);
+        const R s8 = lane_value(sd[8], j + 1), s3 = lane_value(sd[3], j);
+        const R s6 = lane_value(sd[6], j - 1), s7 = lane_value(sd[7], j + 1);
+        if (wave == 0) {
+          l1 = s1;
+          l5 = s5;
+          l8 = s8;
+        }
+        if (wave == TWO_WAVES - 1) {
+          r3 = s3;
+          r6 = s6;
+          r7 = s7;
+        }
+      }
       if (lane == 0) {
         nb[1] = l1;
         nb[5] = l5;
@@ -605,6 +641,12 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
         S::store_cell(dst, rr * p.pitch + x, o, p, 0u);
       }
     }
+    el5m = el5;
+    er6m = er6;
+    el1 = nl1;
+    el5 = nl5;
+    er3 = nr3;
+    er6 = nr6;
     kd[0] = kc[2];
     kd[1] = kc[5];
     kd[2] = kc[6];
@@ -623,19 +665,21 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
 
 // Two plain steps of a rectangle of the lattice in one pass over the state:
 // METAF_t is read once and METAF_{t+2} written once, where two single steps
-// read and write the whole state twice.  One wave per tile (x0, y0, rows,
-// walls): lane l works on padded column x0 + l (x0 a multiple of 64, so every
-// row of a tile is whole cache lines: stores that share a line with another
-// tile's lose half the bandwidth); the outputs are rows y0 .. y0 + rows - 1.
-// The wave marches up the rows.  For row r it forms step t+1 of its 64
-// columns (pull from METAF_t, collide, round as the store would) and keeps
-// the values in registers; with rows r-2, r-1, r in hand it forms step t+2
-// of row r-1, pulling the vertical directions from its own kept rows and the
-// diagonal and horizontal ones from lanes l-1 / l+1 by lane shuffles, and
-// stores it.  The two side columns x0-1 and x0+64 are formed at t+1 in
-// batches of 32 rows, two lanes per row, and lanes 0 and 63 read them by
-// readlane; rows y0-1 and y0+rows are formed at t+1 and not stored.
-// `walls` is set when any lane of the tile holds a class-4 node.
+// read and write the whole state twice.  One workgroup of four waves per
+// tile (x0, y0, rows, walls): lane l of wave w works on padded column
+// x0 + 64 w + l (x0 a multiple of 256, so every row of a tile is whole cache
+// lines: stores that share a line with another tile's lose half the
+// bandwidth); the outputs are rows y0 .. y0 + rows - 1.  The waves march up
+// the rows together.  For row r each forms step t+1 of its 64 columns (pull
+// from METAF_t, collide, round as the store would) and keeps the values in
+// registers; with rows r-2, r-1, r in hand it forms step t+2 of row r-1,
+// pulling the vertical directions from its own kept rows, the diagonal and
+// horizontal ones from lanes l-1 / l+1 by lane shuffles and, at its edge
+// lanes, from the neighbouring waves through LDS, and stores it.  The two
+// outer columns x0-1 and x0+256 are formed at t+1 in batches of 64 rows and
+// read by readlane; rows y0-1 and y0+rows are formed at t+1 and not stored.
+// `walls` is set when any column of the tile or its outer columns holds a
+// class-4 node.
 // cls carries bit 7 on the nodes the tiles advance: interior (class 1) or
 // in-tile wall (class 4) nodes whose fluid neighbours are all of class 1 or
 // 4 too, so that every t+1 value they pull is formed here without a
@@ -645,10 +689,11 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
 // (pull_cell, collide_bulk, the store's rounding), so the result equals two
 // single steps bit for bit.
 template <class S, int COLL>
-__global__ __launch_bounds__(64) void k_two_steps(const typename S::T* __restrict__ src,
-                                                  typename S::T* __restrict__ dst,
-                                                  const uint8_t* __restrict__ cls,
-                                                  const int4* __restrict__ tiles, KParams p) {
+__global__ __launch_bounds__(TWO_COLS) void k_two_steps(const typename S::T* __restrict__ src,
+                                                        typename S::T* __restrict__ dst,
+                                                        const uint8_t* __restrict__ cls,
+                                                        const int4* __restrict__ tiles,
+                                                        KParams p) {
   const int4 tl = tiles[blockIdx.x];  // x0, y0, rows, walls
   if (tl.z <= 0) return;
   if (tl.w)

@@ -5,6 +5,10 @@
 namespace lbmdev {
 
 constexpr int INST_PAR_W = 12;  // batched parameter row: omega, omega-, e[9], unused
+// Waves of a two-step workgroup (k_two_steps): 4 waves side by side cover a
+// tile of 256 columns.
+constexpr int TWO_WAVES = 4;
+constexpr int TWO_COLS = 64 * TWO_WAVES;
 
 enum KFlags : int {
   KF_TRT_FP64 = 1,       // TRT parts in FP64 (reference rounds them to binary32)

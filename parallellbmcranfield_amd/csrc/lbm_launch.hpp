@@ -19,7 +19,7 @@ struct Launch {
   int nb_bulk = 0;             // bulk grid (multiple of 8)
   int block = 256;             // bulk workgroup size
   bool pairs = false;          // FP16 plain steps: paired-cell kernel (k_step_pair)
-  const int4* tiles = nullptr; // two-step tiles (x0, y0, rows, 0), one wave each
+  const int4* tiles = nullptr; // two-step tiles (x0, y0, rows, walls), one workgroup each
   int ntiles = 0;
   KParams kp{};
   KOut out{};
@@ -43,7 +43,7 @@ hipError_t launch_fill_solid_f64(const Launch& L, void* buf);
 hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& L);
 hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L);
 hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf);
-// Resident waves of the two-step kernel per CU (0: no such kernel).
+// Resident workgroups of the two-step kernel per CU (0: no such kernel).
 int two_step_occupancy_f64(int coll);
 int two_step_occupancy_lp(int prec, int coll);
 

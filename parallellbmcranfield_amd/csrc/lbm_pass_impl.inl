@@ -65,7 +65,7 @@ static hipError_t pass_mode(int pass, const Launch& L) {
     case PASS_TWO:
       if constexpr (MODE == 0 && !S::SR) {
         if (L.ntiles <= 0) return hipSuccess;
-        k_two_steps<S, COLL><<<L.ntiles, 64, 0, L.stream>>>(src, dst, L.cls, L.tiles, L.kp);
+        k_two_steps<S, COLL><<<L.ntiles, TWO_COLS, 0, L.stream>>>(src, dst, L.cls, L.tiles, L.kp);
         return hipGetLastError();
       }
       return hipErrorInvalidValue;
@@ -88,15 +88,15 @@ static hipError_t pass_impl(int mode, int pass, const Launch& L) {
   return hipErrorInvalidValue;
 }
 
-// Resident waves of k_two_steps per CU (one wave per workgroup); 0 where the
-// kernel does not exist.
+// Resident workgroups of k_two_steps per CU; 0 where the kernel does not
+// exist.
 template <class S, int COLL>
 static int two_step_occupancy_impl() {
   if constexpr (S::SR) {
     return 0;
   } else {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_two_steps<S, COLL>, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_two_steps<S, COLL>, TWO_COLS, 0) !=
         hipSuccess)
       return 0;
     return nb;

@@ -241,14 +241,15 @@ static int two_step_rows(const lbm_solver* s, int nstrip) {
   hipDeviceProp_t prop;
   int cus = 0;
   if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) cus = prop.multiProcessorCount;
-  const long long waves = (long long)std::max(per_cu, 1) * std::max(cus, 1);
-  const long long segs = std::max(1LL, waves / std::max(nstrip, 1));
+  const long long groups = (long long)std::max(per_cu, 1) * std::max(cus, 1);
+  const long long segs = std::max(1LL, groups / std::max(nstrip, 1));
   const int m = s->cs->m;
   return std::max(TWO_STEP_MIN_ROWS, (int)((m + segs - 1) / segs));
 }
 
-// Tiles of k_two_steps and the band's launches.  A tile is a strip of 64
-// padded columns (64 s .. 64 s + 63) and a run of at most `rows` output rows.
+// Tiles of k_two_steps and the band's launches.  A tile is a strip of 256
+// padded columns (256 s .. 256 s + 255, one workgroup of four waves) and a
+// run of at most `rows` output rows.
 // A node is the tile's to advance (bit 7 of its class byte) when it is an
 // interior or in-tile wall node (class 1 or 4) and so are all its fluid
 // neighbours (class 0 = no node): the t+1 values it pulls can then be formed
@@ -264,7 +265,7 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
                           long long active) {
   const lbm_options* o = &s->opt;
   const int n = s->cs->n, m = s->cs->m, pitch = s->pitch;
-  const int R = o->two_step_rows > 0 ? o->two_step_rows : two_step_rows(s, n / 64 + 1);
+  const int R = o->two_step_rows > 0 ? o->two_step_rows : two_step_rows(s, n / TWO_COLS + 1);
   auto ok = [](uint8_t c) { return c == 0 || c == 1 || c == 4; };
   auto bulk = [](uint8_t c) { return c == 1 || c == 4; };
   std::vector<uint8_t> cls2(cls);
@@ -282,16 +283,16 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
       }
     }
   std::vector<int4> tiles;
-  for (int x0 = 0; x0 <= n; x0 += 64) {
-    const int o1 = std::min(x0 + 64, pitch);  // one past the last output column
+  for (int x0 = 0; x0 <= n; x0 += TWO_COLS) {
+    const int o1 = std::min(x0 + TWO_COLS, pitch);  // one past the last output column
     for (int y = 1; y <= m; y += R) {
       const int y1 = std::min(y + R - 1, m);
       bool any = false, walls = false;
       for (int r = y; r <= y1 && !any; r++)
         for (int x = x0; x < o1 && !any; x++) any = cls2[(size_t)r * pitch + x] & 0x80;
-      // class-4 nodes anywhere in columns x0-1 .. x0+64 of rows y-1 .. y1+1
+      // class-4 nodes anywhere in columns x0-1 .. x0+256 of rows y-1 .. y1+1
       for (int r = y - 1; r <= y1 + 1 && any && !walls; r++)
-        for (int x = std::max(x0 - 1, 0); x <= std::min(x0 + 64, pitch - 1) && !walls; x++)
+        for (int x = std::max(x0 - 1, 0); x <= std::min(x0 + TWO_COLS, pitch - 1) && !walls; x++)
           walls = cls[(size_t)r * pitch + x] == 4;
       if (any) tiles.push_back(make_int4(x0, y, y1 - y + 1, walls ? 1 : 0));
     }
