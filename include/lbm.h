@@ -131,9 +131,10 @@ typedef struct lbm_options {
  * default nodes whose only rule is straight-wall bounce-back run in their
  * bulk tile).
  * LBM_TUNE_NO_TWO_STEP: every step its own launch.  By default two plain
- * steps in a row run as one pass over the state where that pays (FP64 and
- * FP32 storage, lattices of at least 2^22 cells, two thirds of them away
- * from the boundary-list nodes; not batched): interior and in-tile wall
+ * steps in a row run as one pass over the state where that pays (FP32
+ * storage on lattices of at least 2^20 cells, FP64 on lattices of at least
+ * 2^22 cells, two thirds of the nodes away from the boundary-list nodes; not
+ * batched): interior and in-tile wall
  * nodes read METAF_t once and write METAF_{t+2} once, the nodes next to
  * boundary-list nodes take two single steps.  two_step_rows > 0 turns them
  * on for any lattice and storage but stochastic rounding. */

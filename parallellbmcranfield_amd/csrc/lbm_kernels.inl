@@ -358,21 +358,23 @@ __global__ __launch_bounds__(256) void k_boundary(const typename S::T* __restric
 
 // One launch per step: workgroups [0, nb_bnd) take BS boundary-list nodes
 // each (dispatched first, so their longer latency hides under the bulk
-// tiles), workgroups [nb_bnd, nb_bnd + nb_bulk) are the bulk tiles.
+// tiles), workgroups [nb_bnd, nb_bnd + nb_bulk) are the bulk tiles.  The
+// work of workgroup `bid` of `ngrid`; `tlist` the bulk tiles to run (null:
+// all); `step` the stored step number.
 template <class S, int COLL, int MODE, int BS>
-__global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ src,
-                                             typename S::T* __restrict__ dst,
-                                             const uint8_t* __restrict__ cls,
-                                             const uint8_t* __restrict__ tflag,
-                                             const BNode* __restrict__ bn, int nbn, KParams p,
-                                             KOut o) {
+__device__ inline void step_block(unsigned bid, unsigned ngrid,
+                                  const typename S::T* __restrict__ src,
+                                  typename S::T* __restrict__ dst,
+                                  const uint8_t* __restrict__ cls,
+                                  const uint8_t* __restrict__ tflag,
+                                  const BNode* __restrict__ bn, int nbn,
+                                  const uint32_t* __restrict__ tlist, const KParams& p,
+                                  const KOut& o, uint32_t step) {
   using R = typename S::R;
-  const uint32_t step = step_number<S>(p);
-  if constexpr (MODE != 2) advance_seed<S>(p, step);
   const int nbb = p.nb_bnd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if ((int)blockIdx.x < nbb) {
-    const int i = blockIdx.x * BS + threadIdx.x;
+  if ((int)bid < nbb) {
+    const int i = bid * BS + threadIdx.x;
     if (i < nbn && !(bn[i].flags & BN_OUTLET) && bnode_live(p, bn[i])) {
       const BNode& nd = bn[i];  // fields read where used
       R F[9];
@@ -386,10 +388,10 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
       finish_node<S, COLL, MODE, true>(F, src, dst, p, o, nd.idx, &nd, acc, nd.inst, step);
     }
   } else {
-    unsigned b = blockIdx.x - nbb;
-    if (p.flags & KF_XCD_REMAP) b = xcd_block(b, gridDim.x - nbb);
+    unsigned b = bid - nbb;
+    if (p.flags & KF_XCD_REMAP) b = xcd_block(b, ngrid - nbb);
     else if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
-    if (p.tile_list) b = p.tile_list[b];
+    if (tlist) b = tlist[b];
     const int tf = b == 0xffffffffu ? 0 : tflag[b];
     if constexpr (MODE != 1) {
       if (tf == 0) return;
@@ -407,7 +409,20 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
       finish_node<S, COLL, MODE, false>(F, src, dst, p, o, idx, nullptr, acc, inst, step);
     }
   }
-  if constexpr (MODE == 1) block_sum<BS>(acc, o.part, blockIdx.x);
+  if constexpr (MODE == 1) block_sum<BS>(acc, o.part, bid);
+}
+
+template <class S, int COLL, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ src,
+                                             typename S::T* __restrict__ dst,
+                                             const uint8_t* __restrict__ cls,
+                                             const uint8_t* __restrict__ tflag,
+                                             const BNode* __restrict__ bn, int nbn, KParams p,
+                                             KOut o) {
+  const uint32_t step = step_number<S>(p);
+  if constexpr (MODE != 2) advance_seed<S>(p, step);
+  step_block<S, COLL, MODE, BS>(blockIdx.x, gridDim.x, src, dst, cls, tflag, bn, nbn,
+                                p.tile_list, p, o, step);
 }
 
 // ---------------------------------------------------------------- two steps per launch
@@ -688,13 +703,24 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
 // third buffer in separate launches.  The arithmetic per cell is k_step's
 // (pull_cell, collide_bulk, the store's rounding), so the result equals two
 // single steps bit for bit.
+// Workgroups [0, nband) run the band's first single step instead (k_step's
+// work over the band and its ring, grid bgrid <= nband, into bdst): it reads
+// only METAF_t and writes only the third buffer, so it shares the launch;
+// nband is a multiple of 8, which keeps the tiles' XCD order.
 template <class S, int COLL>
-__global__ __launch_bounds__(TWO_COLS) void k_two_steps(const typename S::T* __restrict__ src,
-                                                        typename S::T* __restrict__ dst,
-                                                        const uint8_t* __restrict__ cls,
-                                                        const int4* __restrict__ tiles,
-                                                        KParams p) {
-  const int4 tl = tiles[blockIdx.x];  // x0, y0, rows, walls
+__global__ __launch_bounds__(TWO_COLS) void k_two_steps(
+    const typename S::T* __restrict__ src, typename S::T* __restrict__ dst,
+    const uint8_t* __restrict__ cls, const int4* __restrict__ tiles, KParams p, int nband,
+    int bgrid, typename S::T* __restrict__ bdst, const uint8_t* __restrict__ bcls,
+    const uint8_t* __restrict__ btflag, const BNode* __restrict__ bn, int nbn,
+    const uint32_t* __restrict__ blist, KOut o) {
+  if ((int)blockIdx.x < nband) {
+    if ((int)blockIdx.x < bgrid)
+      step_block<S, COLL, 0, TWO_COLS>(blockIdx.x, bgrid, src, bdst, bcls, btflag, bn, nbn, blist,
+                                       p, o, 0u);
+    return;
+  }
+  const int4 tl = tiles[blockIdx.x - nband];  // x0, y0, rows, walls
   if (tl.z <= 0) return;
   if (tl.w)
     two_steps_tile<S, COLL, true>(src, dst, cls, tl, p);

@@ -21,6 +21,14 @@ struct Launch {
   bool pairs = false;          // FP16 plain steps: paired-cell kernel (k_step_pair)
   const int4* tiles = nullptr; // two-step tiles (x0, y0, rows, walls), one workgroup each
   int ntiles = 0;
+  // the band's first single step in the two-step launch: bgrid workgroups of
+  // k_step's work (boundary list + bulk tiles `blist` of classes `bcls`) into
+  // bdst, padded to nband (a multiple of 8) ahead of the tiles
+  int nband = 0, bgrid = 0;
+  void* bdst = nullptr;
+  const uint8_t* bcls = nullptr;
+  const uint8_t* btflag = nullptr;
+  const uint32_t* blist = nullptr;
   KParams kp{};
   KOut out{};
   hipStream_t stream = nullptr;

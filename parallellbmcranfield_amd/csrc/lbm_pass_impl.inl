@@ -65,7 +65,9 @@ static hipError_t pass_mode(int pass, const Launch& L) {
     case PASS_TWO:
       if constexpr (MODE == 0 && !S::SR) {
         if (L.ntiles <= 0) return hipSuccess;
-        k_two_steps<S, COLL><<<L.ntiles, TWO_COLS, 0, L.stream>>>(src, dst, L.cls, L.tiles, L.kp);
+        k_two_steps<S, COLL><<<L.nband + L.ntiles, TWO_COLS, 0, L.stream>>>(
+            src, dst, L.cls, L.tiles, L.kp, L.nband, L.bgrid, (T*)L.bdst, L.bcls, L.btflag, L.bn,
+            L.nbn, L.blist, L.out);
         return hipGetLastError();
       }
       return hipErrorInvalidValue;

@@ -299,13 +299,15 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
   }
   std::vector<uint8_t> covered((size_t)s->plane, 0);
   for (size_t i = 0; i < cls2.size(); i++) covered[i] = (cls2[i] & 0x80) != 0;
-  // By default on FP64 / FP32 lattices of at least 2^22 cells (at 1025^2 a
-  // step is on its latency floor and the band's two extra launches cost
-  // more than the halved traffic saves; FP16's paired-cell single step moves
-  // its bytes faster than the one-cell two-step kernel).
+  // By default on FP32 lattices of at least 2^20 cells and FP64 ones of at
+  // least 2^22 (measured: FP32 at 1025^2 57.6 k against 52.1 k MLUPS for
+  // single steps; FP64 at 1025^2, with fewer resident waves, loses);
+  // FP16's paired-cell single step moves its bytes faster than the one-cell
+  // two-step kernel.
+  const long long cells_min = s->prec == LBM_PREC_F32 ? (1LL << 20) : (1LL << 22);
   const bool want = o->two_step_rows > 0 ||
                     ((s->prec == LBM_PREC_F64 || s->prec == LBM_PREC_F32) &&
-                     (long long)n * m >= (1LL << 22) && 3 * cells >= 2 * active);
+                     (long long)n * m >= cells_min && 3 * cells >= 2 * active);
   if (tiles.empty() || !want) return LBM_OK;
   std::stable_sort(tiles.begin(), tiles.end(), [](const int4& a, const int4& b) {
     return a.y != b.y ? a.y < b.y : a.x < b.x;
@@ -906,11 +908,12 @@ static int launch_step_passes(lbm_solver* s, const Launch& L) {
 }
 
 // Two plain steps from buf[src_buf] into buf[src_buf ^ 1]: the two-step
-// tiles (between events e0 and e1 when given), then the band's two single
-// steps through buf3.  The tiles read only buf[src_buf] and write only their
-// own nodes, so the order of the three launches is free.  (Running the band
-// on a second stream beside the tiles measured no faster: its workgroups
-// then share the CUs with the tiles.)
+// tiles (between events e0 and e1 when given) and the band's first single
+// step into buf3 in one launch (the tiles read only buf[src_buf] and write
+// only their own nodes; the band's first step reads buf[src_buf] and writes
+// only buf3), then the band's second step from buf3.  (Running the band on
+// a second stream beside the tiles measured no faster: its workgroups then
+// share the CUs with the tiles.)
 static int launch_two_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
                             hipEvent_t e1 = nullptr) {
   Launch L = make_launch(s, src_buf);
@@ -918,14 +921,25 @@ static int launch_two_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
   L.pairs = false;
   L.tiles = s->d_tiles;
   L.ntiles = s->ntiles;
+  // with the fused single-step layout (256-node bulk tiles) the band's first
+  // step runs as the first workgroups of the two-step launch
+  const bool fold = s->fused;
   {
     Launch T = L;
     T.cls = s->d_cls_two;
+    if (fold) {
+      T.bgrid = s->kp.nb_bnd + s->nlist_b[0];
+      T.nband = (T.bgrid + 7) / 8 * 8;
+      T.bdst = s->buf3;
+      T.bcls = s->d_cls_b[0];
+      T.btflag = s->d_tflag_b[0];
+      T.blist = s->d_list_b[0];
+    }
     if (e0) HC(hipEventRecord(e0, s->stream));
     HC(pass(s, MODE_STEP, PASS_TWO, T));
     if (e1) HC(hipEventRecord(e1, s->stream));
   }
-  for (int b = 0; b < 2; b++) {
+  for (int b = fold ? 1 : 0; b < 2; b++) {
     Launch B = L;
     B.src = b == 0 ? s->buf[src_buf] : s->buf3;
     B.dst = b == 0 ? s->buf3 : s->buf[src_buf ^ 1];
@@ -933,7 +947,15 @@ static int launch_two_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
     B.tflag = s->d_tflag_b[b];
     B.kp.tile_list = s->d_list_b[b];
     B.nb_bulk = s->nlist_b[b];
-    int rc = launch_step_passes(s, B);
+    int rc = LBM_OK;
+    if (b == 1 && fold) {
+      // the first step's open-outlet rows, after the feeders it folded in
+      Launch O = B;
+      O.src = s->buf[src_buf];
+      O.dst = s->buf3;
+      HC(pass(s, MODE_STEP, PASS_OUTLET, O));
+    }
+    rc = launch_step_passes(s, B);
     if (rc) return rc;
   }
   return LBM_OK;
