@@ -96,6 +96,9 @@ struct lbm_solver {
   int4* d_tiles = nullptr;    // [ntiles] (x0, y0, rows, 0), XCD-ordered, rows 0 = padding
   uint8_t* d_cls_two = nullptr;  // classes, bit 7 set on the nodes the tiles advance
   int ntiles = 0, ntiles_real = 0, two_rows = 0;
+  std::vector<int4> two_all;           // every two-step tile, row-major
+  std::vector<uint32_t> two_first;     // batched: tile t's instances are two_ids[first[t] .. [t+1])
+  std::vector<uint16_t> two_ids;
   long long two_cells = 0, band_cells = 0;
   void* buf3 = nullptr;       // METAF_{t+1} of the band and its ring
   uint8_t* d_cls_b[2] = {nullptr, nullptr};    // classes of the band's first / second step

@@ -427,15 +427,16 @@ __global__ __launch_bounds__(BS) void k_step(const typename S::T* __restrict__ s
 
 // ---------------------------------------------------------------- two steps per launch
 // Macros and collision of a bulk node (class 1 or 4) exactly as finish_node
-// forms them for MODE 0 outside the boundary list.
+// forms them for MODE 0 outside the boundary list; inst: the node's batched
+// instance (-1: the solver's own parameters).
 template <class S, int COLL>
 __device__ inline void collide_bulk(const typename S::R F[9], const KParams& p,
-                                    typename S::R out[9]) {
+                                    typename S::R out[9], int inst = -1) {
   using R = typename S::R;
   R Rh, U, V, dr = 0;
   if constexpr (S::DEV) {
     macro_dev(F, p.rho0_f, dr, Rh, U, V);
-    collide_dev<COLL>(F, dr, Rh, U, V, coll_par(p, -1), out);
+    collide_dev<COLL>(F, dr, Rh, U, V, coll_par(p, inst), out);
   } else {
     macro_f64(F, Rh, U, V);
     collide_f64<COLL>(F, Rh, U, V, p, out);
@@ -484,12 +485,12 @@ __device__ inline double lane_value(double v, int j) {
 // others, which no advanced node reads), rounded as the store would.
 template <class S, int COLL, bool WALLS>
 __device__ inline void form_t1(typename S::R kn[9], typename S::R F[9], int c, uint32_t w,
-                               const KParams& p) {
+                               const KParams& p, int inst = -1) {
   using R = typename S::R;
   if (c == 1 || c == 4) {
     if (WALLS && c == 4) wall_bounce(F, w);
     R o[9];
-    collide_bulk<S, COLL>(F, p, o);
+    collide_bulk<S, COLL>(F, p, o, inst);
 #pragma unroll
     for (int k = 0; k < 9; k++) kn[k] = S::round_trip(o[k], p);
   } else {
@@ -499,8 +500,10 @@ __device__ inline void form_t1(typename S::R kn[9], typename S::R F[9], int c, u
 }
 
 // One tile of k_two_steps; WALLS: the tile holds class-4 nodes (their stream
-// and wall masks are fetched with the class bytes).
-template <class S, int COLL, bool WALLS>
+// and wall masks are fetched with the class bytes); BATCH: a packed lattice
+// of instances (each node's instance is fetched with its class byte: its
+// relaxation rates, and nodes of stopped instances are not stored).
+template <class S, int COLL, bool WALLS, bool BATCH>
 __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
                                       typename S::T* __restrict__ dst,
                                       const uint8_t* __restrict__ cls, const int4 tl,
@@ -515,6 +518,14 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
   auto cls_at = [&](int r) -> int { return xin ? (int)cls[(uint32_t)(r * p.pitch + xc)] : 0; };
   auto mask_at = [&](int r) -> uint32_t {
     return WALLS ? p.wmask[(uint32_t)(r * p.pitch + xc)] : 0u;
+  };
+  auto inst_at = [&](int r) -> int {
+    if constexpr (BATCH) {
+      const int v = xin ? (int)p.node_inst[(uint32_t)(r * p.pitch + xc)] : 0xffff;
+      return v == 0xffff ? -1 : v;
+    } else {
+      return -1;
+    }
   };
   // Edges between the waves: each row, lane 0 of every wave publishes the
   // t+1 values its left neighbour pulls (directions 3, 6, 7) and lane 63 the
@@ -531,7 +542,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
   const int xs = wave == 0 ? tl.x - 1 : tl.x + TWO_COLS;
   R sd[9];
   R sq[9];
-  int sc = 0;
+  int sc = 0, si = -1;
   uint32_t sw = 0;
   int sidx = 0;
   auto side_classes = [&](int base) {
@@ -540,12 +551,17 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     sidx = (ok ? row : r0) * p.pitch + (ok ? xs : xc);
     sc = ok ? (int)cls[(uint32_t)sidx] & 0x7f : 0;
     sw = (WALLS && ok) ? p.wmask[(uint32_t)sidx] : 0u;
+    if constexpr (BATCH) {
+      const int v = ok ? (int)p.node_inst[(uint32_t)sidx] : 0xffff;
+      si = v == 0xffff ? -1 : v;
+    }
   };
   int sbase = r0;
   R kd[3] = {0, 0, 0};                     // row rr-1 at t+1: directions 2, 5, 6
   R kc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // row rr at t+1
   int cc = 0;                              // class byte of (x, rr)
   uint32_t wc = 0;                         // masks of (x, rr)
+  int ic = -1;                             // instance of (x, rr)
   // neighbour columns' t+1 values: left (1, 5, 8) and right (3, 6, 7) of
   // rows rr (1 / 3), rr-1 (5 / 6) and rr+1 (8 / 7)
   R el1 = 0, el5 = 0, el5m = 0, er3 = 0, er6 = 0, er6m = 0;
@@ -554,6 +570,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
   // to r+3 are in flight.
   int c0 = cls_at(r0), c1 = cls_at(min(r0 + 1, rlast)), c2 = cls_at(min(r0 + 2, rlast));
   uint32_t w0 = mask_at(r0), w1 = mask_at(min(r0 + 1, rlast)), w2 = mask_at(min(r0 + 2, rlast));
+  int i0 = inst_at(r0), i1 = inst_at(min(r0 + 1, rlast)), i2 = inst_at(min(r0 + 2, rlast));
   R FA[9], FB[9];
   issue_pulls<S>(FA, src, p, r0 * p.pitch + xc, c0 & 0x7f, w0);
   issue_pulls<S>(FB, src, p, min(r0 + 1, rlast) * p.pitch + xc, c1 & 0x7f, w1);
@@ -563,6 +580,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     const int r = r0 + it;  // row of step t+1
     const int cb = c0;
     const uint32_t w = w0;
+    const int ib = i0;
     if (side_owner) {
       // batch b serves step-t+2 rows from it = 2 + 62 b: classes at
       // it = 62 b, pulls at it = 1 + 62 b
@@ -571,13 +589,16 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     }
     c0 = c1;
     w0 = w1;
+    i0 = i1;
     c1 = c2;
     w1 = w2;
+    i1 = i2;
     const int rn2 = min(r + 2, rlast), rn3 = min(r + 3, rlast);
     c2 = cls_at(rn3);
     w2 = mask_at(rn3);
+    i2 = inst_at(rn3);
     R kn[9];  // row r at t+1
-    form_t1<S, COLL, WALLS>(kn, Fb, cb & 0x7f, w, p);
+    form_t1<S, COLL, WALLS>(kn, Fb, cb & 0x7f, w, p, ib);
     issue_pulls<S>(Fb, src, p, rn2 * p.pitch + xc, c1 & 0x7f, w1);
     // publish the edge lanes' row-r values, then take the neighbours'
     const int slot = it & 1;
@@ -599,7 +620,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
       const int rr = r - 1;  // row of step t+2
       if (side_owner && (it - 2) % 62 == 0) {
         sbase = rr - 1;
-        form_t1<S, COLL, WALLS>(sd, sq, sc, sw, p);
+        form_t1<S, COLL, WALLS>(sd, sq, sc, sw, p, si);
       }
       // F[k](x) = f_k(x - c_k) at t+1; lanes 0 and 63 take the neighbour
       // columns: the next wave's edge lanes, or the batch at the outer edges
@@ -640,7 +661,11 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
         nb[6] = r6;
         nb[7] = r7;
       }
-      if (xin && (cc & 0x80)) {
+      bool store = xin && (cc & 0x80);
+      if constexpr (BATCH) {
+        if (store && ic >= 0 && p.inst_active && !p.inst_active[ic]) store = false;
+      }
+      if (store) {
         R G[9];
         if (WALLS && (cc & 0x7f) == 4) {
           // StreamingStep mask, then WallBC, as pull_cell
@@ -652,7 +677,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
           for (int k = 0; k < 9; k++) G[k] = nb[k];
         }
         R o[9];
-        collide_bulk<S, COLL>(G, p, o);
+        collide_bulk<S, COLL>(G, p, o, ic);
         S::store_cell(dst, rr * p.pitch + x, o, p, 0u);
       }
     }
@@ -669,6 +694,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     for (int k = 0; k < 9; k++) kc[k] = kn[k];
     cc = cb;
     wc = w;
+    ic = ib;
   };
   // one copy of the two-row body: unrolled copies only grow the code
 #pragma unroll 1
@@ -722,10 +748,19 @@ __global__ __launch_bounds__(TWO_COLS) void k_two_steps(
   }
   const int4 tl = tiles[blockIdx.x - nband];  // x0, y0, rows, walls
   if (tl.z <= 0) return;
+  if constexpr (S::DEV) {
+    if (p.ninst) {
+      if (tl.w)
+        two_steps_tile<S, COLL, true, true>(src, dst, cls, tl, p);
+      else
+        two_steps_tile<S, COLL, false, true>(src, dst, cls, tl, p);
+      return;
+    }
+  }
   if (tl.w)
-    two_steps_tile<S, COLL, true>(src, dst, cls, tl, p);
+    two_steps_tile<S, COLL, true, false>(src, dst, cls, tl, p);
   else
-    two_steps_tile<S, COLL, false>(src, dst, cls, tl, p);
+    two_steps_tile<S, COLL, false, false>(src, dst, cls, tl, p);
 }
 
 // Plain steps with FP16 storage: the bulk workgroups take PAIRS of tiles

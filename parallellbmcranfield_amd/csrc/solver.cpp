@@ -261,8 +261,38 @@ static int two_step_rows(const lbm_solver* s, int nstrip) {
 // by row, then column, and dealt to the 8 XCDs in contiguous runs (launch
 // slot w goes to XCD w % 8), so the tiles that share edge lines meet in one
 // L2.
+// The tiles of running instances (all tiles for a single lattice) in XCD
+// order into d_tiles; captured two-step graphs are dropped (grid and list
+// change).
+static int rebuild_two_tiles(lbm_solver* s) {
+  std::vector<int4> run;
+  const int nt = (int)s->two_all.size();
+  for (int t = 0; t < nt; t++) {
+    bool live = !s->ninst;
+    for (uint32_t e = s->two_first[t]; !live && e < s->two_first[t + 1]; e++)
+      live = s->inst_active[s->two_ids[e]] != 0;
+    if (live) run.push_back(s->two_all[(size_t)t]);
+  }
+  const int nr = (int)run.size();
+  const int q = (nr + 7) / 8;  // tiles per XCD
+  std::vector<int4> order((size_t)8 * std::max(q, 1), make_int4(0, 0, 0, 0));
+  for (int x = 0; x < 8; x++)
+    for (int k = 0; k < q; k++)
+      if (x * q + k < nr) order[(size_t)8 * k + x] = run[(size_t)x * q + k];
+  HC(hipMemcpyAsync(s->d_tiles, order.data(), sizeof(int4) * order.size(),
+                    hipMemcpyHostToDevice, s->stream));
+  HC(hipStreamSynchronize(s->stream));
+  s->ntiles = (int)order.size();
+  for (hipGraphExec_t& g : s->gexec2)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  return LBM_OK;
+}
+
 static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS, int align,
-                          long long active) {
+                          long long active, const std::vector<uint16_t>& node_inst) {
   const lbm_options* o = &s->opt;
   const int n = s->cs->n, m = s->cs->m, pitch = s->pitch;
   const int R = o->two_step_rows > 0 ? o->two_step_rows : two_step_rows(s, n / TWO_COLS + 1);
@@ -313,11 +343,24 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
     return a.y != b.y ? a.y < b.y : a.x < b.x;
   });
   const int nt = (int)tiles.size();
-  const int q = (nt + 7) / 8;  // tiles per XCD
-  std::vector<int4> order((size_t)8 * q, make_int4(0, 0, 0, 0));
-  for (int x = 0; x < 8; x++)
-    for (int k = 0; k < q; k++)
-      if (x * q + k < nt) order[(size_t)8 * k + x] = tiles[(size_t)x * q + k];
+  // batched: the instances whose nodes each tile advances
+  s->two_first.assign(1, 0u);
+  s->two_ids.clear();
+  for (const int4& t : tiles) {
+    if (s->ninst) {
+      std::vector<uint16_t> ids;
+      for (int r = t.y; r < t.y + t.z; r++)
+        for (int x = t.x; x < std::min(t.x + TWO_COLS, pitch); x++) {
+          const size_t idx = (size_t)r * pitch + x;
+          if ((cls2[idx] & 0x80) && node_inst[idx] != 0xffff &&
+              std::find(ids.begin(), ids.end(), node_inst[idx]) == ids.end())
+            ids.push_back(node_inst[idx]);
+        }
+      s->two_ids.insert(s->two_ids.end(), ids.begin(), ids.end());
+    }
+    s->two_first.push_back((uint32_t)s->two_ids.size());
+  }
+  s->two_all = tiles;
   // the band, and the band with its ring
   std::vector<uint8_t> band((size_t)s->plane, 0), ring((size_t)s->plane, 0);
   long long nband = 0;
@@ -351,7 +394,7 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
     }
     while (list[b].empty() || list[b].size() % align) list[b].push_back(0xffffffffu);
   }
-  int rc = dev_alloc(s, (void**)&s->d_tiles, sizeof(int4) * order.size());
+  int rc = dev_alloc(s, (void**)&s->d_tiles, sizeof(int4) * 8 * (size_t)((nt + 7) / 8));
   if (!rc) rc = dev_alloc(s, (void**)&s->d_cls_two, cls2.size());
   if (!rc) rc = check_hip(hipMemcpy(s->d_cls_two, cls2.data(), cls2.size(), hipMemcpyHostToDevice), "upload two-step classes");
   if (!rc) rc = dev_alloc(s, &s->buf3, (size_t)9 * s->plane * s->elem + 256);
@@ -364,11 +407,10 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
     if (!rc) rc = check_hip(hipMemcpy(s->d_list_b[b], list[b].data(), sizeof(uint32_t) * list[b].size(), hipMemcpyHostToDevice), "upload band tiles");
     s->nlist_b[b] = (int)list[b].size();
   }
-  if (!rc) rc = check_hip(hipMemcpy(s->d_tiles, order.data(), sizeof(int4) * order.size(), hipMemcpyHostToDevice), "upload two-step tiles");
+  if (!rc) rc = rebuild_two_tiles(s);
   if (rc) return rc;
   s->two = true;
   s->two_rows = R;
-  s->ntiles = (int)order.size();
   s->ntiles_real = nt;
   s->two_cells = cells;
   s->band_cells = nband;
@@ -855,10 +897,10 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
       return rc;
     }
   }
-  if (ninst == 0 && s->prec != LBM_PREC_F16_SR && !(o->tune_flags & LBM_TUNE_NO_TWO_STEP)) {
+  if (s->prec != LBM_PREC_F16_SR && !(o->tune_flags & LBM_TUNE_NO_TWO_STEP)) {
     long long active = 0;
     for (uint8_t cv : cls) active += cv != 0;
-    rc = setup_two_step(s, cls, BS, align, active);
+    rc = setup_two_step(s, cls, BS, align, active, node_inst);
     if (rc) {
       if (d_grad) (void)hipFree(d_grad);
       lbm_solver_destroy(s);
@@ -1082,6 +1124,7 @@ static int rebuild_tile_list(lbm_solver* s) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+  if (s->two) return rebuild_two_tiles(s);
   return LBM_OK;
 }
 

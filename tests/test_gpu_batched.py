@@ -330,3 +330,33 @@ def test_batched_launch_shapes_are_bit_identical(prec):
         assert np.array_equal(it, out[0][1]), shape
         for a, b in zip(out[0][0], fields):
             assert np.array_equal(a, b), shape
+
+
+@pytest.mark.parametrize("prec,coll", [(lbm.PREC_F32, 3), (lbm.PREC_F32, 1), (lbm.PREC_F16, 3)])
+def test_batched_two_step_equals_single_steps(prec, coll):
+    """Pairs of plain steps of a packed lattice as two-step launches (each
+    node with its instance's rates; nodes of stopped instances not stored;
+    the tile list shrinking as instances stop) against every step its own
+    launch: same stop iterations, same fields bit for bit."""
+    print("batched two-step", prec, coll, flush=True)
+    cases = [lbm.Case.cavity_ex(nx, ny, r) for nx, ny, r, _, _ in SET]
+    packed = lbm.Case.pack(cases)
+    inst = [(nu, u) for *_, nu, u in SET]
+    out = []
+    for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (8, 0), (3, lbm.TUNE_NO_GRAPH)):
+        s = lbm.Solver(packed, batch_opts(prec, coll, resid_every=10, crit_v=1e-3,
+                                          use_stop_rule=1, two_step_rows=rows,
+                                          tune_flags=flags), instances=inst)
+        assert (s.two_step_info()["cells"] > 0) == (rows > 0)
+        for chunk in (7, 200, 393):
+            s.step(chunk)
+        it, _, _ = s.batch_status()
+        out.append((s.macro(), it, (rows, flags)))
+        s.close()
+    # instances stop at different iterations, so stopped ones sit beside
+    # running ones for part of the run
+    assert len(set(out[0][1][out[0][1] > 0])) > 1, out[0][1]
+    for fields, it, shape in out[1:]:
+        assert np.array_equal(it, out[0][1]), shape
+        for a, b in zip(out[0][0], fields):
+            assert np.array_equal(a, b), shape
