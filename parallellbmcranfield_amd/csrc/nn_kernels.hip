@@ -250,24 +250,50 @@ __device__ inline void wave_sync() {
 // waves, not the workgroup; a wave past the chunk's end loads zeros.  The 4
 // waves' results are summed in a fixed order through LDS (reusing the
 // slices) after a workgroup barrier.
+// Work item of a weight-gradient workgroup: (chunk, co block, tap, ci block)
+// of T x T tiles.  The grid is one-dimensional, a multiple of 8 workgroups:
+// workgroups are dealt to the 8 XCDs in turn, so workgroup b runs on XCD
+// b % 8, and it takes work item (b % 8) q + b / 8 (q = grid / 8).  Each XCD
+// then holds a contiguous range of items, and the items of one chunk (every
+// tap and channel block, all reading the same pixel groups) run side by
+// side on one XCD, where its L2 serves the pixels to all of them.  Items past
+// nwork return before any barrier.
+struct WgItem {
+  int chunk, co, t, ci;
+};
+__device__ inline bool wgrad_item(int T, int Cout, int Cin, int nwork, WgItem& w) {
+  const unsigned q = gridDim.x >> 3;
+  const unsigned id = (blockIdx.x & 7u) * q + (blockIdx.x >> 3);
+  if (id >= (unsigned)nwork) return false;
+  const int nci = Cin / T, per = (Cout / T) * 9 * nci;
+  w.chunk = (int)id / per;
+  int rem = (int)id % per;
+  w.ci = (rem % nci) * T;
+  rem /= nci;
+  w.t = rem % 9;
+  w.co = (rem / 9) * T;
+  return true;
+}
+
 constexpr int WG_PX = 64;          // pixels per K step of a wave
 constexpr int WG_LD = WG_PX + 8;   // LDS row (bf16): 144-byte rows keep 16-byte reads aligned
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ dout, int Cout,
-                                               float* __restrict__ part, int grp_per_chunk) {
+                                               float* __restrict__ part, int grp_per_chunk,
+                                               int nwork) {
   // [0]: dout^T [co][px], [1]: input^T [ci][px], per wave (36 864 B); the
   // final reduction reads it as float [4][32][33] (16 896 B)
   __shared__ __attribute__((aligned(16))) bf16 lds[2][4][32][WG_LD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
   const int Cin = s.C0 + s.C1;
-  const int cob = blockIdx.y * 32;
-  const int nci = Cin / 32;
-  const int t = blockIdx.z / nci, cib = (blockIdx.z % nci) * 32;
+  WgItem wi;
+  if (!wgrad_item(32, Cout, Cin, nwork, wi)) return;
+  const int cob = wi.co, t = wi.t, cib = wi.ci;
   const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
   const int P = s.B * s.H * s.W;
   const int ngrp = (P + WG_PX - 1) / WG_PX;
-  const int g0 = blockIdx.x * grp_per_chunk;
+  const int g0 = wi.chunk * grp_per_chunk;
   const int g1 = min(ngrp, g0 + grp_per_chunk);
   const int nit = (g1 - g0 + 3) / 4;  // the same for every wave
   // loader lanes: pixels 4 lq .. 4 lq + 3 of the group x 8-channel group lc
@@ -354,7 +380,7 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
     const int row = e / 32, col = e % 32;
     const int co = cob + row, ci = cib + col;
     const float v = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
-    part[(((size_t)blockIdx.x * Cout + co) * 9 + t) * Cin + ci] = v;
+    part[(((size_t)wi.chunk * Cout + co) * 9 + t) * Cin + ci] = v;
   }
 }
 
@@ -368,19 +394,20 @@ __global__ __launch_bounds__(256) void k_wgrad(Src s, const bf16* __restrict__ d
 // MFMAs; one workgroup barrier per group, reached by every thread (the
 // iteration count is the workgroup's).
 __global__ __launch_bounds__(256) void k_wgrad64(Src s, const bf16* __restrict__ dout, int Cout,
-                                                 float* __restrict__ part, int grp_per_chunk) {
+                                                 float* __restrict__ part, int grp_per_chunk,
+                                                 int nwork) {
   // [buffer][dout^T | input^T][channel][pixel] (36 864 B)
   __shared__ __attribute__((aligned(16))) bf16 lds[2][2][64][WG_LD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
   const int Cin = s.C0 + s.C1;
-  const int cob = blockIdx.y * 64;
-  const int nci = Cin / 64;
-  const int t = blockIdx.z / nci, cib = (blockIdx.z % nci) * 64;
+  WgItem wi;
+  if (!wgrad_item(64, Cout, Cin, nwork, wi)) return;
+  const int cob = wi.co, t = wi.t, cib = wi.ci;
   const int dy = (t / 3 - 1) * s.dil, dx = (t % 3 - 1) * s.dil;
   const int P = s.B * s.H * s.W;
   const int ngrp = (P + WG_PX - 1) / WG_PX;
-  const int g0 = blockIdx.x * grp_per_chunk;
+  const int g0 = wi.chunk * grp_per_chunk;
   const int nit = min(ngrp, g0 + grp_per_chunk) - g0;  // >= 1 for every launched workgroup
   // loader: operand `which`, pixels 4 lq .. 4 lq + 3 of the group, channels lc .. lc + 7
   const int which = threadIdx.x >> 7, li = threadIdx.x & 127;
@@ -467,7 +494,7 @@ __global__ __launch_bounds__(256) void k_wgrad64(Src s, const bf16* __restrict__
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int co = cob + arow + g * 16 + kb * 4 + i, ci = cib + brow + h * 16 + r;
-        part[(((size_t)blockIdx.x * Cout + co) * 9 + t) * Cin + ci] = acc[g][h][i];
+        part[(((size_t)wi.chunk * Cout + co) * 9 + t) * Cin + ci] = acc[g][h][i];
       }
 }
 
@@ -910,9 +937,11 @@ hipError_t wgrad(const ConvSrc& cs, const u16* dout, int Cout, float* part, int 
       (long)cs.B * cs.H * cs.W + WG_PX > 0x7fffffffL)
     return hipErrorInvalidValue;
   const int T = wgrad_tile(Cout, Cin);
-  dim3 g((unsigned)wgrad_chunks(cs, grp_per_chunk), (unsigned)(Cout / T), (unsigned)(9 * (Cin / T)));
-  if (T == 64) k_wgrad64<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk);
-  else k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk);
+  const long nwork = (long)wgrad_chunks(cs, grp_per_chunk) * (Cout / T) * 9 * (Cin / T);
+  if (nwork > 0x7ffffff0L) return hipErrorInvalidValue;
+  const unsigned g = (unsigned)((nwork + 7) / 8 * 8);  // a multiple of 8 (wgrad_item)
+  if (T == 64) k_wgrad64<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk, (int)nwork);
+  else k_wgrad<<<g, 256, 0, st>>>(src_of(cs), cb(dout), Cout, part, grp_per_chunk, (int)nwork);
   return hipGetLastError();
 }
 
