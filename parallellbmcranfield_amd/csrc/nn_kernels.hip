@@ -105,17 +105,28 @@ __device__ inline float load1(const Src& s, int b, int y, int x, int ci) {
 // the loop (waves past the last pixel load zeros) so every barrier is
 // reached.  Cout % (16 TN) == 0 (the host checks).
 constexpr int CV_LD = 40;  // LDS row of a weight tile (bf16): 80-byte rows, 16-byte aligned reads
+// The grid is one-dimensional, a multiple of 8 workgroups, dealt to the 8
+// XCDs in turn (workgroup b on XCD b % 8); workgroup b takes work item
+// (b % 8) q + b / 8 (q = grid / 8), item = pixel block x channel blocks +
+// channel block.  Each XCD then holds a contiguous run of pixel blocks, so
+// the rows above and below a block, which its stencil reads, are the
+// neighbouring blocks' own rows in the same L2, and the channel blocks of a
+// pixel block, which read the same pixels, run side by side.  Items past
+// nwork return before any barrier.
 template <int TM, int TN, bool RELU>
 __global__ __launch_bounds__(256) void k_conv3x3(Src s, const bf16* __restrict__ w,
                                                   const float* __restrict__ bias,
-                                                  bf16* __restrict__ out, int Cout) {
+                                                  bf16* __restrict__ out, int Cout, int nwork) {
   __shared__ __attribute__((aligned(16))) bf16 sw[2][16 * TN][CV_LD];
+  const unsigned item = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (item >= (unsigned)nwork) return;
+  const int ncb = Cout / (16 * TN);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kb = lane >> 4;
   const int P = s.B * s.H * s.W;
   const int Cin = s.C0 + s.C1;
-  const int pb = (blockIdx.x * 4 + wave) * (16 * TM);
-  const int cb = blockIdx.y * (16 * TN);
+  const int pb = ((int)(item / (unsigned)ncb) * 4 + wave) * (16 * TM);
+  const int cb = (int)(item % (unsigned)ncb) * (16 * TN);
   int pbb[TM], py[TM], px[TM];
   bool pv[TM];
 #pragma unroll
@@ -896,9 +907,10 @@ static inline Src src_of(const ConvSrc& c) {
 template <int TM, int TN>
 static void launch_conv(const Src& s, const u16* w, const float* bias, u16* out, int Cout,
                         long P, bool relu, hipStream_t st) {
-  dim3 g((unsigned)((P + 64 * TM - 1) / (64 * TM)), (unsigned)(Cout / (16 * TN)));
-  if (relu) k_conv3x3<TM, TN, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
-  else k_conv3x3<TM, TN, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout);
+  const long nwork = (P + 64 * TM - 1) / (64 * TM) * (Cout / (16 * TN));
+  const unsigned g = (unsigned)((nwork + 7) / 8 * 8);  // a multiple of 8 (k_conv3x3)
+  if (relu) k_conv3x3<TM, TN, true><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout, (int)nwork);
+  else k_conv3x3<TM, TN, false><<<g, 256, 0, st>>>(s, cb(w), bias, mb(out), Cout, (int)nwork);
 }
 
 // Tile choice: 64 output channels per wave where Cout allows, 64 pixels per
