@@ -26,7 +26,12 @@ ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 # Device-code flags.  packed-fp32-ops is disabled: v_pk_* FP32 forms are not
 # emitted.  The FP64 unit forbids contraction so FP64 results follow the C
-# evaluation order; the low-precision unit may contract.
+# evaluation order; the low-precision unit may contract.  The network unit
+# keeps the MFMA accumulators in VGPRs (amdgpu-mfma-vgpr-form): left to its
+# heuristics the compiler held them in VGPRs across the K loop and copied
+# them to AGPRs and back around every K step's MFMAs (128 v_accvgpr moves
+# per step in the 64 x 64 conv tile) and counted both register sets
+# against occupancy.
 HIP_COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang",
               "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
@@ -38,7 +43,7 @@ UNITS = [
     ("kernels_lp.hip", "hip", []),
     ("residuals.hip", "hip", ["-ffp-contract=off"]),
     ("residuals_noalloc.hip", "hip", ["-ffp-contract=off"]),
-    ("nn_kernels.hip", "hip", []),
+    ("nn_kernels.hip", "hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("case_io.cpp", "host", []),
     ("solver.cpp", "host", []),
     ("run.cpp", "host", []),
