@@ -116,6 +116,8 @@ typedef struct lbm_options {
   int    force_every;      /* force sampled every K steps (lbm_solver_force_history) */
   int    two_step_rows;    /* output rows per two-step tile (> 0 also turns two-step launches on
                               at any lattice size); 0 = tuned default */
+  int    three_step_rows;  /* output rows per three-step tile (> 0 also turns three-step
+                              launches on at any lattice size); 0 = tuned default */
 } lbm_options;
 
 /* Launch-shape switches (results are identical; only speed changes).
@@ -133,13 +135,22 @@ typedef struct lbm_options {
  * LBM_TUNE_NO_TWO_STEP: every step its own launch.  By default two plain
  * steps in a row run as one pass over the state where that pays (FP32
  * storage on lattices of at least 2^20 cells, FP64 on lattices of at least
- * 2^22 cells, two thirds of the nodes away from the boundary-list nodes; not
- * batched): interior and in-tile wall
+ * 2^22 cells, two thirds of the nodes away from the boundary-list nodes; also
+ * on packed lattices of instances): interior and in-tile wall
  * nodes read METAF_t once and write METAF_{t+2} once, the nodes next to
  * boundary-list nodes take two single steps.  two_step_rows > 0 turns them
- * on for any lattice and storage but stochastic rounding. */
+ * on for any lattice and storage but stochastic rounding.
+ * LBM_TUNE_NO_THREE_STEP: no three-step launches.  By default three plain
+ * steps in a row run as one pass over the state (METAF_t read once,
+ * METAF_{t+3} written once) where two-step launches are on and the lattice
+ * is not a packed one of instances; the nodes within two nodes of
+ * boundary-list nodes take three single steps.  three_step_rows > 0 turns
+ * them on for any single lattice and storage but stochastic rounding.  Runs
+ * of plain steps take three-step launches first, then a two-step or single
+ * one for the rest. */
 enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2, LBM_TUNE_NO_GRAPH = 4,
-       LBM_TUNE_NO_PAIRS = 8, LBM_TUNE_NO_TILE_WALLS = 16, LBM_TUNE_NO_TWO_STEP = 32 };
+       LBM_TUNE_NO_PAIRS = 8, LBM_TUNE_NO_TILE_WALLS = 16, LBM_TUNE_NO_TWO_STEP = 32,
+       LBM_TUNE_NO_THREE_STEP = 64 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;
@@ -251,9 +262,11 @@ int lbm_solver_step(lbm_solver* s, int nsteps, int* done);
 /* Same, timing the step kernel with HIP events on the solver's stream around
  * every `sample_every`-th launch (1 = every launch).  total_ms: stream time of
  * the nsteps; kernel_ms: average duration of one sampled step-kernel launch.
- * With two-step launches on (lbm_solver_two_step_info) the sampled kernel is
- * the two-step kernel, one launch per two steps, and single steps are not
- * sampled. */
+ * With multi-step launches on (lbm_solver_two_step_info,
+ * lbm_solver_three_step_info) the sampled kernel is the three-step kernel
+ * (one launch per three steps) where three-step launches are on, else the
+ * two-step kernel (one launch per two steps); launches of fewer steps are
+ * not sampled. */
 int lbm_solver_step_timed(lbm_solver* s, int nsteps, int sample_every, double* total_ms,
                           double* kernel_ms);
 int lbm_solver_sync(lbm_solver* s);
@@ -328,15 +341,22 @@ int lbm_solver_load(lbm_solver* s, const char* path);
 /* Kernel layout (for profiling): bit 0 set if interior and boundary-list
  * nodes share one launch per step (else separate launches); bit 1 set if
  * plain steps run the paired-cell FP16 kernel; bit 2 set if pairs of plain
- * steps run as two-step launches. */
+ * steps run as two-step launches; bit 3 set if runs of three plain steps run
+ * as three-step launches. */
 int lbm_solver_fused(const lbm_solver* s);
 /* Two-step launches of this solver: *cells = nodes the two-step kernel
  * advances by two steps per launch (0: two-step launches off), *tiles = its
- * tiles (one wave each), *rows = output rows per tile, *band = nodes that
- * take two single steps instead.  Any pointer may be NULL.  No counterpart
- * in the reference. */
+ * tiles (one workgroup each), *rows = output rows per tile, *band = nodes
+ * that take two single steps instead.  Any pointer may be NULL.  No
+ * counterpart in the reference. */
 int lbm_solver_two_step_info(const lbm_solver* s, long long* cells, int* tiles, int* rows,
                              long long* band);
+/* Three-step launches, as lbm_solver_two_step_info: nodes the three-step
+ * kernel advances by three steps per launch (0: off), its tiles, output rows
+ * per tile, nodes that take three single steps instead.  No counterpart in
+ * the reference. */
+int lbm_solver_three_step_info(const lbm_solver* s, long long* cells, int* tiles, int* rows,
+                               long long* band);
 
 /* Bandwidth of a plain device-to-device copy kernel over a buffer of `bytes`
  * (read + write counted), best of `reps` launches of each of several launch

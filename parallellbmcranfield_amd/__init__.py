@@ -29,7 +29,7 @@ INIT_REFERENCE, INIT_REST, INIT_FIELD, INIT_FIELD_NEQ = 0, 1, 2, 3
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
 (TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY, TUNE_NO_GRAPH, TUNE_NO_PAIRS, TUNE_NO_TILE_WALLS,
- TUNE_NO_TWO_STEP) = (1, 2, 4, 8, 16, 32)
+ TUNE_NO_TWO_STEP, TUNE_NO_THREE_STEP) = (1, 2, 4, 8, 16, 32, 64)
 
 # Every function include/lbm.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -43,7 +43,7 @@ EXPORTS = [
     "lbm_solver_residual_history", "lbm_solver_force_history", "lbm_solver_get_macro",
     "lbm_solver_get_f",
     "lbm_solver_write_results", "lbm_solver_timing", "lbm_solver_device_bytes",
-    "lbm_solver_fused", "lbm_solver_two_step_info", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs",
+    "lbm_solver_fused", "lbm_solver_two_step_info", "lbm_solver_three_step_info", "lbm_solver_save", "lbm_solver_load", "lbm_device_copy_gbs",
     "lbm_compute_residuals", "lbm_compute_residuals_noalloc", "lbm_device_stream_probe",
     "lbm_run",
     "lbm_run_checkpointed", "lbm_iteration",
@@ -84,7 +84,7 @@ class Options(C.Structure):
         ("crit_v", C.c_float), ("crit_rho", C.c_float), ("max_iterations", C.c_int),
         ("f16_scale", C.c_float), ("block_size", C.c_int), ("tune_flags", C.c_int),
         ("xcd_group", C.c_int), ("outlet_rho", C.c_double), ("force_id", C.c_int),
-        ("force_every", C.c_int), ("two_step_rows", C.c_int),
+        ("force_every", C.c_int), ("two_step_rows", C.c_int), ("three_step_rows", C.c_int),
     ]
 
 
@@ -157,6 +157,7 @@ def lib() -> C.CDLL:
         "lbm_device_stream_probe": (I, [I, I, I, I, C.POINTER(D), C.POINTER(D)]),
         "lbm_solver_fused": (I, [P]),
         "lbm_solver_two_step_info": (I, [P, P, P, P, P]),
+        "lbm_solver_three_step_info": (I, [P, P, P, P, P]),
         "lbm_compute_residuals": (I, [P, P, P, P, P, I, I, I, I, P, P]),
         "lbm_compute_residuals_noalloc": (I, [P, P, P, P, P, I, I, I, I, P, P]),
         "lbm_solver_save": (I, [P, C.c_char_p]),
@@ -441,6 +442,15 @@ class Solver:
         tiles, rows = C.c_int(0), C.c_int(0)
         _check(lib().lbm_solver_two_step_info(self.h, C.byref(cells), C.byref(tiles),
                                               C.byref(rows), C.byref(band)))
+        return dict(cells=cells.value, tiles=tiles.value, rows=rows.value, band=band.value)
+
+    def three_step_info(self) -> dict:
+        """Three-step launches: nodes advanced three steps per launch (0 =
+        off), the kernel's tiles, and the nodes that take three single steps."""
+        cells, band = C.c_longlong(0), C.c_longlong(0)
+        tiles, rows = C.c_int(0), C.c_int(0)
+        _check(lib().lbm_solver_three_step_info(self.h, C.byref(cells), C.byref(tiles),
+                                                C.byref(rows), C.byref(band)))
         return dict(cells=cells.value, tiles=tiles.value, rows=rows.value, band=band.value)
 
     def save(self, path: str) -> None:

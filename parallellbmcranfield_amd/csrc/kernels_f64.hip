@@ -16,7 +16,7 @@ hipError_t launch_pass_f64(int coll, int mode, int pass, const Launch& L) {
   return hipErrorInvalidValue;
 }
 
-int two_step_occupancy_f64(int coll) { return two_step_occupancy_coll<StF64>(coll); }
+int two_step_occupancy_f64(int coll, int steps) { return two_step_occupancy_coll<StF64>(coll, steps); }
 
 hipError_t launch_init_f64(int coll, int f0_kind, const Launch& L) {
   switch (coll) {

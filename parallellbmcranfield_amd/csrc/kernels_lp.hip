@@ -30,9 +30,9 @@ hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& 
   return hipErrorInvalidValue;
 }
 
-int two_step_occupancy_lp(int prec, int coll) {
-  if (prec == 1) return two_step_occupancy_coll<StF32>(coll);
-  if (prec == 2) return two_step_occupancy_coll<StF16>(coll);
+int two_step_occupancy_lp(int prec, int coll, int steps) {
+  if (prec == 1) return two_step_occupancy_coll<StF32>(coll, steps);
+  if (prec == 2) return two_step_occupancy_coll<StF16>(coll, steps);
   return 0;
 }
 

@@ -106,9 +106,25 @@ struct lbm_solver {
   uint32_t* d_list_b[2] = {nullptr, nullptr};  // bulk tiles of each band step (padded)
   int nlist_b[2] = {0, 0};
   hipGraphExec_t gexec2[2] = {nullptr, nullptr};  // two-step launches by source parity
-  // the last launch was a two-step one: buf[cur^1] holds the state two steps
-  // back, not the one step back that observe passes and checkpoints read
+  // three plain steps per launch: k_three_steps over its tiles, the band
+  // three single steps through buf3 (t+1 of the band and two rings) and buf4
+  // (t+2 of the band and one ring); single-lattice solvers only
+  bool three = false;
+  int4* d_tiles3 = nullptr;      // as d_tiles
+  uint8_t* d_cls_three = nullptr;  // classes, bit 7 set on the nodes the tiles advance
+  int ntiles3 = 0, ntiles3_real = 0, three_rows = 0;
+  long long three_cells = 0, band3_cells = 0;
+  void* buf4 = nullptr;
+  uint8_t* d_cls_b3[3] = {nullptr, nullptr, nullptr};  // classes of the band's three steps
+  uint8_t* d_tflag_b3[3] = {nullptr, nullptr, nullptr};
+  uint32_t* d_list_b3[3] = {nullptr, nullptr, nullptr};
+  int nlist_b3[3] = {0, 0, 0};
+  hipGraphExec_t gexec3[2] = {nullptr, nullptr};  // three-step launches by source parity
+  // the last launch was a two- or three-step one: buf[cur^1] holds the state
+  // prev_lag (2 or 3) steps back, not the one step back that observe passes
+  // and checkpoints read
   bool prev_stale = false;
+  int prev_lag = 2;
   lbmdev::KParams kp{};
   // host state
   int iter = 0;

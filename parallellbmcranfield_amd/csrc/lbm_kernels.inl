@@ -763,6 +763,388 @@ __global__ __launch_bounds__(TWO_COLS) void k_two_steps(
     two_steps_tile<S, COLL, false, false>(src, dst, cls, tl, p);
 }
 
+// ---------------------------------------------------------------- three steps per launch
+// The nine pulls of a cell at row rr from its column's kept rows (prev = row
+// rr-1, directions 2, 5, 6 only; cur = row rr; next = row rr+1) and lanes
+// l-1 / l+1; lanes 0 and 63 are completed by the caller.
+template <class R>
+__device__ inline void gather_rows(R nb[9], const R prev[3], const R cur[9], const R next[9]) {
+  nb[0] = cur[0];
+  nb[1] = __shfl_up(cur[1], 1, 64);
+  nb[2] = prev[0];
+  nb[3] = __shfl_down(cur[3], 1, 64);
+  nb[4] = next[4];
+  nb[5] = __shfl_up(prev[1], 1, 64);
+  nb[6] = __shfl_down(prev[2], 1, 64);
+  nb[7] = __shfl_down(next[7], 1, 64);
+  nb[8] = __shfl_up(next[8], 1, 64);
+}
+
+// The next step of a bulk cell (class 1 or 4) from its pulls nb and its own
+// values own (what a class-4 cell keeps where its stream mask is clear), as
+// pull_cell + finish_node form it, rounded as the store would; zeros for the
+// other classes (no advanced node reads them).
+template <class S, int COLL, bool WALLS>
+__device__ inline void form_next(typename S::R out[9], const typename S::R nb[9],
+                                 const typename S::R own[9], int c, uint32_t w,
+                                 const KParams& p) {
+  using R = typename S::R;
+  if (c == 1 || c == 4) {
+    R G[9];
+    if (WALLS && c == 4) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) G[k] = ((w >> k) & 1) ? nb[k] : own[k];
+      wall_bounce(G, w);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; k++) G[k] = nb[k];
+    }
+    R o[9];
+    collide_bulk<S, COLL>(G, p, o);
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[k] = S::round_trip(o[k], p);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[k] = 0;
+  }
+}
+
+// Step-t+3 rows one side batch serves (64 lanes: a batch's first and last two
+// rows only feed its own step-t+2 cells).
+constexpr int THREE_SIDE_ROWS = 60;
+
+// One tile of k_three_steps: k_two_steps' march one stage deeper.  For row r
+// each wave forms step t+1 of row r (pulls of METAF_t), step t+2 of row r-1
+// (from its kept t+1 rows r-2 .. r) and step t+3 of row r-2 (from its kept
+// t+2 rows r-3 .. r-1), which it stores.  Edge lanes take the neighbouring
+// waves' t+1 and t+2 values through LDS (one barrier per stage and row).  The
+// outer waves form the tile's outer columns x0-1 and x0+256 at t+1 and t+2 in
+// batches of 64 rows (lane j = row sbase + j): t+1 of the columns xs-1, xs,
+// xs+1 (classes fetched one row before the pulls, each column formed a row
+// after its pulls), then t+2 of xs from those, all kept in LDS and read there
+// by the edge lane.  Rows -1 and m+2, which a tile at the lattice's edge
+// reaches, hold nothing (class 0; their loads read rows 0 / m+1).
+// LDS of a three-step workgroup: wave edges [slot][wave][0: lane 0 (3, 6, 7),
+// 1: lane 63 (1, 5, 8)] at t+1 and t+2; outer columns [side][batch
+// parity][slot][batch lane]: slots 0-2 t+1 of xs-1 (directions 1, 5, 8), 3-11
+// t+1 of xs (0-8), 12-14 t+1 of xs+1 (3, 6, 7), 15-17 t+2 of xs (1, 5, 8 left;
+// 3, 6, 7 right).
+template <class R>
+struct ThreeLds {
+  R edge1[2][TWO_WAVES][2][3];
+  R edge2[2][TWO_WAVES][2][3];
+  R side[2][2][18][64];
+};
+
+template <class S, int COLL, bool WALLS>
+__device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
+                                        typename S::T* __restrict__ dst,
+                                        const uint8_t* __restrict__ cls, const int4 tl,
+                                        const KParams& p, ThreeLds<typename S::R>& lds) {
+  using R = typename S::R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = tl.x + 64 * wave + lane;
+  const bool xin = x < p.pitch;
+  const int xc = xin ? x : p.pitch - 1;
+  const int nit = tl.z + 4;  // step-t+1 rows y0-2 .. y0+rows+1
+  const int r0 = tl.y - 2, rlast = r0 + nit - 1;
+  const int rtop = p.m + 1;
+  auto rowc = [&](int r) { return min(max(r, 0), rtop); };
+  auto cls_at = [&](int r) -> int {
+    return (xin && r >= 0 && r <= rtop) ? (int)cls[(uint32_t)(r * p.pitch + xc)] : 0;
+  };
+  auto mask_at = [&](int r) -> uint32_t {
+    return WALLS ? p.wmask[(uint32_t)(rowc(r) * p.pitch + xc)] : 0u;
+  };
+  auto& edge1 = lds.edge1;
+  auto& edge2 = lds.edge2;
+  auto& side = lds.side;
+  const bool side_owner = wave == 0 || wave == TWO_WAVES - 1;
+  const int sn = wave == 0 ? 0 : 1;
+  const int xs = wave == 0 ? tl.x - 1 : tl.x + TWO_COLS;
+  auto clj = [](int j) { return min(max(j, 0), 63); };
+  // side batch state: classes fetched (A), pulls in flight (B)
+  R sq[9];
+  int scA = 0, scB = 0, sc1 = 0, sidxA = 0;
+  uint32_t swA = 0, swB = 0, sw1 = 0;
+  auto side_classes = [&](int col, int base) {
+    const int row = base + lane, xx = xs - 1 + col;
+    const bool ok = xx >= 0 && xx < p.pitch && row >= 0 && row <= rtop;
+    sidxA = rowc(row) * p.pitch + min(max(xx, 0), p.pitch - 1);
+    scA = ok ? (int)cls[(uint32_t)sidxA] & 0x7f : 0;
+    swA = (WALLS && ok) ? p.wmask[(uint32_t)sidxA] : 0u;
+  };
+  auto side_pulls = [&]() {
+    issue_pulls<S>(sq, src, p, sidxA, scA, swA);
+    scB = scA;
+    swB = swA;
+  };
+  auto side_form = [&](int col, int buf) {
+    R v[9];
+    form_t1<S, COLL, WALLS>(v, sq, scB, swB, p);
+    R* s = &side[sn][buf][0][0];
+    if (col == 0) {
+      s[0 * 64 + lane] = v[1];
+      s[1 * 64 + lane] = v[5];
+      s[2 * 64 + lane] = v[8];
+    } else if (col == 1) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) s[(3 + k) * 64 + lane] = v[k];
+      sc1 = scB;
+      sw1 = swB;
+    } else {
+      s[12 * 64 + lane] = v[3];
+      s[13 * 64 + lane] = v[6];
+      s[14 * 64 + lane] = v[7];
+    }
+  };
+  auto side_t2 = [&](int buf) {
+    R* s = &side[sn][buf][0][0];
+    const int jm = clj(lane - 1), jp = clj(lane + 1);
+    R own[9], nb[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) own[k] = s[(3 + k) * 64 + lane];
+    nb[0] = own[0];
+    nb[1] = s[0 * 64 + lane];
+    nb[5] = s[1 * 64 + jm];
+    nb[8] = s[2 * 64 + jp];
+    nb[2] = s[(3 + 2) * 64 + jm];
+    nb[4] = s[(3 + 4) * 64 + jp];
+    nb[3] = s[12 * 64 + lane];
+    nb[6] = s[13 * 64 + jm];
+    nb[7] = s[14 * 64 + jp];
+    R v[9];
+    form_next<S, COLL, WALLS>(v, nb, own, sc1, sw1, p);
+    if (sn == 0) {
+      s[15 * 64 + lane] = v[1];
+      s[16 * 64 + lane] = v[5];
+      s[17 * 64 + lane] = v[8];
+    } else {
+      s[15 * 64 + lane] = v[3];
+      s[16 * 64 + lane] = v[6];
+      s[17 * 64 + lane] = v[7];
+    }
+  };
+  // batch 0 (rows r0 .. r0+63), before the march
+  if (side_owner) {
+    for (int col = 0; col < 3; col++) {
+      side_classes(col, r0);
+      side_pulls();
+      side_form(col, 0);
+    }
+  }
+  __syncthreads();
+  if (side_owner) side_t2(0);
+  __syncthreads();
+  // kept state: t+1 rows r-2 (directions 2, 5, 6), r-1; t+2 rows r-3
+  // (directions 2, 5, 6), r-2; classes and masks of rows r-1 (cc1) and r-2 (cc2)
+  R kd1[3] = {0, 0, 0}, kc1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  R kd2[3] = {0, 0, 0}, kc2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int cc1 = 0, cc2 = 0;
+  uint32_t wc1 = 0, wc2 = 0;
+  // neighbour waves' edge values (as k_two_steps' el*/er*), t+1 and t+2
+  R el1 = 0, el5 = 0, el5m = 0, er3 = 0, er6 = 0, er6m = 0;
+  R fl1 = 0, fl5 = 0, fl5m = 0, fr3 = 0, fr6 = 0, fr6m = 0;
+  int c0 = cls_at(r0), c1 = cls_at(min(r0 + 1, rlast)), c2 = cls_at(min(r0 + 2, rlast));
+  uint32_t w0 = mask_at(r0), w1 = mask_at(min(r0 + 1, rlast)), w2 = mask_at(min(r0 + 2, rlast));
+  R FA[9], FB[9];
+  issue_pulls<S>(FA, src, p, rowc(r0) * p.pitch + xc, c0 & 0x7f, w0);
+  issue_pulls<S>(FB, src, p, rowc(min(r0 + 1, rlast)) * p.pitch + xc, c1 & 0x7f, w1);
+  auto row = [&](int it, R (&Fb)[9]) {
+    const int r = r0 + it;  // row of step t+1
+    const int cb = c0;
+    const uint32_t w = w0;
+    // side batch b >= 1 (rows r0 + 60 b ..): formed over iterations 60 b .. 60 b + 4
+    const int b = it / THREE_SIDE_ROWS, k = it - b * THREE_SIDE_ROWS;
+    const bool forming = side_owner && b >= 1 && k <= 4 && b * THREE_SIDE_ROWS + 4 < nit;
+    const int bbase = r0 + b * THREE_SIDE_ROWS;
+    if (forming) {
+      if (k >= 2) side_form(k - 2, b & 1);
+      if (k >= 1 && k <= 3) side_pulls();
+      if (k <= 2) side_classes(k, bbase);
+    }
+    // the batch in use: 0 up to iteration 63, then b from iteration 60 b + 4
+    const int bu = it < 64 ? 0 : (it - 4) / THREE_SIDE_ROWS;
+    const int ubuf = bu & 1, ubase = r0 + bu * THREE_SIDE_ROWS;
+    c0 = c1;
+    w0 = w1;
+    c1 = c2;
+    w1 = w2;
+    const int rn2 = min(r + 2, rlast), rn3 = min(r + 3, rlast);
+    c2 = cls_at(rn3);
+    w2 = mask_at(rn3);
+    R kn1[9];  // row r at t+1
+    form_t1<S, COLL, WALLS>(kn1, Fb, cb & 0x7f, w, p);
+    issue_pulls<S>(Fb, src, p, rowc(rn2) * p.pitch + xc, c1 & 0x7f, w1);
+    const int slot = it & 1;
+    if (lane == 0) {
+      edge1[slot][wave][0][0] = kn1[3];
+      edge1[slot][wave][0][1] = kn1[6];
+      edge1[slot][wave][0][2] = kn1[7];
+    }
+    if (lane == 63) {
+      edge1[slot][wave][1][0] = kn1[1];
+      edge1[slot][wave][1][1] = kn1[5];
+      edge1[slot][wave][1][2] = kn1[8];
+    }
+    __syncthreads();
+    if (forming && k == 4) side_t2(b & 1);
+    const int wl = wave > 0 ? wave - 1 : 0, wr = wave < TWO_WAVES - 1 ? wave + 1 : wave;
+    const R nl1 = edge1[slot][wl][1][0], nl5 = edge1[slot][wl][1][1], nl8 = edge1[slot][wl][1][2];
+    const R nr3 = edge1[slot][wr][0][0], nr6 = edge1[slot][wr][0][1], nr7 = edge1[slot][wr][0][2];
+    if (it >= 2) {
+      const int rr = r - 1;  // row of step t+2
+      R nb[9];
+      gather_rows(nb, kd1, kc1, kn1);
+      R l1 = el1, l5 = el5m, l8 = nl8, r3 = er3, r6 = er6m, r7 = nr7;
+      if (side_owner) {
+        const int j = rr - ubase;
+        const R* sv = &side[sn][ubuf][3][0];  // t+1 of xs, direction d at sv[d * 64 + row]
+        if (wave == 0) {
+          l1 = sv[1 * 64 + clj(j)];
+          l5 = sv[5 * 64 + clj(j - 1)];
+          l8 = sv[8 * 64 + clj(j + 1)];
+        }
+        if (wave == TWO_WAVES - 1) {
+          r3 = sv[3 * 64 + clj(j)];
+          r6 = sv[6 * 64 + clj(j - 1)];
+          r7 = sv[7 * 64 + clj(j + 1)];
+        }
+      }
+      if (lane == 0) {
+        nb[1] = l1;
+        nb[5] = l5;
+        nb[8] = l8;
+      }
+      if (lane == 63) {
+        nb[3] = r3;
+        nb[6] = r6;
+        nb[7] = r7;
+      }
+      R kn2[9];  // row rr at t+2
+      form_next<S, COLL, WALLS>(kn2, nb, kc1, cc1 & 0x7f, wc1, p);
+      if (lane == 0) {
+        edge2[slot][wave][0][0] = kn2[3];
+        edge2[slot][wave][0][1] = kn2[6];
+        edge2[slot][wave][0][2] = kn2[7];
+      }
+      if (lane == 63) {
+        edge2[slot][wave][1][0] = kn2[1];
+        edge2[slot][wave][1][1] = kn2[5];
+        edge2[slot][wave][1][2] = kn2[8];
+      }
+      __syncthreads();
+      const R ml1 = edge2[slot][wl][1][0], ml5 = edge2[slot][wl][1][1], ml8 = edge2[slot][wl][1][2];
+      const R mr3 = edge2[slot][wr][0][0], mr6 = edge2[slot][wr][0][1], mr7 = edge2[slot][wr][0][2];
+      if (it >= 4) {
+        const int rs = r - 2;  // row of step t+3
+        R nc[9];
+        gather_rows(nc, kd2, kc2, kn2);
+        R m1 = fl1, m5 = fl5m, m8 = ml8, m3 = fr3, m6 = fr6m, m7 = mr7;
+        if (side_owner) {
+          const int j = rs - ubase;
+          const R* sv = &side[sn][ubuf][15][0];  // t+2 of xs: three directions
+          if (wave == 0) {
+            m1 = sv[0 * 64 + clj(j)];
+            m5 = sv[1 * 64 + clj(j - 1)];
+            m8 = sv[2 * 64 + clj(j + 1)];
+          }
+          if (wave == TWO_WAVES - 1) {
+            m3 = sv[0 * 64 + clj(j)];
+            m6 = sv[1 * 64 + clj(j - 1)];
+            m7 = sv[2 * 64 + clj(j + 1)];
+          }
+        }
+        if (lane == 0) {
+          nc[1] = m1;
+          nc[5] = m5;
+          nc[8] = m8;
+        }
+        if (lane == 63) {
+          nc[3] = m3;
+          nc[6] = m6;
+          nc[7] = m7;
+        }
+        if (xin && (cc2 & 0x80)) {
+          R G[9];
+          if (WALLS && (cc2 & 0x7f) == 4) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) G[q] = ((wc2 >> q) & 1) ? nc[q] : kc2[q];
+            wall_bounce(G, wc2);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 9; q++) G[q] = nc[q];
+          }
+          R o[9];
+          collide_bulk<S, COLL>(G, p, o);
+          S::store_cell(dst, rs * p.pitch + x, o, p, 0u);
+        }
+      }
+      fl5m = fl5;
+      fr6m = fr6;
+      fl1 = ml1;
+      fl5 = ml5;
+      fr3 = mr3;
+      fr6 = mr6;
+      kd2[0] = kc2[2];
+      kd2[1] = kc2[5];
+      kd2[2] = kc2[6];
+#pragma unroll
+      for (int q = 0; q < 9; q++) kc2[q] = kn2[q];
+      cc2 = cc1;
+      wc2 = wc1;
+    }
+    el5m = el5;
+    er6m = er6;
+    el1 = nl1;
+    el5 = nl5;
+    er3 = nr3;
+    er6 = nr6;
+    kd1[0] = kc1[2];
+    kd1[1] = kc1[5];
+    kd1[2] = kc1[6];
+#pragma unroll
+    for (int q = 0; q < 9; q++) kc1[q] = kn1[q];
+    cc1 = cb;
+    wc1 = w;
+  };
+#pragma unroll 1
+  for (int it = 0; it < nit; it += 2) {
+    row(it, FA);
+    if (it + 1 < nit) row(it + 1, FB);
+  }
+}
+
+// Three plain steps of a rectangle of the lattice in one pass over the state
+// (METAF_t read once, METAF_{t+3} written once).  Tiles as k_two_steps'; cls
+// carries bit 7 on the nodes these tiles advance: class 1 or 4 with every
+// fluid node within two nodes (rows and columns) of class 1 or 4, so every
+// t+1 and t+2 value they pull is formed here.  The other nodes take three
+// single steps through two more buffers; workgroups [0, nband) run the first
+// of them (k_step's work over the band and its two rings into bdst), as in
+// k_two_steps.  Not for packed lattices of instances.
+template <class S, int COLL>
+__global__ __launch_bounds__(TWO_COLS) void k_three_steps(
+    const typename S::T* __restrict__ src, typename S::T* __restrict__ dst,
+    const uint8_t* __restrict__ cls, const int4* __restrict__ tiles, KParams p, int nband,
+    int bgrid, typename S::T* __restrict__ bdst, const uint8_t* __restrict__ bcls,
+    const uint8_t* __restrict__ btflag, const BNode* __restrict__ bn, int nbn,
+    const uint32_t* __restrict__ blist, KOut o) {
+  if ((int)blockIdx.x < nband) {
+    if ((int)blockIdx.x < bgrid)
+      step_block<S, COLL, 0, TWO_COLS>(blockIdx.x, bgrid, src, bdst, bcls, btflag, bn, nbn, blist,
+                                       p, o, 0u);
+    return;
+  }
+  __shared__ ThreeLds<typename S::R> lds;
+  const int4 tl = tiles[blockIdx.x - nband];  // x0, y0, rows, walls
+  if (tl.z <= 0) return;
+  if (tl.w)
+    three_steps_tile<S, COLL, true>(src, dst, cls, tl, p, lds);
+  else
+    three_steps_tile<S, COLL, false>(src, dst, cls, tl, p, lds);
+}
+
 // Plain steps with FP16 storage: the bulk workgroups take PAIRS of tiles
 // (2 x 256 consecutive cells) and each thread two neighbouring cells, so the
 // loads and stores of all-interior pairs move one 32-bit word per two cells:

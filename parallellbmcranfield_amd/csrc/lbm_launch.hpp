@@ -34,9 +34,17 @@ struct Launch {
   hipStream_t stream = nullptr;
 };
 
-// PASS_TWO: two plain steps of the two-step tiles (MODE_STEP, not with
-// stochastic rounding).
-enum Pass { PASS_BULK = 0, PASS_BOUNDARY = 1, PASS_OUTLET = 2, PASS_FUSED = 3, PASS_TWO = 4 };
+// PASS_TWO / PASS_THREE: two / three plain steps of the multi-step tiles
+// (MODE_STEP, not with stochastic rounding; three: not for packed lattices
+// of instances).
+enum Pass {
+  PASS_BULK = 0,
+  PASS_BOUNDARY = 1,
+  PASS_OUTLET = 2,
+  PASS_FUSED = 3,
+  PASS_TWO = 4,
+  PASS_THREE = 5
+};
 enum Mode { MODE_STEP = 0, MODE_RESID = 1, MODE_OBSERVE = 2 };
 
 // prec: 0 F64, 1 F32, 2 F16.  coll: 1 BGKW, 2 TRT, 3 MRT.
@@ -51,9 +59,10 @@ hipError_t launch_fill_solid_f64(const Launch& L, void* buf);
 hipError_t launch_pass_lp(int prec, int coll, int mode, int pass, const Launch& L);
 hipError_t launch_init_lp(int prec, int coll, int f0_kind, const Launch& L);
 hipError_t launch_fill_solid_lp(int prec, const Launch& L, void* buf);
-// Resident workgroups of the two-step kernel per CU (0: no such kernel).
-int two_step_occupancy_f64(int coll);
-int two_step_occupancy_lp(int prec, int coll);
+// Resident workgroups of the two-step (steps = 2) or three-step (3) kernel
+// per CU (0: no such kernel).
+int two_step_occupancy_f64(int coll, int steps = 2);
+int two_step_occupancy_lp(int prec, int coll, int steps = 2);
 
 // Deterministic FP64 sum of nparts x 6 partials -> out[6].
 hipError_t launch_reduce_partials(const double* part, int nparts, double* out, hipStream_t st);

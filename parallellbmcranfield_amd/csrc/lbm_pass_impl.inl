@@ -71,6 +71,16 @@ static hipError_t pass_mode(int pass, const Launch& L) {
         return hipGetLastError();
       }
       return hipErrorInvalidValue;
+    case PASS_THREE:
+      if constexpr (MODE == 0 && !S::SR) {
+        if (L.ntiles <= 0) return hipSuccess;
+        if (L.kp.ninst) return hipErrorInvalidValue;
+        k_three_steps<S, COLL><<<L.nband + L.ntiles, TWO_COLS, 0, L.stream>>>(
+            src, dst, L.cls, L.tiles, L.kp, L.nband, L.bgrid, (T*)L.bdst, L.bcls, L.btflag, L.bn,
+            L.nbn, L.blist, L.out);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
     case PASS_OUTLET:
       if (L.norows <= 0) return hipSuccess;
       k_outlet<S, COLL, MODE><<<(L.norows + 63) / 64, 64, 0, L.stream>>>(
@@ -93,24 +103,25 @@ static hipError_t pass_impl(int mode, int pass, const Launch& L) {
 // Resident workgroups of k_two_steps per CU; 0 where the kernel does not
 // exist.
 template <class S, int COLL>
-static int two_step_occupancy_impl() {
+static int two_step_occupancy_impl(int steps) {
   if constexpr (S::SR) {
     return 0;
   } else {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_two_steps<S, COLL>, TWO_COLS, 0) !=
-        hipSuccess)
-      return 0;
-    return nb;
+    const hipError_t e =
+        steps == 3
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_three_steps<S, COLL>, TWO_COLS, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_two_steps<S, COLL>, TWO_COLS, 0);
+    return e == hipSuccess ? nb : 0;
   }
 }
 
 template <class S>
-static int two_step_occupancy_coll(int coll) {
+static int two_step_occupancy_coll(int coll, int steps) {
   switch (coll) {
-    case 1: return two_step_occupancy_impl<S, 1>();
-    case 2: return two_step_occupancy_impl<S, 2>();
-    case 3: return two_step_occupancy_impl<S, 3>();
+    case 1: return two_step_occupancy_impl<S, 1>(steps);
+    case 2: return two_step_occupancy_impl<S, 2>(steps);
+    case 3: return two_step_occupancy_impl<S, 3>(steps);
   }
   return 0;
 }

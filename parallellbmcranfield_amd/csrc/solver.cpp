@@ -176,7 +176,8 @@ int flush_history(lbm_solver* s) {
 // After a two-step launch the state one step back (METAF_t, what observe
 // passes pull from) existed only inside the launch: form it again from the
 // state two steps back with one plain step into buf3 (the same arithmetic, so
-// the same bits) and put it in buf[cur^1].
+// the same bits) and put it in buf[cur^1]; after a three-step launch, with
+// two plain steps from the state three steps back.
 int refresh_previous(lbm_solver* s) {
   if (!s->prev_stale) return LBM_OK;
   Launch L = make_launch(s, s->cur ^ 1);
@@ -184,8 +185,19 @@ int refresh_previous(lbm_solver* s) {
   L.dst = s->buf3;
   int rc = launch_step_passes(s, L);
   if (rc) return rc;
-  HC(hipMemcpyAsync(s->buf[s->cur ^ 1], s->buf3, (size_t)9 * s->plane * s->elem,
-                    hipMemcpyDeviceToDevice, s->stream));
+  if (s->prev_lag == 3) {
+    // after a three-step launch: the second step from buf3 straight into
+    // buf[cur^1] (the state three steps back is no longer needed)
+    Launch M = make_launch(s, s->cur ^ 1);
+    M.kp.res_kind = RK_NONE;
+    M.src = s->buf3;
+    M.dst = s->buf[s->cur ^ 1];
+    rc = launch_step_passes(s, M);
+    if (rc) return rc;
+  } else {
+    HC(hipMemcpyAsync(s->buf[s->cur ^ 1], s->buf3, (size_t)9 * s->plane * s->elem,
+                      hipMemcpyDeviceToDevice, s->stream));
+  }
   s->prev_stale = false;
   return LBM_OK;
 }
@@ -235,9 +247,9 @@ static int rebuild_tile_list(lbm_solver* s);
 // for a second round, but at least TWO_STEP_MIN_ROWS (the two ring rows
 // cost a step-1 row each).
 constexpr int TWO_STEP_MIN_ROWS = 8;
-static int two_step_rows(const lbm_solver* s, int nstrip) {
-  const int per_cu = s->prec == 0 ? two_step_occupancy_f64(s->coll)
-                                  : two_step_occupancy_lp(s->prec, s->coll);
+static int two_step_rows(const lbm_solver* s, int nstrip, int steps = 2) {
+  const int per_cu = s->prec == 0 ? two_step_occupancy_f64(s->coll, steps)
+                                  : two_step_occupancy_lp(s->prec, s->coll, steps);
   hipDeviceProp_t prop;
   int cus = 0;
   if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) cus = prop.multiProcessorCount;
@@ -414,6 +426,134 @@ static int setup_two_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS
   s->ntiles_real = nt;
   s->two_cells = cells;
   s->band_cells = nband;
+  return LBM_OK;
+}
+
+// ---- three plain steps per launch (single lattices)
+// Tiles of k_three_steps and the band's three single steps, as
+// setup_two_step one stage deeper: a node is advanced by the tiles when it is
+// of class 1 or 4 and every fluid node within two nodes (rows and columns) is
+// of class 1 or 4; the band (every other fluid node) takes three single
+// steps: the first over the band and its two-node ring from buf[cur] into
+// buf3 (inside the three-step launch), the second over the band and its
+// one-node ring from buf3 into buf4, the third over the band from buf4 into
+// buf[cur^1].
+static int setup_three_step(lbm_solver* s, const std::vector<uint8_t>& cls, int BS, int align,
+                            long long active) {
+  const lbm_options* o = &s->opt;
+  const int n = s->cs->n, m = s->cs->m, pitch = s->pitch;
+  auto bulk = [](uint8_t c) { return c == 1 || c == 4; };
+  // distance to the nearest fluid node that is not bulk, capped at 3
+  std::vector<uint8_t> cls3(cls);
+  long long cells = 0;
+  for (int r = 1; r <= m; r++)
+    for (int x = 1; x <= n; x++) {
+      const size_t idx = (size_t)r * pitch + x;
+      if (!bulk(cls[idx])) continue;
+      bool g = true;
+      for (int dy = -2; dy <= 2 && g; dy++) {
+        const int rr = r + dy;
+        if (rr < 0 || rr > m + 1) continue;
+        for (int dx = -2; dx <= 2 && g; dx++) {
+          const int xx = x + dx;
+          if (xx < 0 || xx >= pitch) continue;
+          const uint8_t c = cls[(size_t)rr * pitch + xx];
+          g = c == 0 || bulk(c);
+        }
+      }
+      if (g) {
+        cls3[idx] |= 0x80;
+        cells++;
+      }
+    }
+  const bool want = o->three_step_rows > 0 || (s->two && 3 * cells >= 2 * active);
+  if (cells == 0 || !want) return LBM_OK;
+  const int R = o->three_step_rows > 0 ? o->three_step_rows : two_step_rows(s, n / TWO_COLS + 1, 3);
+  std::vector<int4> tiles;
+  for (int x0 = 0; x0 <= n; x0 += TWO_COLS) {
+    const int o1 = std::min(x0 + TWO_COLS, pitch);  // one past the last output column
+    for (int y = 1; y <= m; y += R) {
+      const int y1 = std::min(y + R - 1, m);
+      bool any = false, walls = false;
+      for (int r = y; r <= y1 && !any; r++)
+        for (int x = x0; x < o1 && !any; x++) any = cls3[(size_t)r * pitch + x] & 0x80;
+      // class-4 nodes anywhere in columns x0-3 .. x0+258 of rows y-3 .. y1+3
+      for (int r = std::max(y - 3, 0); r <= std::min(y1 + 3, m + 1) && any && !walls; r++)
+        for (int x = std::max(x0 - 3, 0); x <= std::min(x0 + TWO_COLS + 2, pitch - 1) && !walls; x++)
+          walls = cls[(size_t)r * pitch + x] == 4;
+      if (any) tiles.push_back(make_int4(x0, y, y1 - y + 1, walls ? 1 : 0));
+    }
+  }
+  std::stable_sort(tiles.begin(), tiles.end(), [](const int4& a, const int4& b) {
+    return a.y != b.y ? a.y < b.y : a.x < b.x;
+  });
+  const int nt = (int)tiles.size();
+  // the band, then its rings: reg[0] band + 2 rings, reg[1] band + 1 ring, reg[2] band
+  std::vector<uint8_t> reg[3];
+  for (auto& v : reg) v.assign((size_t)s->plane, 0);
+  long long nband = 0;
+  for (int r = 1; r <= m; r++)
+    for (int x = 1; x <= n; x++) {
+      const size_t idx = (size_t)r * pitch + x;
+      if (cls[idx] == 0 || (cls3[idx] & 0x80)) continue;
+      nband++;
+      reg[2][idx] = 1;
+      for (int dy = -2; dy <= 2; dy++)
+        for (int dx = -2; dx <= 2; dx++) {
+          const int rr = r + dy, xx = x + dx;
+          if (rr < 0 || rr > m + 1 || xx < 0 || xx >= pitch) continue;
+          const size_t j = (size_t)rr * pitch + xx;
+          reg[0][j] = 1;
+          if (dy >= -1 && dy <= 1 && dx >= -1 && dx <= 1) reg[1][j] = 1;
+        }
+    }
+  std::vector<uint8_t> cb[3], tf[3];
+  std::vector<uint32_t> list[3];
+  for (int b = 0; b < 3; b++) {
+    cb[b].assign((size_t)s->plane, 0);
+    for (size_t i = 0; i < cb[b].size(); i++)
+      if (reg[b][i] && bulk(cls[i])) cb[b][i] = cls[i];
+    tf[b].assign((size_t)s->nb_bulk, 0);
+    for (int t = 0; t < s->nb_bulk; t++) {
+      for (int e = 0; e < BS; e++) {
+        const long idx = (long)s->kp.bulk_begin + (long)t * BS + e;
+        if (idx < s->kp.bulk_end && cb[b][idx]) {
+          tf[b][t] = 2;
+          break;
+        }
+      }
+      if (tf[b][t]) list[b].push_back((uint32_t)t);
+    }
+    while (list[b].empty() || list[b].size() % align) list[b].push_back(0xffffffffu);
+  }
+  // XCD order as rebuild_two_tiles
+  const int q = (nt + 7) / 8;
+  std::vector<int4> order((size_t)8 * std::max(q, 1), make_int4(0, 0, 0, 0));
+  for (int x = 0; x < 8; x++)
+    for (int k = 0; k < q; k++)
+      if (x * q + k < nt) order[(size_t)8 * k + x] = tiles[(size_t)x * q + k];
+  int rc = dev_alloc(s, (void**)&s->d_tiles3, sizeof(int4) * order.size());
+  if (!rc) rc = check_hip(hipMemcpy(s->d_tiles3, order.data(), sizeof(int4) * order.size(), hipMemcpyHostToDevice), "upload three-step tiles");
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_cls_three, cls3.size());
+  if (!rc) rc = check_hip(hipMemcpy(s->d_cls_three, cls3.data(), cls3.size(), hipMemcpyHostToDevice), "upload three-step classes");
+  if (!rc && !s->buf3) rc = dev_alloc(s, &s->buf3, (size_t)9 * s->plane * s->elem + 256);
+  if (!rc) rc = dev_alloc(s, &s->buf4, (size_t)9 * s->plane * s->elem + 256);
+  for (int b = 0; b < 3 && !rc; b++) {
+    rc = dev_alloc(s, (void**)&s->d_cls_b3[b], cb[b].size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_tflag_b3[b], tf[b].size());
+    if (!rc) rc = dev_alloc(s, (void**)&s->d_list_b3[b], sizeof(uint32_t) * list[b].size());
+    if (!rc) rc = check_hip(hipMemcpy(s->d_cls_b3[b], cb[b].data(), cb[b].size(), hipMemcpyHostToDevice), "upload band classes");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_tflag_b3[b], tf[b].data(), tf[b].size(), hipMemcpyHostToDevice), "upload band tile flags");
+    if (!rc) rc = check_hip(hipMemcpy(s->d_list_b3[b], list[b].data(), sizeof(uint32_t) * list[b].size(), hipMemcpyHostToDevice), "upload band tiles");
+    s->nlist_b3[b] = (int)list[b].size();
+  }
+  if (rc) return rc;
+  s->three = true;
+  s->ntiles3 = (int)order.size();
+  s->ntiles3_real = nt;
+  s->three_rows = R;
+  s->three_cells = cells;
+  s->band3_cells = nband;
   return LBM_OK;
 }
 }
@@ -901,6 +1041,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
     long long active = 0;
     for (uint8_t cv : cls) active += cv != 0;
     rc = setup_two_step(s, cls, BS, align, active, node_inst);
+    if (!rc && !s->ninst && !(o->tune_flags & LBM_TUNE_NO_THREE_STEP))
+      rc = setup_three_step(s, cls, BS, align, active);
     if (rc) {
       if (d_grad) (void)hipFree(d_grad);
       lbm_solver_destroy(s);
@@ -914,6 +1056,8 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   if (e == hipSuccess) e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf[1]) : launch_fill_solid_lp(s->prec, L, s->buf[1]);
   if (e == hipSuccess && s->buf3)
     e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf3) : launch_fill_solid_lp(s->prec, L, s->buf3);
+  if (e == hipSuccess && s->buf4)
+    e = s->prec == 0 ? launch_fill_solid_f64(L, s->buf4) : launch_fill_solid_lp(s->prec, L, s->buf4);
   if (e == hipSuccess) e = s->prec == 0 ? launch_init_f64(s->coll, s->f0_kind, L) : launch_init_lp(s->prec, s->coll, s->f0_kind, L);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (d_grad) (void)hipFree(d_grad);
@@ -1020,6 +1164,71 @@ static int build_graph2(lbm_solver* s, int parity) {
   return check_hip(e, "hipGraphInstantiate");
 }
 
+// Three plain steps from buf[src_buf] into buf[src_buf ^ 1]: the three-step
+// tiles (between events e0 and e1 when given) and the band's first single
+// step (band + two rings, into buf3) in one launch, then the band's second
+// (band + one ring, buf3 -> buf4) and third (band, buf4 -> buf[src_buf ^ 1]).
+static int launch_three_steps(lbm_solver* s, int src_buf, hipEvent_t e0 = nullptr,
+                              hipEvent_t e1 = nullptr) {
+  Launch L = make_launch(s, src_buf);
+  L.kp.res_kind = RK_NONE;
+  L.pairs = false;
+  const bool fold = s->fused;
+  {
+    Launch T = L;
+    T.cls = s->d_cls_three;
+    T.tiles = s->d_tiles3;
+    T.ntiles = s->ntiles3;
+    if (fold) {
+      T.bgrid = s->kp.nb_bnd + s->nlist_b3[0];
+      T.nband = (T.bgrid + 7) / 8 * 8;
+      T.bdst = s->buf3;
+      T.bcls = s->d_cls_b3[0];
+      T.btflag = s->d_tflag_b3[0];
+      T.blist = s->d_list_b3[0];
+    }
+    if (e0) HC(hipEventRecord(e0, s->stream));
+    HC(pass(s, MODE_STEP, PASS_THREE, T));
+    if (e1) HC(hipEventRecord(e1, s->stream));
+  }
+  void* bsrc[3] = {s->buf[src_buf], s->buf3, s->buf4};
+  void* bdst[3] = {s->buf3, s->buf4, s->buf[src_buf ^ 1]};
+  for (int b = fold ? 1 : 0; b < 3; b++) {
+    Launch B = L;
+    B.src = bsrc[b];
+    B.dst = bdst[b];
+    B.cls = s->d_cls_b3[b];
+    B.tflag = s->d_tflag_b3[b];
+    B.kp.tile_list = s->d_list_b3[b];
+    B.nb_bulk = s->nlist_b3[b];
+    if (b == 1 && fold) {
+      // the first step's open-outlet rows, after the feeders it folded in
+      Launch O = B;
+      O.src = s->buf[src_buf];
+      O.dst = s->buf3;
+      HC(pass(s, MODE_STEP, PASS_OUTLET, O));
+    }
+    int rc = launch_step_passes(s, B);
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+static int build_graph3(lbm_solver* s, int parity) {
+  hipGraph_t g = nullptr;
+  HC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_three_steps(s, parity);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc != LBM_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HC(e);
+  e = hipGraphInstantiate(&s->gexec3[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return check_hip(e, "hipGraphInstantiate");
+}
+
 // Capture steps (parity, parity^1): after replay the state is back in
 // buf[parity] two iterations later.  Kernel nodes only (no memset nodes).
 static int build_graph(lbm_solver* s, int parity) {
@@ -1083,13 +1292,18 @@ void lbm_solver_destroy(lbm_solver* s) {
                   s->d_part, s->d_hist, s->d_inst_par, s->d_tile_inst, s->d_node_inst,
                   s->d_inst_active, s->d_inst_rect, s->d_inst_res, s->d_tile_list,
                   s->d_wmask, s->d_seed, s->d_tiles, s->d_cls_two, s->buf3, s->d_cls_b[0], s->d_cls_b[1],
-                  s->d_tflag_b[0], s->d_tflag_b[1], s->d_list_b[0], s->d_list_b[1]};
+                  s->d_tflag_b[0], s->d_tflag_b[1], s->d_list_b[0], s->d_list_b[1],
+                  s->d_tiles3, s->d_cls_three, s->buf4, s->d_cls_b3[0], s->d_cls_b3[1],
+                  s->d_cls_b3[2], s->d_tflag_b3[0], s->d_tflag_b3[1], s->d_tflag_b3[2],
+                  s->d_list_b3[0], s->d_list_b3[1], s->d_list_b3[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   for (hipGraphExec_t g : s->gexec)
     if (g) (void)hipGraphExecDestroy(g);
   for (hipGraphExec_t g : s->gexec2)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (hipGraphExec_t g : s->gexec3)
     if (g) (void)hipGraphExecDestroy(g);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -1232,9 +1446,40 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
                             !forces_at(s->iter + 1) && !stop_can_trigger &&
                             (!o.use_stop_rule || s->iter + 2 <= o.max_iterations);
     const bool sample_pair = evs && (t % sample_every == 0 || (t + 1) % sample_every == 0);
+    // Three plain steps in a row with three-step launches: one pass over the state.
+    const bool plain_triple = plain_pair && t + 2 < nsteps && !evals_at(s->iter + 2) &&
+                              !forces_at(s->iter + 2) &&
+                              (!o.use_stop_rule || s->iter + 3 <= o.max_iterations);
+    if (s->three && plain_triple) {
+      const bool sample = evs && (t % sample_every == 0 || (t + 1) % sample_every == 0 ||
+                                  (t + 2) % sample_every == 0);
+      if (sample) {
+        const int ts = t % sample_every == 0 ? t : ((t + 1) % sample_every == 0 ? t + 1 : t + 2);
+        const size_t ei = 2 * (size_t)(ts / sample_every);
+        int rc = launch_three_steps(s, s->cur, (*evs)[ei], (*evs)[ei + 1]);
+        if (rc) return rc;
+        if (rec) rec->push_back(ei);
+      } else if (s->use_graph) {
+        if (!s->gexec3[s->cur]) {
+          int rc = build_graph3(s, s->cur);
+          if (rc) return rc;
+        }
+        HC(hipGraphLaunch(s->gexec3[s->cur], s->stream));
+      } else {
+        int rc = launch_three_steps(s, s->cur);
+        if (rc) return rc;
+      }
+      s->cur ^= 1;
+      s->iter += 3;
+      ran += 3;
+      t += 2;
+      s->prev_stale = true;
+      s->prev_lag = 3;
+      continue;
+    }
     // Two plain steps in a row with two-step launches: one pass over the state.
     if (s->two && plain_pair) {
-      if (sample_pair) {
+      if (sample_pair && !s->three) {  // with three-step launches only those are sampled
         const size_t ei = 2 * (size_t)((t % sample_every == 0 ? t : t + 1) / sample_every);
         int rc = launch_two_steps(s, s->cur, (*evs)[ei], (*evs)[ei + 1]);
         if (rc) return rc;
@@ -1254,6 +1499,7 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
       ran += 2;
       t += 1;
       s->prev_stale = true;
+      s->prev_lag = 2;
       continue;
     }
     s->prev_stale = false;  // the launches below leave the previous state in buf[cur^1]
@@ -1275,8 +1521,8 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     L.kp.res_kind = rpart ? (rk == LBM_RESID_FULL ? RK_FULL : RK_VEL) : RK_NONE;
     L.kp.force_on = feval ? 1 : 0;
     if (rk == LBM_RESID_REFERENCE) L.out.cres = s->d_cres;
-    // with two-step launches only those are sampled
-    const bool timed = evs && !s->two && (t % sample_every == 0);
+    // with multi-step launches only those are sampled
+    const bool timed = evs && !s->two && !s->three && (t % sample_every == 0);
     const size_t ei = 2 * (size_t)(t / (sample_every > 0 ? sample_every : 1));
     if (timed) HC(hipEventRecord((*evs)[ei], s->stream));
     if (timed && rec) rec->push_back(ei);
@@ -1491,7 +1737,7 @@ size_t lbm_solver_device_bytes(const lbm_solver* s) { return s ? s->dev_bytes : 
 int lbm_solver_fused(const lbm_solver* s) {
   if (!s) return 0;
   return (s->fused ? 1 : 0) | (make_launch(const_cast<lbm_solver*>(s), 0).pairs ? 2 : 0) |
-         (s->two ? 4 : 0);
+         (s->two ? 4 : 0) | (s->three ? 8 : 0);
 }
 
 int lbm_solver_two_step_info(const lbm_solver* s, long long* cells, int* tiles, int* rows,
@@ -1501,6 +1747,16 @@ int lbm_solver_two_step_info(const lbm_solver* s, long long* cells, int* tiles, 
   if (tiles) *tiles = s->two ? s->ntiles_real : 0;
   if (rows) *rows = s->two ? s->two_rows : 0;
   if (band) *band = s->two ? s->band_cells : 0;
+  return LBM_OK;
+}
+
+int lbm_solver_three_step_info(const lbm_solver* s, long long* cells, int* tiles, int* rows,
+                               long long* band) {
+  if (!s) return LBM_EINVAL;
+  if (cells) *cells = s->three ? s->three_cells : 0;
+  if (tiles) *tiles = s->three ? s->ntiles3_real : 0;
+  if (rows) *rows = s->three ? s->three_rows : 0;
+  if (band) *band = s->three ? s->band3_cells : 0;
   return LBM_OK;
 }
 

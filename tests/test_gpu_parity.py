@@ -115,6 +115,28 @@ def test_two_step_launches_bit_exact_on_medium(name):
     assert sha(fd) == str(g["f_sha256"])
 
 
+@pytest.mark.parametrize("name", medium_cases())
+def test_three_step_launches_bit_exact_on_medium(name):
+    """The same 1000 steps with every run of three plain steps as one
+    three-step launch (16-row tiles; the last single step on its own): the
+    reference's u, v, rho and F bit for bit."""
+    print("three-step medium", name, flush=True)
+    g = load(name)
+    nodes, conn = meshgen.cavity(int(g["N"])).arrays()
+    case, s = run_gpu(nodes, conn, opts_from_golden(g, resid_kind=lbm.RESID_NONE,
+                                                    three_step_rows=16), int(g["ini"][8]))
+    info = s.three_step_info()
+    assert info["cells"] > 0.9 * case.info.num_fluid, info
+    assert info["cells"] + info["band"] == case.info.num_fluid, info
+    rho, u, v = s.macro()
+    n, m = int(g["n"]), int(g["m"])
+    for f, a in (("u", u), ("v", v), ("rho", rho)):
+        assert np.array_equal(a.reshape(m, n)[g["rows"]], g[f"{f}_rows"]), f
+        assert sha(a) == str(g[f"{f}_sha256"]), f
+    fd, _ = s.distributions()
+    assert sha(fd) == str(g["f_sha256"])
+
+
 # ------------------------------------------------------- switches vs the oracle
 SWITCH_CASES = {
     # name: (mesh factory, oracle params, steps)
@@ -705,14 +727,16 @@ def test_two_step_launches_are_bit_identical(prec, coll, geom):
     make, extra = TWO_STEP_GEOMS[geom]
     case = make()
     runs = []
-    for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (16, 0), (5, 0), (1, 0),
-                        (5, lbm.TUNE_NO_GRAPH)):
+    no3 = lbm.TUNE_NO_THREE_STEP
+    for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (16, no3), (5, no3), (1, no3),
+                        (5, lbm.TUNE_NO_GRAPH | no3)):
         o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
                              precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_FULL,
                              resid_every=7, trt_lambda=0.25, two_step_rows=rows,
                              tune_flags=flags, **extra)
         s = lbm.Solver(case, o)
         assert (s.two_step_info()["cells"] > 0) == (flags & lbm.TUNE_NO_TWO_STEP == 0)
+        assert s.three_step_info()["cells"] == 0
         for chunk in (1, 20, 2, 38):
             s.step(chunk)
         f, mf = s.distributions()
@@ -725,6 +749,75 @@ def test_two_step_launches_are_bit_identical(prec, coll, geom):
             assert np.array_equal(a, b), r[5]
         assert np.array_equal(base[1], r[1]) and np.array_equal(base[2], r[2]), r[5]
         assert np.array_equal(base[3], r[3]) and np.array_equal(base[4], r[4]), r[5]
+
+
+@pytest.mark.parametrize("geom", sorted(TWO_STEP_GEOMS))
+@pytest.mark.parametrize("prec,coll", [(lbm.PREC_F64, lbm.BGKW), (lbm.PREC_F64, lbm.MRT),
+                                       (lbm.PREC_F32, lbm.MRT), (lbm.PREC_F32, lbm.TRT),
+                                       (lbm.PREC_F16, lbm.BGKW), (lbm.PREC_F16, lbm.MRT)])
+def test_three_step_launches_are_bit_identical(prec, coll, geom):
+    """Runs of three plain steps as three-step launches (tiles of 1, 5 and 16
+    rows; alone, with single steps for the rest, and beside two-step
+    launches) against every step its own launch: same fields, distributions
+    and residual history bit for bit, with residual steps every 7th step and
+    observations after chunks that end on a three-step launch, eager and
+    graph-replayed."""
+    print("three-step", prec, coll, geom, flush=True)
+    make, extra = TWO_STEP_GEOMS[geom]
+    case = make()
+    runs = []
+    for rows3, rows2, flags in ((0, 0, lbm.TUNE_NO_TWO_STEP), (16, 0, 0), (5, 0, 0), (1, 0, 0),
+                                (5, 5, 0), (5, 0, lbm.TUNE_NO_GRAPH), (1, 16, lbm.TUNE_NO_GRAPH)):
+        o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
+                             precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_FULL,
+                             resid_every=7, trt_lambda=0.25, two_step_rows=rows2,
+                             three_step_rows=rows3, tune_flags=flags, **extra)
+        s = lbm.Solver(case, o)
+        info = s.three_step_info()
+        assert (info["cells"] > 0) == (rows3 > 0), info
+        assert (s.two_step_info()["cells"] > 0) == (rows2 > 0)
+        parts = []
+        for chunk in (1, 20, 2, 38, 3, 6):
+            s.step(chunk)
+            parts.append(s.macro())
+        f, mf = s.distributions()
+        its, vals = s.residual_history()
+        runs.append((parts, f, mf, its, vals, (rows3, rows2, flags)))
+        s.close()
+    base = runs[0]
+    for r in runs[1:]:
+        for pa, pb in zip(base[0], r[0]):
+            for a, b in zip(pa, pb):
+                assert np.array_equal(a, b), r[5]
+        assert np.array_equal(base[1], r[1]) and np.array_equal(base[2], r[2]), r[5]
+        assert np.array_equal(base[3], r[3]) and np.array_equal(base[4], r[4]), r[5]
+
+
+@pytest.mark.parametrize("prec", [lbm.PREC_F32, lbm.PREC_F64])
+def test_three_step_default_on_2049(prec):
+    """At 2049^2 three-step launches are on by default for FP32 and FP64
+    (beside two-step ones); 301 plain steps (100 three-step launches and a
+    single step) equal 301 single steps bit for bit, and so does the
+    distribution after a further 2 (one two-step launch)."""
+    print("three-step 2049", prec, flush=True)
+    case = lbm.Case.cavity(2049)
+    out = []
+    for flags in (0, lbm.TUNE_NO_TWO_STEP):
+        o = lbm.make_options(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.1,
+                             precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE,
+                             tune_flags=flags)
+        s = lbm.Solver(case, o)
+        info = s.three_step_info()
+        assert (info["cells"] > 0.99 * case.info.num_fluid) == (flags == 0), info
+        s.step(301)
+        m1 = s.macro()
+        s.step(2)
+        f, _ = s.distributions()
+        out.append((m1, f))
+        s.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 @pytest.mark.parametrize("prec,default_on", [(lbm.PREC_F32, True), (lbm.PREC_F64, True),
@@ -740,7 +833,7 @@ def test_two_step_default_on_2049(prec, default_on):
     for flags in (0, lbm.TUNE_NO_TWO_STEP):
         o = lbm.make_options(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.1,
                              precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE,
-                             tune_flags=flags)
+                             tune_flags=flags | lbm.TUNE_NO_THREE_STEP)
         s = lbm.Solver(case, o)
         assert (s.two_step_info()["cells"] > 0) == (flags == 0 and default_on)
         if flags == 0 and not default_on:
