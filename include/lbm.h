@@ -142,8 +142,9 @@ typedef struct lbm_options {
  * on for any lattice and storage but stochastic rounding.
  * LBM_TUNE_NO_THREE_STEP: no three-step launches.  By default three plain
  * steps in a row run as one pass over the state (METAF_t read once,
- * METAF_{t+3} written once) where two-step launches are on and the lattice
- * is not a packed one of instances; the nodes within two nodes of
+ * METAF_{t+3} written once) with FP32 storage on lattices of at least 2^22
+ * cells where two-step launches are on (not on packed lattices of
+ * instances); the nodes within two nodes of
  * boundary-list nodes take three single steps.  three_step_rows > 0 turns
  * them on for any single lattice and storage but stochastic rounding.  Runs
  * of plain steps take three-step launches first, then a two-step or single

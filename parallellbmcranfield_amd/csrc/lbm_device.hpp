@@ -365,45 +365,51 @@ __device__ inline void collide_trt_dev(const float g[9], float drho, float rho, 
 
 __device__ inline void collide_mrt_dev(const float g[9], float drho, float rho, float u, float v,
                                        const CollPar& p, float out[9]) {
+  // No implicit contraction: the fused multiply-adds are written out, so
+  // every kernel that inlines this (one cell or two per thread, one step or
+  // several per launch) rounds the same way.
+#pragma clang fp contract(off)
   // moments of g (rows of M)
   const float s14 = g[1] + g[2] + g[3] + g[4];
   const float s58 = g[5] + g[6] + g[7] + g[8];
   const float jx = (g[1] - g[3]) + (g[5] - g[7]) + (g[8] - g[6]);
   const float jy = (g[2] - g[4]) + (g[5] - g[7]) + (g[6] - g[8]);
-  const float m1 = -4.f * g[0] - s14 + 2.f * s58;
-  const float m2 = 4.f * g[0] - 2.f * s14 + s58;
-  const float m4 = 2.f * (g[3] - g[1]) + (g[5] - g[7]) + (g[8] - g[6]);
-  const float m6 = 2.f * (g[4] - g[2]) + (g[5] - g[7]) + (g[6] - g[8]);
+  const float m1 = fmaf(2.f, s58, fmaf(-4.f, g[0], -s14));
+  const float m2 = fmaf(4.f, g[0], fmaf(-2.f, s14, s58));
+  const float m4 = fmaf(2.f, g[3] - g[1], (g[5] - g[7]) + (g[8] - g[6]));
+  const float m6 = fmaf(2.f, g[4] - g[2], (g[5] - g[7]) + (g[6] - g[8]));
   const float m7 = (g[1] + g[3]) - (g[2] + g[4]);
   const float m8 = (g[5] + g[7]) - (g[6] + g[8]);
-  const float usq = u * u + v * v;
+  const float usq = fmaf(u, u, v * v);
   const float r3 = (p.flags & KF_MRT_TEXTBOOK) ? 3.f * rho * usq : 3.f * rho * rho * usq;
   // d_l = m_l(f) - m_l^eq; M w = (1,-2,1,0,...) absorbs rho0
-  float d[9];
-  d[0] = 0.f;
-  d[1] = m1 + 2.f * drho - r3;
-  d[2] = m2 - drho + r3;
-  d[3] = jx - rho * u;
-  d[4] = m4 + rho * u;
-  d[5] = jy - rho * v;
-  d[6] = m6 + rho * v;
-  d[7] = m7 - rho * (u * u - v * v);
-  d[8] = m8 - rho * u * v;
-  // e_l = s_l d_l / |M_l|^2 ; out = g - M^T e
-  float e[9];
-#pragma unroll
-  for (int l = 0; l < 9; l++) e[l] = p.e[l] * d[l];
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    float acc = 0.f;
-#pragma unroll
-    for (int l = 1; l < 9; l++) {
-      if (MM[l][k] == 1) acc += e[l];
-      else if (MM[l][k] == -1) acc -= e[l];
-      else if (MM[l][k] != 0) acc += (float)MM[l][k] * e[l];
-    }
-    out[k] = g[k] - acc;
-  }
+  const float d1 = fmaf(2.f, drho, m1) - r3;
+  const float d2 = (m2 - drho) + r3;
+  const float d3 = fmaf(-rho, u, jx);
+  const float d4 = fmaf(rho, u, m4);
+  const float d5 = fmaf(-rho, v, jy);
+  const float d6 = fmaf(rho, v, m6);
+  const float d7 = fmaf(-rho, fmaf(u, u, -(v * v)), m7);
+  const float d8 = fmaf(-rho, u * v, m8);
+  // e_l = s_l d_l / |M_l|^2 ; out = g - M^T e, with the terms the columns of
+  // M share formed once: columns 1-4 hold -e1 - 2 e2, columns 5-8 2 e1 + e2,
+  // and the momentum / energy-flux rows enter as (e3 -+ 2 e4), (e5 -+ 2 e6)
+  // or (e3 + e4), (e5 + e6) with the signs of c_x, c_y
+  const float e1 = p.e[1] * d1, e2 = p.e[2] * d2, e3 = p.e[3] * d3, e4 = p.e[4] * d4;
+  const float e5 = p.e[5] * d5, e6 = p.e[6] * d6, e7 = p.e[7] * d7, e8 = p.e[8] * d8;
+  const float a = fmaf(-2.f, e2, -e1), b = fmaf(2.f, e1, e2);
+  const float X = fmaf(-2.f, e4, e3), Y = fmaf(-2.f, e6, e5), P = e3 + e4, Q = e5 + e6;
+  const float ap = a + e7, am = a - e7, bp = b + e8, bm = b - e8;
+  const float PQ = P + Q, PmQ = P - Q;
+  out[0] = fmaf(-4.f, e2 - e1, g[0]);
+  out[1] = g[1] - (ap + X);
+  out[2] = g[2] - (am + Y);
+  out[3] = g[3] - (ap - X);
+  out[4] = g[4] - (am - Y);
+  out[5] = g[5] - (bp + PQ);
+  out[6] = g[6] - (bm - PmQ);
+  out[7] = g[7] - (bp - PQ);
+  out[8] = g[8] - (bm + PmQ);
 }
 
 template <int COLL>

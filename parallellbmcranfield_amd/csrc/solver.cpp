@@ -466,7 +466,14 @@ static int setup_three_step(lbm_solver* s, const std::vector<uint8_t>& cls, int 
         cells++;
       }
     }
-  const bool want = o->three_step_rows > 0 || (s->two && 3 * cells >= 2 * active);
+  // By default with FP32 storage on lattices of at least 2^22 cells beside
+  // two-step launches (MRT, MLUPS three-step against two-step: 4097^2 134.4 k
+  // / 114.5 k, 3073^2 126.7 k / 112.0 k, 2049^2 111.9 k / 101.6 k; at 1025^2
+  // the four ring rows of its 8-row tiles lose, 52.2 k / 56.7 k; FP64 at
+  // 4097^2, with two resident workgroups per CU, 55.3 k / 56.9 k)
+  const bool want = o->three_step_rows > 0 ||
+                    (s->two && s->prec == LBM_PREC_F32 && (long long)n * m >= (1LL << 22) &&
+                     3 * cells >= 2 * active);
   if (cells == 0 || !want) return LBM_OK;
   const int R = o->three_step_rows > 0 ? o->three_step_rows : two_step_rows(s, n / TWO_COLS + 1, 3);
   std::vector<int4> tiles;

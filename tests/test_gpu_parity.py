@@ -793,12 +793,13 @@ def test_three_step_launches_are_bit_identical(prec, coll, geom):
         assert np.array_equal(base[3], r[3]) and np.array_equal(base[4], r[4]), r[5]
 
 
-@pytest.mark.parametrize("prec", [lbm.PREC_F32, lbm.PREC_F64])
-def test_three_step_default_on_2049(prec):
-    """At 2049^2 three-step launches are on by default for FP32 and FP64
-    (beside two-step ones); 301 plain steps (100 three-step launches and a
-    single step) equal 301 single steps bit for bit, and so does the
-    distribution after a further 2 (one two-step launch)."""
+@pytest.mark.parametrize("prec,default_on", [(lbm.PREC_F32, True), (lbm.PREC_F64, False)])
+def test_three_step_default_on_2049(prec, default_on):
+    """At 2049^2 three-step launches are on by default for FP32 (beside
+    two-step ones; FP64 keeps two-step launches unless asked, here with
+    40-row tiles); 301 plain steps (100 three-step launches and a single
+    step) equal 301 single steps bit for bit, and so does the distribution
+    after a further 2 (one two-step launch)."""
     print("three-step 2049", prec, flush=True)
     case = lbm.Case.cavity(2049)
     out = []
@@ -807,6 +808,11 @@ def test_three_step_default_on_2049(prec):
                              precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_NONE,
                              tune_flags=flags)
         s = lbm.Solver(case, o)
+        assert (s.three_step_info()["cells"] > 0) == (flags == 0 and default_on)
+        if flags == 0 and not default_on:
+            s.close()
+            o.three_step_rows = 40
+            s = lbm.Solver(case, o)
         info = s.three_step_info()
         assert (info["cells"] > 0.99 * case.info.num_fluid) == (flags == 0), info
         s.step(301)
