@@ -820,9 +820,9 @@ constexpr int THREE_SIDE_ROWS = 60;
 // waves' t+1 and t+2 values through LDS (one barrier per stage and row).  The
 // outer waves form the tile's outer columns x0-1 and x0+256 at t+1 and t+2 in
 // batches of 64 rows (lane j = row sbase + j): t+1 of the columns xs-1, xs,
-// xs+1 (classes fetched one row before the pulls, each column formed a row
-// after its pulls), then t+2 of xs from those, all kept in LDS and read there
-// by the edge lane.  Rows -1 and m+2, which a tile at the lattice's edge
+// xs+1 (classes fetched one row before the pulls, the columns formed a row
+// after them), then t+2 of xs from those, all kept in LDS and read there by
+// the edge lane.  Rows -1 and m+2, which a tile at the lattice's edge
 // reaches, hold nothing (class 0; their loads read rows 0 / m+1).
 // LDS of a three-step workgroup: wave edges [slot][wave][0: lane 0 (3, 6, 7),
 // 1: lane 63 (1, 5, 8)] at t+1 and t+2; outer columns [side][batch
@@ -863,40 +863,41 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
   const int sn = wave == 0 ? 0 : 1;
   const int xs = wave == 0 ? tl.x - 1 : tl.x + TWO_COLS;
   auto clj = [](int j) { return min(max(j, 0), 63); };
-  // side batch state: classes fetched (A), pulls in flight (B)
-  R sq[9];
-  int scA = 0, scB = 0, sc1 = 0, sidxA = 0;
-  uint32_t swA = 0, swB = 0, sw1 = 0;
-  auto side_classes = [&](int col, int base) {
-    const int row = base + lane, xx = xs - 1 + col;
-    const bool ok = xx >= 0 && xx < p.pitch && row >= 0 && row <= rtop;
-    sidxA = rowc(row) * p.pitch + min(max(xx, 0), p.pitch - 1);
-    scA = ok ? (int)cls[(uint32_t)sidxA] & 0x7f : 0;
-    swA = (WALLS && ok) ? p.wmask[(uint32_t)sidxA] : 0u;
+  // side batch state: the three columns' classes, masks and indices, then
+  // their pulls, all issued together (the columns share their cache lines,
+  // which other tiles' batches would evict between separate passes)
+  R sq[3][9];
+  int sc[3] = {0, 0, 0}, sidx[3] = {0, 0, 0};
+  uint32_t sw[3] = {0, 0, 0};
+  auto side_classes = [&](int base) {
+    const int row = base + lane;
+#pragma unroll
+    for (int col = 0; col < 3; col++) {
+      const int xx = xs - 1 + col;
+      const bool ok = xx >= 0 && xx < p.pitch && row >= 0 && row <= rtop;
+      sidx[col] = rowc(row) * p.pitch + min(max(xx, 0), p.pitch - 1);
+      sc[col] = ok ? (int)cls[(uint32_t)sidx[col]] & 0x7f : 0;
+      sw[col] = (WALLS && ok) ? p.wmask[(uint32_t)sidx[col]] : 0u;
+    }
   };
   auto side_pulls = [&]() {
-    issue_pulls<S>(sq, src, p, sidxA, scA, swA);
-    scB = scA;
-    swB = swA;
-  };
-  auto side_form = [&](int col, int buf) {
-    R v[9];
-    form_t1<S, COLL, WALLS>(v, sq, scB, swB, p);
-    R* s = &side[sn][buf][0][0];
-    if (col == 0) {
-      s[0 * 64 + lane] = v[1];
-      s[1 * 64 + lane] = v[5];
-      s[2 * 64 + lane] = v[8];
-    } else if (col == 1) {
 #pragma unroll
-      for (int k = 0; k < 9; k++) s[(3 + k) * 64 + lane] = v[k];
-      sc1 = scB;
-      sw1 = swB;
-    } else {
-      s[12 * 64 + lane] = v[3];
-      s[13 * 64 + lane] = v[6];
-      s[14 * 64 + lane] = v[7];
-    }
+    for (int col = 0; col < 3; col++) issue_pulls<S>(sq[col], src, p, sidx[col], sc[col], sw[col]);
+  };
+  auto side_form = [&](int buf) {
+    R* s = &side[sn][buf][0][0];
+    R v[9];
+    form_t1<S, COLL, WALLS>(v, sq[0], sc[0], sw[0], p);
+    s[0 * 64 + lane] = v[1];
+    s[1 * 64 + lane] = v[5];
+    s[2 * 64 + lane] = v[8];
+    form_t1<S, COLL, WALLS>(v, sq[1], sc[1], sw[1], p);
+#pragma unroll
+    for (int k = 0; k < 9; k++) s[(3 + k) * 64 + lane] = v[k];
+    form_t1<S, COLL, WALLS>(v, sq[2], sc[2], sw[2], p);
+    s[12 * 64 + lane] = v[3];
+    s[13 * 64 + lane] = v[6];
+    s[14 * 64 + lane] = v[7];
   };
   auto side_t2 = [&](int buf) {
     R* s = &side[sn][buf][0][0];
@@ -914,7 +915,7 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
     nb[6] = s[13 * 64 + jm];
     nb[7] = s[14 * 64 + jp];
     R v[9];
-    form_next<S, COLL, WALLS>(v, nb, own, sc1, sw1, p);
+    form_next<S, COLL, WALLS>(v, nb, own, sc[1], sw[1], p);
     if (sn == 0) {
       s[15 * 64 + lane] = v[1];
       s[16 * 64 + lane] = v[5];
@@ -927,11 +928,9 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
   };
   // batch 0 (rows r0 .. r0+63), before the march
   if (side_owner) {
-    for (int col = 0; col < 3; col++) {
-      side_classes(col, r0);
-      side_pulls();
-      side_form(col, 0);
-    }
+    side_classes(r0);
+    side_pulls();
+    side_form(0);
   }
   __syncthreads();
   if (side_owner) side_t2(0);
@@ -954,14 +953,15 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
     const int r = r0 + it;  // row of step t+1
     const int cb = c0;
     const uint32_t w = w0;
-    // side batch b >= 1 (rows r0 + 60 b ..): formed over iterations 60 b .. 60 b + 4
+    // side batch b >= 1 (rows r0 + 60 b ..): classes at iteration 60 b,
+    // pulls at 60 b + 1, t+1 formed at 60 b + 2 and t+2 after its first
+    // barrier; in use from 60 b + 4
     const int b = it / THREE_SIDE_ROWS, k = it - b * THREE_SIDE_ROWS;
-    const bool forming = side_owner && b >= 1 && k <= 4 && b * THREE_SIDE_ROWS + 4 < nit;
-    const int bbase = r0 + b * THREE_SIDE_ROWS;
+    const bool forming = side_owner && b >= 1 && k <= 2 && b * THREE_SIDE_ROWS + 4 < nit;
     if (forming) {
-      if (k >= 2) side_form(k - 2, b & 1);
-      if (k >= 1 && k <= 3) side_pulls();
-      if (k <= 2) side_classes(k, bbase);
+      if (k == 0) side_classes(r0 + b * THREE_SIDE_ROWS);
+      if (k == 1) side_pulls();
+      if (k == 2) side_form(b & 1);
     }
     // the batch in use: 0 up to iteration 63, then b from iteration 60 b + 4
     const int bu = it < 64 ? 0 : (it - 4) / THREE_SIDE_ROWS;
@@ -988,7 +988,7 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
       edge1[slot][wave][1][2] = kn1[8];
     }
     __syncthreads();
-    if (forming && k == 4) side_t2(b & 1);
+    if (forming && k == 2) side_t2(b & 1);
     const int wl = wave > 0 ? wave - 1 : 0, wr = wave < TWO_WAVES - 1 ? wave + 1 : wave;
     const R nl1 = edge1[slot][wl][1][0], nl5 = edge1[slot][wl][1][1], nl8 = edge1[slot][wl][1][2];
     const R nr3 = edge1[slot][wr][0][0], nr6 = edge1[slot][wr][0][1], nr7 = edge1[slot][wr][0][2];
