@@ -469,6 +469,31 @@ __device__ inline void issue_pulls(typename S::R F[9], const typename S::T* src,
   }
 }
 
+// Value of v in lane l-1 (lane_up1) / l+1 (lane_down1) of the wave, by a DPP
+// wave shift (a VALU move, no LDS round trip); lane 0 / 63 gets 0.
+__device__ inline float lane_up1(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ inline float lane_down1(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+__device__ inline double lane_up1(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                (unsigned)lo));
+}
+__device__ inline double lane_down1(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                (unsigned)lo));
+}
+
 // Value of v in lane j (j the same in every lane).
 __device__ inline float lane_value(float v, int j) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
@@ -626,14 +651,14 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
       // columns: the next wave's edge lanes, or the batch at the outer edges
       R nb[9];
       nb[0] = kc[0];
-      nb[1] = __shfl_up(kc[1], 1, 64);
+      nb[1] = lane_up1(kc[1]);
       nb[2] = kd[0];
-      nb[3] = __shfl_down(kc[3], 1, 64);
+      nb[3] = lane_down1(kc[3]);
       nb[4] = kn[4];
-      nb[5] = __shfl_up(kd[1], 1, 64);
-      nb[6] = __shfl_down(kd[2], 1, 64);
-      nb[7] = __shfl_down(kn[7], 1, 64);
-      nb[8] = __shfl_up(kn[8], 1, 64);
+      nb[5] = lane_up1(kd[1]);
+      nb[6] = lane_down1(kd[2]);
+      nb[7] = lane_down1(kn[7]);
+      nb[8] = lane_up1(kn[8]);
       R l1 = el1, l5 = el5m, l8 = nl8, r3 = er3, r6 = er6m, r7 = nr7;
       if (side_owner) {
         const int j = rr - sbase;  // batch lane of row rr
@@ -766,7 +791,9 @@ __global__ __launch_bounds__(TWO_COLS) void k_two_steps(
 // ---------------------------------------------------------------- three steps per launch
 // The nine pulls of a cell at row rr from its column's kept rows (prev = row
 // rr-1, directions 2, 5, 6 only; cur = row rr; next = row rr+1) and lanes
-// l-1 / l+1; lanes 0 and 63 are completed by the caller.
+// l-1 / l+1; lanes 0 and 63 are completed by the caller.  (LDS permutes
+// here: DPP wave shifts, which take VALU issue slots, measured 346 -> 352 us
+// per launch in this VALU-heavy kernel.)
 template <class R>
 __device__ inline void gather_rows(R nb[9], const R prev[3], const R cur[9], const R next[9]) {
   nb[0] = cur[0];
