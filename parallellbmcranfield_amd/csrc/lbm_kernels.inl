@@ -469,6 +469,24 @@ __device__ inline void issue_pulls(typename S::R F[9], const typename S::T* src,
   }
 }
 
+// Pulls of one row of a tile without in-tile walls, every lane read as class
+// 1: lanes of other classes discard the values (their step is formed as
+// zeros), and for rows 1 .. m every address stays inside the allocation (a
+// halo lane's pull lands in the next or previous row or plane); rows outside
+// load their own words.  The offsets are then the same in every lane and go
+// into each plane's scalar base.
+template <class S>
+__device__ inline void issue_pulls_row(typename S::R F[9], const typename S::T* src,
+                                       const KParams& p, int idx, bool inside) {
+  using T = typename S::T;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int off = inside ? -CX[k] - CY[k] * p.pitch : 0;
+    const T* pk = src + (size_t)k * (size_t)p.plane + off;
+    F[k] = S::load((const T*)((const char*)pk + (uint32_t)idx * (uint32_t)sizeof(T)), p);
+  }
+}
+
 // Value of v in lane l-1 (lane_up1) / l+1 (lane_down1) of the wave, by a DPP
 // wave shift (a VALU move, no LDS round trip); lane 0 / 63 gets 0.
 __device__ inline float lane_up1(float v) {
@@ -596,9 +614,15 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
   int c0 = cls_at(r0), c1 = cls_at(min(r0 + 1, rlast)), c2 = cls_at(min(r0 + 2, rlast));
   uint32_t w0 = mask_at(r0), w1 = mask_at(min(r0 + 1, rlast)), w2 = mask_at(min(r0 + 2, rlast));
   int i0 = inst_at(r0), i1 = inst_at(min(r0 + 1, rlast)), i2 = inst_at(min(r0 + 2, rlast));
+  auto row_pulls = [&](R (&F)[9], int r, int c, uint32_t w) {
+    if constexpr (WALLS || !S::DEV)  // FP64: measured slower (569 -> 582 us per launch)
+      issue_pulls<S>(F, src, p, r * p.pitch + xc, c & 0x7f, w);
+    else
+      issue_pulls_row<S>(F, src, p, r * p.pitch + xc, r >= 1 && r <= p.m);
+  };
   R FA[9], FB[9];
-  issue_pulls<S>(FA, src, p, r0 * p.pitch + xc, c0 & 0x7f, w0);
-  issue_pulls<S>(FB, src, p, min(r0 + 1, rlast) * p.pitch + xc, c1 & 0x7f, w1);
+  row_pulls(FA, r0, c0, w0);
+  row_pulls(FB, min(r0 + 1, rlast), c1, w1);
   // Row r = r0 + it from the buffer Fb that holds its pulls; Fb is refilled
   // with the pulls of row r+2.
   auto row = [&](int it, R (&Fb)[9]) {
@@ -624,7 +648,7 @@ __device__ inline void two_steps_tile(const typename S::T* __restrict__ src,
     i2 = inst_at(rn3);
     R kn[9];  // row r at t+1
     form_t1<S, COLL, WALLS>(kn, Fb, cb & 0x7f, w, p, ib);
-    issue_pulls<S>(Fb, src, p, rn2 * p.pitch + xc, c1 & 0x7f, w1);
+    row_pulls(Fb, rn2, c1, w1);
     // publish the edge lanes' row-r values, then take the neighbours'
     const int slot = it & 1;
     if (lane == 0) {
@@ -973,9 +997,15 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
   R fl1 = 0, fl5 = 0, fl5m = 0, fr3 = 0, fr6 = 0, fr6m = 0;
   int c0 = cls_at(r0), c1 = cls_at(min(r0 + 1, rlast)), c2 = cls_at(min(r0 + 2, rlast));
   uint32_t w0 = mask_at(r0), w1 = mask_at(min(r0 + 1, rlast)), w2 = mask_at(min(r0 + 2, rlast));
+  auto row_pulls = [&](R (&F)[9], int r, int c, uint32_t w) {
+    if constexpr (WALLS || !S::DEV)  // FP64: measured slower (569 -> 582 us per two-step launch)
+      issue_pulls<S>(F, src, p, rowc(r) * p.pitch + xc, c & 0x7f, w);
+    else
+      issue_pulls_row<S>(F, src, p, rowc(r) * p.pitch + xc, r >= 1 && r <= p.m);
+  };
   R FA[9], FB[9];
-  issue_pulls<S>(FA, src, p, rowc(r0) * p.pitch + xc, c0 & 0x7f, w0);
-  issue_pulls<S>(FB, src, p, rowc(min(r0 + 1, rlast)) * p.pitch + xc, c1 & 0x7f, w1);
+  row_pulls(FA, r0, c0, w0);
+  row_pulls(FB, min(r0 + 1, rlast), c1, w1);
   auto row = [&](int it, R (&Fb)[9]) {
     const int r = r0 + it;  // row of step t+1
     const int cb = c0;
@@ -1002,7 +1032,7 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
     w2 = mask_at(rn3);
     R kn1[9];  // row r at t+1
     form_t1<S, COLL, WALLS>(kn1, Fb, cb & 0x7f, w, p);
-    issue_pulls<S>(Fb, src, p, rowc(rn2) * p.pitch + xc, c1 & 0x7f, w1);
+    row_pulls(Fb, rn2, c1, w1);
     const int slot = it & 1;
     if (lane == 0) {
       edge1[slot][wave][0][0] = kn1[3];
