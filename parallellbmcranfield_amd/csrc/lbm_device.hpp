@@ -443,7 +443,9 @@ __device__ inline void macro_dev(const float g[9], float rho0, float& drho, floa
   rho = rho0 + drho;
   const float jx = (g[1] - g[3]) + (g[5] - g[7]) + (g[8] - g[6]);
   const float jy = (g[2] - g[4]) + (g[5] - g[7]) + (g[6] - g[8]);
-  const float ir = 1.f / rho;
+  // the hardware reciprocal (within 1 ulp) instead of the IEEE division's
+  // ten-instruction sequence: 3 of a three-step launch's 343 us at 4097^2
+  const float ir = __builtin_amdgcn_rcpf(rho);
   u = jx * ir;
   v = jy * ir;
 }
