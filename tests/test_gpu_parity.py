@@ -539,6 +539,41 @@ def test_checkpoint_restart_is_exact(prec, tmp_path):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32])
+@pytest.mark.parametrize("rows2,rows3", [(5, 0), (0, 7), (5, 7)])
+def test_checkpoint_after_multi_step_launches(prec, rows2, rows3, tmp_path):
+    """A checkpoint taken right after a two- or three-step launch (whose
+    one-step-back state existed only inside the launch) restarts to the same
+    state bit for bit as the run that kept going, and as single steps; saved
+    at t = 33 (11 three-step launches, or 16 pairs and a single step) and at
+    t = 47 after a residual step."""
+    print("checkpoint multi-step", prec, rows2, rows3, flush=True)
+    case = lbm.Case.cavity(120)
+    kw = dict(collision=lbm.MRT, uavg=0.1, rho_ini=1.0, viscosity=0.03, init=lbm.INIT_REST,
+              precision=prec, resid_kind=lbm.RESID_VELOCITY, resid_every=46, use_stop_rule=0)
+    ref = lbm.Solver(case, lbm.make_options(tune_flags=lbm.TUNE_NO_TWO_STEP, **kw))
+    ref.step(80)
+    opts = lbm.make_options(two_step_rows=rows2, three_step_rows=rows3, **kw)
+    for t_save in (33, 47):
+        ck = str(tmp_path / f"ck{t_save}.bin")
+        a = lbm.Solver(case, opts)
+        assert (a.three_step_info()["cells"] > 0) == (rows3 > 0)
+        a.step(t_save)
+        a.save(ck)
+        a.step(80 - t_save)
+        b = lbm.Solver(case, opts)
+        b.load(ck)
+        b.step(80 - t_save)
+        fa, ma = a.distributions()
+        fb, mb = b.distributions()
+        fr, mr = ref.distributions()
+        assert np.array_equal(fa, fr) and np.array_equal(ma, mr), t_save
+        assert np.array_equal(fb, fr) and np.array_equal(mb, mr), t_save
+        assert np.array_equal(a.residuals(), ref.residuals())
+        a.close()
+        b.close()
+
+
 def test_parallel_writer_matches_oracle_bytes(tmp_path):
     """Result files of a 160^2 lattice (threaded formatter) are byte-identical
     to the oracle's sequential fprintf writer on the same FP64 state."""
