@@ -828,6 +828,37 @@ def test_three_step_launches_are_bit_identical(prec, coll, geom):
         assert np.array_equal(base[3], r[3]) and np.array_equal(base[4], r[4]), r[5]
 
 
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32])
+def test_multi_step_launches_with_split_passes(prec):
+    """With 512-cell bulk tiles (bulk and boundary-list passes in separate
+    launches) the band's first single step of a two- or three-step launch
+    runs as launches of its own instead of inside it: still bit-identical to
+    single steps, on the cylinder channel with its open outlet."""
+    print("multi-step split passes", prec, flush=True)
+    make, extra = TWO_STEP_GEOMS["cylinder_bouzidi_open_outlet"]
+    case = make()
+    runs = []
+    for rows2, rows3, flags in ((0, 0, lbm.TUNE_NO_TWO_STEP), (5, 0, lbm.TUNE_NO_THREE_STEP),
+                                (0, 5, 0), (4, 6, 0)):
+        o = lbm.make_options(collision=lbm.MRT, uavg=0.05, rho_ini=1.0, viscosity=0.03,
+                             precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_FULL,
+                             resid_every=7, block_size=512, two_step_rows=rows2,
+                             three_step_rows=rows3, tune_flags=flags, **extra)
+        s = lbm.Solver(case, o)
+        assert not s.fused
+        for chunk in (20, 3, 38):
+            s.step(chunk)
+        f, mf = s.distributions()
+        runs.append((s.macro(), f, mf, s.residual_history(), (rows2, rows3, flags)))
+        s.close()
+    base = runs[0]
+    for r in runs[1:]:
+        for a, b in zip(base[0], r[0]):
+            assert np.array_equal(a, b), r[4]
+        assert np.array_equal(base[1], r[1]) and np.array_equal(base[2], r[2]), r[4]
+        assert np.array_equal(base[3][0], r[3][0]) and np.array_equal(base[3][1], r[3][1]), r[4]
+
+
 @pytest.mark.parametrize("prec,default_on", [(lbm.PREC_F32, True), (lbm.PREC_F64, False)])
 def test_three_step_default_on_2049(prec, default_on):
     """At 2049^2 three-step launches are on by default for FP32 (beside
