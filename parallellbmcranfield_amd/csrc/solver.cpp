@@ -1467,8 +1467,9 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
         if (rc) return rc;
         if (rec) rec->push_back(ei);
       } else if (s->use_graph) {
-        if (!s->gexec3[s->cur]) {
+        if (!s->gexec3[s->cur]) {  // both parities at once: the other is next
           int rc = build_graph3(s, s->cur);
+          if (!rc && !s->gexec3[s->cur ^ 1]) rc = build_graph3(s, s->cur ^ 1);
           if (rc) return rc;
         }
         HC(hipGraphLaunch(s->gexec3[s->cur], s->stream));
@@ -1492,8 +1493,9 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
         if (rc) return rc;
         if (rec) rec->push_back(ei);
       } else if (s->use_graph) {
-        if (!s->gexec2[s->cur]) {
+        if (!s->gexec2[s->cur]) {  // both parities at once
           int rc = build_graph2(s, s->cur);
+          if (!rc && !s->gexec2[s->cur ^ 1]) rc = build_graph2(s, s->cur ^ 1);
           if (rc) return rc;
         }
         HC(hipGraphLaunch(s->gexec2[s->cur], s->stream));
