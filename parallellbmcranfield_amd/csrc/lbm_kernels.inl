@@ -12,6 +12,8 @@
 // MODE 0: step.  MODE 1: step + residual partials.  MODE 2: observe (no
 // state write; macros / F / METAF of step t to the KOut arrays).
 #pragma once
+#include <type_traits>
+
 #include "lbm_device.hpp"
 
 namespace lbmdev {
@@ -986,10 +988,13 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
   __syncthreads();
   if (side_owner) side_t2(0);
   __syncthreads();
-  // kept state: t+1 rows r-2 (directions 2, 5, 6), r-1; t+2 rows r-3
-  // (directions 2, 5, 6), r-2; classes and masks of rows r-1 (cc1) and r-2 (cc2)
-  R kd1[3] = {0, 0, 0}, kc1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  R kd2[3] = {0, 0, 0}, kc2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // kept rows in two-slot rings by row parity (the march body is unrolled
+  // by two, so every slot index is a constant and no row is copied): K1[P]
+  // t+1 of the rows r-2 (then r), K1[1-P] of row r-1; K2[P] t+2 of row r-3
+  // (then r-1), K2[1-P] of row r-2; classes and masks of rows r-1 (cc1) and
+  // r-2 (cc2)
+  R K1[2][9] = {{0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  R K2[2][9] = {{0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int cc1 = 0, cc2 = 0;
   uint32_t wc1 = 0, wc2 = 0;
   // neighbour waves' edge values (as k_two_steps' el*/er*), t+1 and t+2
@@ -1006,7 +1011,8 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
   R FA[9], FB[9];
   row_pulls(FA, r0, c0, w0);
   row_pulls(FB, min(r0 + 1, rlast), c1, w1);
-  auto row = [&](int it, R (&Fb)[9]) {
+  auto row = [&](int it, R (&Fb)[9], auto parity) {
+    constexpr int P = decltype(parity)::value;  // it & 1
     const int r = r0 + it;  // row of step t+1
     const int cb = c0;
     const uint32_t w = w0;
@@ -1030,7 +1036,10 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
     const int rn2 = min(r + 2, rlast), rn3 = min(r + 3, rlast);
     c2 = cls_at(rn3);
     w2 = mask_at(rn3);
-    R kn1[9];  // row r at t+1
+    // row r-2 at t+1 (directions 2, 5, 6) before its slot takes row r
+    const R kd1[3] = {K1[P][2], K1[P][5], K1[P][6]};
+    R(&kn1)[9] = K1[P];  // row r at t+1
+    R(&kc1)[9] = K1[1 - P];  // row r-1
     form_t1<S, COLL, WALLS>(kn1, Fb, cb & 0x7f, w, p);
     row_pulls(Fb, rn2, c1, w1);
     const int slot = it & 1;
@@ -1078,7 +1087,9 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
         nb[6] = r6;
         nb[7] = r7;
       }
-      R kn2[9];  // row rr at t+2
+      const R kd2[3] = {K2[P][2], K2[P][5], K2[P][6]};  // row r-3 at t+2
+      R(&kn2)[9] = K2[P];  // row rr at t+2
+      R(&kc2)[9] = K2[1 - P];  // row r-2
       form_next<S, COLL, WALLS>(kn2, nb, kc1, cc1 & 0x7f, wc1, p);
       if (lane == 0) {
         edge2[slot][wave][0][0] = kn2[3];
@@ -1143,11 +1154,6 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
       fl5 = ml5;
       fr3 = mr3;
       fr6 = mr6;
-      kd2[0] = kc2[2];
-      kd2[1] = kc2[5];
-      kd2[2] = kc2[6];
-#pragma unroll
-      for (int q = 0; q < 9; q++) kc2[q] = kn2[q];
       cc2 = cc1;
       wc2 = wc1;
     }
@@ -1157,18 +1163,13 @@ __device__ inline void three_steps_tile(const typename S::T* __restrict__ src,
     el5 = nl5;
     er3 = nr3;
     er6 = nr6;
-    kd1[0] = kc1[2];
-    kd1[1] = kc1[5];
-    kd1[2] = kc1[6];
-#pragma unroll
-    for (int q = 0; q < 9; q++) kc1[q] = kn1[q];
     cc1 = cb;
     wc1 = w;
   };
 #pragma unroll 1
   for (int it = 0; it < nit; it += 2) {
-    row(it, FA);
-    if (it + 1 < nit) row(it + 1, FB);
+    row(it, FA, std::integral_constant<int, 0>{});
+    if (it + 1 < nit) row(it + 1, FB, std::integral_constant<int, 1>{});
   }
 }
 
