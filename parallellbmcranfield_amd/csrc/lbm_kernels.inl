@@ -1203,17 +1203,20 @@ __global__ __launch_bounds__(TWO_COLS) void k_three_steps(
     three_steps_tile<S, COLL, false>(src, dst, cls, tl, p, lds);
 }
 
-// Plain steps with FP16 storage: the bulk workgroups take PAIRS of tiles
-// (2 x 256 consecutive cells) and each thread two neighbouring cells, so the
-// loads and stores of all-interior pairs move one 32-bit word per two cells:
-// twice the bytes per memory instruction (+15 % at 4097^2; the same pairing
-// with 64-bit words made FP32 7 % slower, so FP32 keeps one cell per thread).
+// Plain steps with FP16 storage: each bulk workgroup of 128 threads takes one
+// tile of 256 cells and each thread two neighbouring cells, so the loads and
+// stores of all-interior tiles move one 32-bit word per two cells: twice the
+// bytes per memory instruction, and as many workgroups as k_step launches
+// (MLUPS against k_step: 1025^2 65.6 k / 58.8 k, 2049^2 116.7 k / 111.8 k,
+// 4097^2 147.6 k / 144.9 k; pairs of tiles in workgroups of 256 measured
+// 65.3 k at 1025^2 and lost at 2049^2 and 4097^2; the same pairing with
+// 64-bit words made FP32 7 % slower, so FP32 keeps one cell per thread).
 // A pull with cx = +-1 starts on an odd cell; its two halves come from the
 // two aligned words around it.  Same arithmetic per cell as k_step
 // (macro_dev and collide_dev on the same floats), so results are identical.
-// Boundary workgroups and pairs with a mixed or empty tile run k_step's
-// per-cell code.  Launch: nb_bnd + nb_bulk / 2 workgroups of 256; XCD
-// grouping by runs of G / 2 pairs (G even, nb_bulk a multiple of 8 G).
+// Boundary workgroups (256 list nodes each) and mixed tiles run k_step's
+// per-cell code, two cells per thread 128 apart.  Launch: nb_bnd + nb_bulk
+// workgroups of 128; XCD grouping by runs of G tiles as k_step.
 template <class S>
 struct Pair;
 template <>
@@ -1244,7 +1247,7 @@ struct Pair<StF16S> : Pair<StF16> {
 };
 
 template <class S, int COLL>
-__global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restrict__ src,
+__global__ __launch_bounds__(128) void k_step_pair(const typename S::T* __restrict__ src,
                                                    typename S::T* __restrict__ dst,
                                                    const uint8_t* __restrict__ cls,
                                                    const uint8_t* __restrict__ tflag,
@@ -1258,8 +1261,9 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
   const int nbb = p.nb_bnd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if ((int)blockIdx.x < nbb) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nbn && !(bn[i].flags & BN_OUTLET)) {
+    for (int e = 0; e < 256; e += 128) {
+      const int i = blockIdx.x * 256 + e + (int)threadIdx.x;
+      if (i >= nbn || (bn[i].flags & BN_OUTLET)) continue;
       const BNode& nd = bn[i];
       R F[9];
       pull_masked<S>(F, nd, src, p);
@@ -1273,11 +1277,11 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
     }
     return;
   }
-  unsigned q = blockIdx.x - nbb;
-  if (p.xcd_group > 1) q = xcd_group_block(q, (unsigned)(p.xcd_group / 2));
-  const unsigned t0 = 2 * q;
-  if (tflag[t0] == 1 && tflag[t0 + 1] == 1) {
-    const int c0 = p.bulk_begin + (int)(t0 * 256 + 2 * threadIdx.x);  // even
+  unsigned b = blockIdx.x - nbb;
+  if (p.xcd_group > 0) b = xcd_group_block(b, (unsigned)p.xcd_group);
+  const int tf = tflag[b];
+  if (tf == 1) {
+    const int c0 = p.bulk_begin + (int)(b * 256 + 2 * threadIdx.x);  // even
     const W* sw = reinterpret_cast<const W*>(src);
     R F0[9], F1[9];
 #pragma unroll
@@ -1309,12 +1313,10 @@ __global__ __launch_bounds__(256) void k_step_pair(const typename S::T* __restri
     }
     return;
   }
-  for (int h = 0; h < 2; h++) {  // per-cell path of k_step for each tile of the pair
-    const unsigned b = t0 + h;
-    const int tf = tflag[b];
-    if (tf == 0) continue;
-    const int idx = p.bulk_begin + (int)(b * 256 + threadIdx.x);
-    const int c = tf == 1 ? 1 : (idx < p.bulk_end ? (int)cls[idx] : 0);
+  if (tf == 0) return;
+  for (int h = 0; h < 256; h += 128) {  // per-cell path of k_step, two cells 128 apart
+    const int idx = p.bulk_begin + (int)(b * 256 + h + threadIdx.x);
+    const int c = idx < p.bulk_end ? (int)cls[idx] : 0;
     if (c != 1 && c != 4) continue;
     R F[9];
     pull_cell<S>(F, src, p, idx, c);

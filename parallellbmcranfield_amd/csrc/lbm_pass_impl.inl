@@ -31,11 +31,11 @@ static hipError_t fused_launch(const Launch& L) {
   const T* src = (const T*)L.src;
   T* dst = (T*)L.dst;
   if constexpr ((std::is_same_v<S, StF16> || std::is_same_v<S, StF16S>) && MODE == 0) {
-    if (L.pairs) {  // paired-cell plain step (nb_bulk even, grouping by G / 2 pairs)
-      const int grid2 = L.kp.nb_bnd + L.nb_bulk / 2;
-      if (grid2 <= 0) return hipSuccess;
-      k_step_pair<S, COLL><<<grid2, 256, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.bn, L.nbn,
-                                                        L.kp, L.out);
+    if (L.pairs) {  // paired-cell plain step: one tile per workgroup of 128 threads
+      const int grid = L.kp.nb_bnd + L.nb_bulk;
+      if (grid <= 0) return hipSuccess;
+      k_step_pair<S, COLL><<<grid, 128, 0, L.stream>>>(src, dst, L.cls, L.tflag, L.bn, L.nbn,
+                                                       L.kp, L.out);
       return hipGetLastError();
     }
   }

@@ -87,12 +87,11 @@ Launch make_launch(lbm_solver* s, int src_buf) {
     L.nb_bulk = s->nb_list;
     L.kp.tile_list = s->d_tile_list;
   }
-  // FP16, fused, single lattice of >= 16384 tiles (at 1025^2, 4.4 k tiles,
-  // half as many workgroups under-fill the chip: 18.4 -> 21.2 us), even XCD
-  // grouping (the pair grouping must stay a permutation): paired-cell steps
-  L.pairs = (s->prec == LBM_PREC_F16 || s->prec == LBM_PREC_F16_SR) && s->nb_bulk >= 16384 && s->fused && !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
-            s->kp.xcd_group % 2 == 0 && s->nb_bulk % (8 * s->kp.xcd_group) == 0 &&
-            !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
+  // FP16, fused, single lattice, grouped XCD order (the grouping must stay a
+  // permutation of the tiles): paired-cell plain steps
+  L.pairs = (s->prec == LBM_PREC_F16 || s->prec == LBM_PREC_F16_SR) && s->fused &&
+            !s->d_tile_list && !(s->kp.flags & KF_XCD_REMAP) &&
+            s->nb_bulk % (8 * s->kp.xcd_group) == 0 && !(s->opt.tune_flags & LBM_TUNE_NO_PAIRS);
   L.out.rho = s->d_rho;
   L.out.u = s->d_u;
   L.out.v = s->d_v;

@@ -597,11 +597,11 @@ def test_paired_cell_kernel_is_bit_identical(coll, prec):
     the per-cell kernel (LBM_TUNE_NO_PAIRS) gives the same bits, on a cavity
     and on a channel whose cylinder makes mixed tiles, with residual and
     observe steps in between (with stochastic rounding too: both draw the
-    same random bits for a value).  The paired kernel runs on lattices of at
-    least 16384 tiles."""
+    same random bits for a value).  One tile of 256 cells per workgroup of
+    128 threads, on lattices from about 110 to 17 k tiles."""
     print("pairs", coll, prec, flush=True)
-    # lattices of >= 16384 tiles of 256 cells (the kernel's threshold)
-    for make in (lambda: lbm.Case.cavity(2050),
+    for make in (lambda: lbm.Case.cavity(2050), lambda: lbm.Case.cavity(301),
+                 lambda: lbm.Case.channel(300, 90, 80.2, 45.3, 12.5),
                  lambda: lbm.Case.channel(4000, 1050, 1200.3, 530.1, 60.5)):
         case = make()
         out = []
