@@ -136,37 +136,87 @@ def run_single(spec, opts=None, chunk: int = 5000, fields: bool = False) -> dict
     return out
 
 
-def autotune(case: Case, opts, instances=None, steps: int = 200,
-             blocks=(256, 512, 1024), groups=(1, 2, 4, 8, 16)) -> tuple[dict, list]:
+def _time_steps(case: Case, o, instances, steps: int) -> tuple[float, dict, dict]:
+    """Seconds per plain step of `steps` steps after 24 untimed ones (graphs
+    are captured on first use), and the solver's multi-step layout."""
+    s = Solver(case, o, instances=instances) if instances else Solver(case, o)
+    try:
+        two, three = s.two_step_info(), s.three_step_info()
+        s.step(24)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        return (time.perf_counter() - t0) / steps, two, three
+    finally:
+        s.close()
+
+
+def autotune(case: Case, opts, instances=None, steps: int = 198,
+             blocks=(256, 512, 1024), groups=(1, 2, 4, 8, 16),
+             two_rows=(0,), three_rows=(0,)) -> tuple[dict, list]:
     """Time `steps` plain steps for each launch shape on this lattice and
-    return the fastest (and every timing).  A bulk tile is `block_size`
-    consecutive cells of a storage row (one workgroup); `xcd_group` runs of
-    that many adjacent tiles go to one XCD (its L2 serves the +-1 pulls the
-    tiles share); FP16 storage also tries one cell per thread against the
-    paired-cell kernel.  Block 256 runs the fused step kernel, larger blocks
-    split bulk and boundary launches.  Results are identical for every shape."""
-    from . import PREC_F16, PREC_F16_SR, TUNE_NO_PAIRS
+    return the fastest (and every timing).  Results are identical for every
+    shape; only the time differs.
+
+    Stage 1, single steps (multi-step launches off while timing): a bulk tile
+    is `block_size` consecutive cells of a storage row (one workgroup);
+    `xcd_group` runs of that many adjacent tiles go to one XCD (its L2 serves
+    the +-1 pulls the tiles share); FP16 storage also tries one cell per
+    thread against the paired-cell kernel.  Block 256 runs the fused step
+    kernel, larger blocks split bulk and boundary launches.
+
+    Stage 2, single lattices only (the batched solver runs single steps over
+    its tile list): at the best stage-1 shape, the tile shape of multi-step
+    launches: two plain steps per launch with `two_rows` output rows per
+    256-column tile, three per launch with `three_rows` (0: the library's
+    own pick, which may be single steps where multi-step launches lose), each
+    against single steps.  `best` holds the options to copy (block_size,
+    xcd_group, two_step_rows, three_step_rows, tune_flags) and what ran."""
+    from . import PREC_F16, PREC_F16_SR, TUNE_NO_PAIRS, TUNE_NO_THREE_STEP, TUNE_NO_TWO_STEP
     fp16 = opts.precision in (PREC_F16, PREC_F16_SR)
+    keep = opts.tune_flags & ~(TUNE_NO_PAIRS | TUNE_NO_TWO_STEP | TUNE_NO_THREE_STEP)
+
+    def variant(bs, g, flags, rows2=0, rows3=0):
+        o = type(opts).from_buffer_copy(opts)
+        o.block_size, o.xcd_group = bs, g
+        o.tune_flags = keep | flags
+        o.two_step_rows, o.three_step_rows = rows2, rows3
+        o.use_stop_rule = 0
+        o.resid_kind = 3
+        o.crit_v = -1.0  # no instance stops while timing
+        return o
+
+    def record(o, dt, two, three, stage):
+        steps_per_launch = 3 if three["cells"] else (2 if two["cells"] else 1)
+        return dict(stage=stage, block_size=o.block_size, xcd_group=o.xcd_group,
+                    pairs=fp16 and o.block_size == 256 and not (o.tune_flags & TUNE_NO_PAIRS),
+                    two_step_rows=o.two_step_rows, three_step_rows=o.three_step_rows,
+                    tune_flags=o.tune_flags, steps_per_launch=steps_per_launch,
+                    tile_rows=three["rows"] if three["cells"] else (two["rows"] if two["cells"] else 0),
+                    ms_per_step=dt * 1e3)
+
+    single = TUNE_NO_TWO_STEP | TUNE_NO_THREE_STEP
     results = []
     for bs in blocks:
         for g in groups:
             for pairs in ((True, False) if fp16 and bs == 256 else (True,)):
-                o = type(opts).from_buffer_copy(opts)
-                o.block_size, o.xcd_group = bs, g
-                o.tune_flags = (opts.tune_flags & ~TUNE_NO_PAIRS) | (0 if pairs else TUNE_NO_PAIRS)
-                o.use_stop_rule = 0
-                o.resid_kind = 3
-                o.crit_v = -1.0  # no instance stops while timing
-                s = Solver(case, o, instances=instances) if instances else Solver(case, o)
-                s.step(20)
-                s.sync()
-                t0 = time.perf_counter()
-                s.step(steps)
-                s.sync()
-                dt = (time.perf_counter() - t0) / steps
-                results.append(dict(block_size=bs, xcd_group=g, pairs=pairs and fp16 and bs == 256,
-                                    ms_per_step=dt * 1e3))
-                s.close()
+                o = variant(bs, g, single | (0 if pairs else TUNE_NO_PAIRS))
+                dt, two, three = _time_steps(case, o, instances, steps)
+                results.append(record(o, dt, two, three, 1))
+    best1 = min(results, key=lambda r: r["ms_per_step"])
+    if not instances:
+        base = best1["tune_flags"] & ~single
+        for r2 in two_rows:
+            o = variant(best1["block_size"], best1["xcd_group"], base | TUNE_NO_THREE_STEP, rows2=r2)
+            dt, two, three = _time_steps(case, o, None, steps)
+            if two["cells"]:
+                results.append(record(o, dt, two, three, 2))
+        for r3 in three_rows:
+            o = variant(best1["block_size"], best1["xcd_group"], base, rows3=r3)
+            dt, two, three = _time_steps(case, o, None, steps)
+            if three["cells"]:
+                results.append(record(o, dt, two, three, 2))
     best = min(results, key=lambda r: r["ms_per_step"])
     return best, results
 
