@@ -166,13 +166,14 @@ def autotune(case: Case, opts, instances=None, steps: int = 198,
     thread against the paired-cell kernel.  Block 256 runs the fused step
     kernel, larger blocks split bulk and boundary launches.
 
-    Stage 2, single lattices only (the batched solver runs single steps over
-    its tile list): at the best stage-1 shape, the tile shape of multi-step
+    Stage 2: at the best stage-1 shape, the tile shape of multi-step
     launches: two plain steps per launch with `two_rows` output rows per
-    256-column tile, three per launch with `three_rows` (0: the library's
-    own pick, which may be single steps where multi-step launches lose), each
-    against single steps.  `best` holds the options to copy (block_size,
-    xcd_group, two_step_rows, three_step_rows, tune_flags) and what ran."""
+    256-column tile, three per launch with `three_rows` (single lattices
+    only; 0: the library's own pick, which may be single steps where
+    multi-step launches lose), each against single steps.  A variant whose
+    solver runs no such launches is dropped.  `best` holds the options to
+    copy (block_size, xcd_group, two_step_rows, three_step_rows, tune_flags)
+    and what ran."""
     from . import PREC_F16, PREC_F16_SR, TUNE_NO_PAIRS, TUNE_NO_THREE_STEP, TUNE_NO_TWO_STEP
     fp16 = opts.precision in (PREC_F16, PREC_F16_SR)
     keep = opts.tune_flags & ~(TUNE_NO_PAIRS | TUNE_NO_TWO_STEP | TUNE_NO_THREE_STEP)
@@ -205,18 +206,17 @@ def autotune(case: Case, opts, instances=None, steps: int = 198,
                 dt, two, three = _time_steps(case, o, instances, steps)
                 results.append(record(o, dt, two, three, 1))
     best1 = min(results, key=lambda r: r["ms_per_step"])
-    if not instances:
-        base = best1["tune_flags"] & ~single
-        for r2 in two_rows:
-            o = variant(best1["block_size"], best1["xcd_group"], base | TUNE_NO_THREE_STEP, rows2=r2)
-            dt, two, three = _time_steps(case, o, None, steps)
-            if two["cells"]:
-                results.append(record(o, dt, two, three, 2))
-        for r3 in three_rows:
-            o = variant(best1["block_size"], best1["xcd_group"], base, rows3=r3)
-            dt, two, three = _time_steps(case, o, None, steps)
-            if three["cells"]:
-                results.append(record(o, dt, two, three, 2))
+    base = best1["tune_flags"] & ~single
+    for r2 in two_rows:
+        o = variant(best1["block_size"], best1["xcd_group"], base | TUNE_NO_THREE_STEP, rows2=r2)
+        dt, two, three = _time_steps(case, o, instances, steps)
+        if two["cells"]:
+            results.append(record(o, dt, two, three, 2))
+    for r3 in (() if instances else three_rows):
+        o = variant(best1["block_size"], best1["xcd_group"], base, rows3=r3)
+        dt, two, three = _time_steps(case, o, None, steps)
+        if three["cells"]:
+            results.append(record(o, dt, two, three, 2))
     best = min(results, key=lambda r: r["ms_per_step"])
     return best, results
 

@@ -51,3 +51,16 @@ def test_autotune_times_launch_and_tile_shapes(prec):
         s.close()
     for a, b in zip(*fields):
         assert np.array_equal(a, b)
+
+
+def test_autotune_on_a_packed_lattice_times_two_step_tiles():
+    print("autotune batched", flush=True)
+    specs = SW.random_cavities(6, seed=4, side=(65, 129))
+    packed, _ = SW.build_batch(specs)
+    inst = [(s["nu"], s["lid"]) for s in specs]
+    o = SW.sweep_options()
+    best, tried = SW.autotune(packed, o, instances=inst, steps=24, blocks=(256,), groups=(1, 4),
+                              two_rows=(4, 8), three_rows=(8,))
+    stage2 = [r for r in tried if r["stage"] == 2]
+    assert sorted((r["steps_per_launch"], r["tile_rows"]) for r in stage2) == [(2, 4), (2, 8)]
+    assert best == min(tried, key=lambda r: r["ms_per_step"])

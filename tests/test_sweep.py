@@ -58,7 +58,7 @@ def _fake_timer(calls):
         no2 = bool(o.tune_flags & lbm.TUNE_NO_TWO_STEP)
         no3 = bool(o.tune_flags & lbm.TUNE_NO_THREE_STEP)
         r2 = 0 if no2 else o.two_step_rows
-        r3 = 0 if no3 else o.three_step_rows
+        r3 = 0 if no3 or instances is not None else o.three_step_rows  # single lattices only
         two = dict(cells=1000 if r2 else 0, tiles=4, rows=r2, band=10)
         three = dict(cells=1000 if r3 else 0, tiles=4, rows=r3, band=10)
         t = 1e-5 * (1 + o.block_size / 256) + 1e-7 * abs(o.xcd_group - 4)
@@ -97,16 +97,18 @@ def test_autotune_stages_on_a_single_lattice(monkeypatch):
     assert o.tune_flags == lbm.TUNE_NO_GRAPH and o.block_size == 0  # caller's options untouched
 
 
-def test_autotune_batched_lattices_time_single_steps_only(monkeypatch):
+def test_autotune_batched_lattices_take_two_step_tiles(monkeypatch):
     calls = []
     monkeypatch.setattr(SW, "_time_steps", _fake_timer(calls))
     o = lbm.make_options(collision=lbm.MRT, uavg=0.05, rho_ini=1.0, viscosity=0.03,
                          precision=lbm.PREC_F32)
     best, tried = SW.autotune(None, o, instances=[object()], steps=12, blocks=(256,),
-                              groups=(1, 2, 4), two_rows=(4,), three_rows=(8,))
-    assert len(tried) == 3 and all(r["stage"] == 1 and not r["pairs"] for r in tried)
-    assert all(c[5] for c in calls)
-    assert best["xcd_group"] == 4 and best["steps_per_launch"] == 1
+                              groups=(1, 2, 4), two_rows=(4, 6), three_rows=(8,))
+    assert [r["stage"] for r in tried] == [1, 1, 1, 2, 2]
+    assert all(c[5] for c in calls)  # every solver built with the instances
+    assert [(r["steps_per_launch"], r["tile_rows"]) for r in tried if r["stage"] == 2] == \
+        [(2, 4), (2, 6)]  # no three-step variant is tried on a batched lattice
+    assert best["steps_per_launch"] == 2 and best["two_step_rows"] == 6
     # stage 2 keeps a variant only if the library ran that many steps per launch
     calls.clear()
     best, tried = SW.autotune(None, o, steps=12, blocks=(256,), groups=(4,),
