@@ -1448,8 +1448,19 @@ static int step_impl(lbm_solver* s, int nsteps, int* done, std::vector<hipEvent_
     };
     const bool feval = forces_at(s->iter);
     const bool eval = reval || feval;
+    // The stop rule reads only evaluated residuals, which change at
+    // evaluation steps alone, and a run of plain steps holds none: the rule
+    // cannot end the run inside it, so runs of two- and three-step launches
+    // form under a stop rule too (the iteration cap aside), on single
+    // lattices that have them.  Otherwise a stop rule keeps single launches:
+    // a batched lattice's band lists cover every instance, stopped or not, so
+    // in the sweep's tail multi-step launches cost more than they save
+    // (256-cavity sweep 25.6 k against 51.4 k MLUPS aggregate), and a small
+    // lattice's launches are faster direct than replayed two to a graph (129^2
+    // under its stop rule 8.1 against 5.7 us per step).
+    const bool runs_ok = !stop_can_trigger || (!s->ninst && (s->two || s->three));
     const bool plain_pair = t + 1 < nsteps && !eval && !evals_at(s->iter + 1) &&
-                            !forces_at(s->iter + 1) && !stop_can_trigger &&
+                            !forces_at(s->iter + 1) && runs_ok &&
                             (!o.use_stop_rule || s->iter + 2 <= o.max_iterations);
     const bool sample_pair = evs && (t % sample_every == 0 || (t + 1) % sample_every == 0);
     // Three plain steps in a row with three-step launches: one pass over the state.

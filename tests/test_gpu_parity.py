@@ -749,6 +749,43 @@ TWO_STEP_GEOMS = {
 }
 
 
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16])
+def test_stop_rule_with_multi_step_launches(prec):
+    """Under a stop rule on the velocity residual (evaluated every 7 steps),
+    the plain steps between evaluations run as two- and three-step launches
+    and the run stops at the same evaluation, with the same fields and
+    residual history, as with every step its own launch."""
+    print("stop rule multi-step", prec, flush=True)
+    case = lbm.Case.cavity(150)
+    base = dict(collision=lbm.MRT, uavg=0.05, rho_ini=1.0, viscosity=0.03, precision=prec,
+                init=lbm.INIT_REST, resid_kind=lbm.RESID_VELOCITY, resid_every=7)
+    single = lbm.TUNE_NO_TWO_STEP | lbm.TUNE_NO_THREE_STEP
+    s = lbm.Solver(case, lbm.make_options(use_stop_rule=0, tune_flags=single, **base))
+    s.step(400)
+    its, vals = s.residual_history()
+    s.close()
+    # met by the evaluation mid-run at the latest (rounded up to the option's FP32)
+    crit = float(np.nextafter(np.float32(vals[len(vals) // 2, 0]), np.float32(np.inf)))
+    runs = []
+    for rows2, rows3, flags in ((0, 0, single), (5, 0, lbm.TUNE_NO_THREE_STEP), (5, 5, 0),
+                                (16, 16, lbm.TUNE_NO_GRAPH)):
+        o = lbm.make_options(use_stop_rule=1, crit_v=crit, crit_rho=-1.0, max_iterations=100000,
+                             two_step_rows=rows2, three_step_rows=rows3, tune_flags=flags, **base)
+        s = lbm.Solver(case, o)
+        assert (s.two_step_info()["cells"] > 0) == (rows2 > 0)
+        assert (s.three_step_info()["cells"] > 0) == (rows3 > 0)
+        s.step(1000)
+        runs.append((s.iter, s.macro(), s.residual_history()))
+        s.close()
+    it0 = runs[0][0]
+    assert it0 % 7 == 0 and 0 < it0 < 400  # stopped after an evaluation, before the cap
+    for it, m, (hi, hv) in runs[1:]:
+        assert it == it0
+        for a, b in zip(runs[0][1], m):
+            assert np.array_equal(a, b)
+        assert np.array_equal(hi, runs[0][2][0]) and np.array_equal(hv, runs[0][2][1])
+
+
 @pytest.mark.parametrize("geom", sorted(TWO_STEP_GEOMS))
 @pytest.mark.parametrize("prec,coll", [(lbm.PREC_F64, lbm.BGKW), (lbm.PREC_F64, lbm.MRT),
                                        (lbm.PREC_F32, lbm.MRT), (lbm.PREC_F32, lbm.TRT),
@@ -784,6 +821,43 @@ def test_two_step_launches_are_bit_identical(prec, coll, geom):
             assert np.array_equal(a, b), r[5]
         assert np.array_equal(base[1], r[1]) and np.array_equal(base[2], r[2]), r[5]
         assert np.array_equal(base[3], r[3]) and np.array_equal(base[4], r[4]), r[5]
+
+
+@pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16])
+def test_stop_rule_with_multi_step_launches(prec):
+    """Under a stop rule on the velocity residual (evaluated every 7 steps),
+    the plain steps between evaluations run as two- and three-step launches
+    and the run stops at the same evaluation, with the same fields and
+    residual history, as with every step its own launch."""
+    print("stop rule multi-step", prec, flush=True)
+    case = lbm.Case.cavity(150)
+    base = dict(collision=lbm.MRT, uavg=0.05, rho_ini=1.0, viscosity=0.03, precision=prec,
+                init=lbm.INIT_REST, resid_kind=lbm.RESID_VELOCITY, resid_every=7)
+    single = lbm.TUNE_NO_TWO_STEP | lbm.TUNE_NO_THREE_STEP
+    s = lbm.Solver(case, lbm.make_options(use_stop_rule=0, tune_flags=single, **base))
+    s.step(400)
+    its, vals = s.residual_history()
+    s.close()
+    # met by the evaluation mid-run at the latest (rounded up to the option's FP32)
+    crit = float(np.nextafter(np.float32(vals[len(vals) // 2, 0]), np.float32(np.inf)))
+    runs = []
+    for rows2, rows3, flags in ((0, 0, single), (5, 0, lbm.TUNE_NO_THREE_STEP), (5, 5, 0),
+                                (16, 16, lbm.TUNE_NO_GRAPH)):
+        o = lbm.make_options(use_stop_rule=1, crit_v=crit, crit_rho=-1.0, max_iterations=100000,
+                             two_step_rows=rows2, three_step_rows=rows3, tune_flags=flags, **base)
+        s = lbm.Solver(case, o)
+        assert (s.two_step_info()["cells"] > 0) == (rows2 > 0)
+        assert (s.three_step_info()["cells"] > 0) == (rows3 > 0)
+        s.step(1000)
+        runs.append((s.iter, s.macro(), s.residual_history()))
+        s.close()
+    it0 = runs[0][0]
+    assert it0 % 7 == 0 and 0 < it0 < 400  # stopped after an evaluation, before the cap
+    for it, m, (hi, hv) in runs[1:]:
+        assert it == it0
+        for a, b in zip(runs[0][1], m):
+            assert np.array_equal(a, b)
+        assert np.array_equal(hi, runs[0][2][0]) and np.array_equal(hv, runs[0][2][1])
 
 
 @pytest.mark.parametrize("geom", sorted(TWO_STEP_GEOMS))
