@@ -125,8 +125,14 @@ typedef struct lbm_options {
  * slower than the hardware's round-robin order on the 4097^2 lattice).
  * LBM_TUNE_SPLIT_BOUNDARY: boundary-list nodes in their own launch after the
  * bulk pass instead of as the first workgroups of one fused launch.
- * LBM_TUNE_NO_GRAPH: launch every step directly instead of replaying captured
- * two-step HIP graphs for steps without residual evaluation.
+ * LBM_TUNE_GRAPH: replay captured HIP graphs for runs of plain steps (two
+ * single steps, or a two- or three-step launch with its band launches)
+ * instead of launching directly.  Off by default: direct launches measured
+ * faster at every size (per step, direct against graph: 129^2 4.2 against
+ * 7.7 us, 1025^2 FP16 12.8 against 15.3 us, 4097^2 FP32 116.9 against
+ * 118.1 us) with identical results.
+ * LBM_TUNE_NO_GRAPH: direct launches even where LBM_TUNE_GRAPH is set (the
+ * default without it).
  * LBM_TUNE_NO_PAIRS: FP16 storage without the paired-cell plain-step kernel
  * (one cell per thread, 16-bit loads and stores).
  * LBM_TUNE_NO_TILE_WALLS: every boundary node in the boundary list (by
@@ -151,7 +157,7 @@ typedef struct lbm_options {
  * one for the rest. */
 enum { LBM_TUNE_XCD_REMAP = 1, LBM_TUNE_SPLIT_BOUNDARY = 2, LBM_TUNE_NO_GRAPH = 4,
        LBM_TUNE_NO_PAIRS = 8, LBM_TUNE_NO_TILE_WALLS = 16, LBM_TUNE_NO_TWO_STEP = 32,
-       LBM_TUNE_NO_THREE_STEP = 64 };
+       LBM_TUNE_NO_THREE_STEP = 64, LBM_TUNE_GRAPH = 128 };
 
 typedef struct lbm_case lbm_case;
 typedef struct lbm_solver lbm_solver;

@@ -29,7 +29,7 @@ INIT_REFERENCE, INIT_REST, INIT_FIELD, INIT_FIELD_NEQ = 0, 1, 2, 3
 RESID_REFERENCE, RESID_FULL, RESID_VELOCITY, RESID_NONE = 0, 1, 2, 3
 FORMAT_CSV, FORMAT_TECPLOT = 1, 2
 (TUNE_XCD_REMAP, TUNE_SPLIT_BOUNDARY, TUNE_NO_GRAPH, TUNE_NO_PAIRS, TUNE_NO_TILE_WALLS,
- TUNE_NO_TWO_STEP, TUNE_NO_THREE_STEP) = (1, 2, 4, 8, 16, 32, 64)
+ TUNE_NO_TWO_STEP, TUNE_NO_THREE_STEP, TUNE_GRAPH) = (1, 2, 4, 8, 16, 32, 64, 128)
 
 # Every function include/lbm.h declares (checked by tests/test_abi.py).
 EXPORTS = [

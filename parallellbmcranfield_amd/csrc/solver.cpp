@@ -789,7 +789,7 @@ static int solver_create(const lbm_case* c, const lbm_options* o, const double* 
   kp.xcd_group = G;
   if (o->tune_flags & LBM_TUNE_XCD_REMAP) kp.flags |= KF_XCD_REMAP;
   s->fused = BS == 256 && !(o->tune_flags & LBM_TUNE_SPLIT_BOUNDARY);
-  s->use_graph = !(o->tune_flags & LBM_TUNE_NO_GRAPH);
+  s->use_graph = (o->tune_flags & LBM_TUNE_GRAPH) && !(o->tune_flags & LBM_TUNE_NO_GRAPH);
   s->nb_bnd = (s->nbn + 255) / 256;
   s->nb_out = (s->norows + 63) / 64;
   // per-tile summary of the classes (tiles of BS cells in launch order)

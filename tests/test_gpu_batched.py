@@ -343,7 +343,7 @@ def test_batched_two_step_equals_single_steps(prec, coll):
     packed = lbm.Case.pack(cases)
     inst = [(nu, u) for *_, nu, u in SET]
     out = []
-    for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (8, 0), (3, lbm.TUNE_NO_GRAPH)):
+    for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (8, 0), (3, lbm.TUNE_GRAPH)):
         s = lbm.Solver(packed, batch_opts(prec, coll, resid_every=10, crit_v=1e-3,
                                           use_stop_rule=1, two_step_rows=rows,
                                           tune_flags=flags), instances=inst)

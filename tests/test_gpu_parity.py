@@ -397,8 +397,9 @@ def test_cavity_129_bgk_fp64_1000_steps(init_rest):
 @pytest.mark.parametrize("prec", [lbm.PREC_F64, lbm.PREC_F32, lbm.PREC_F16_SR])
 @pytest.mark.parametrize("geom", ["cavity", "channel_open_outlet"])
 def test_graph_replay_matches_eager_launches(prec, geom):
-    """Plain steps replay a captured two-step HIP graph; every replay, with
-    eager work and host reads in between, equals eager launches bit for bit."""
+    """With LBM_TUNE_GRAPH plain steps replay a captured two-step HIP graph;
+    every replay, with eager work and host reads in between, equals the
+    default direct launches bit for bit."""
     print("graph", prec, geom, flush=True)
     if geom == "cavity":
         nodes, conn = meshgen.cavity(48).arrays()
@@ -407,9 +408,9 @@ def test_graph_replay_matches_eager_launches(prec, geom):
         nodes, conn = meshgen.channel(60, 20).arrays()
         kw = dict(collision=lbm.BGKW, uavg=0.05, rho_ini=1.0, viscosity=0.05, outlet_profile=2)
     case = lbm.Case.from_arrays(nodes, conn)
-    sg = lbm.Solver(case, lbm.make_options(**kw, precision=prec, resid_kind=lbm.RESID_NONE))
-    se = lbm.Solver(case, lbm.make_options(**kw, precision=prec, resid_kind=lbm.RESID_NONE,
-                                           tune_flags=lbm.TUNE_NO_GRAPH))
+    sg = lbm.Solver(case, lbm.make_options(**kw, precision=prec, resid_kind=lbm.RESID_NONE,
+                                           tune_flags=lbm.TUNE_GRAPH))
+    se = lbm.Solver(case, lbm.make_options(**kw, precision=prec, resid_kind=lbm.RESID_NONE))
     for chunk in (1, 2, 7, 8, 3, 16, 5):
         sg.step(chunk)
         se.step(chunk)
@@ -768,7 +769,7 @@ def test_stop_rule_with_multi_step_launches(prec):
     crit = float(np.nextafter(np.float32(vals[len(vals) // 2, 0]), np.float32(np.inf)))
     runs = []
     for rows2, rows3, flags in ((0, 0, single), (5, 0, lbm.TUNE_NO_THREE_STEP), (5, 5, 0),
-                                (16, 16, lbm.TUNE_NO_GRAPH)):
+                                (16, 16, lbm.TUNE_GRAPH)):
         o = lbm.make_options(use_stop_rule=1, crit_v=crit, crit_rho=-1.0, max_iterations=100000,
                              two_step_rows=rows2, three_step_rows=rows3, tune_flags=flags, **base)
         s = lbm.Solver(case, o)
@@ -801,7 +802,7 @@ def test_two_step_launches_are_bit_identical(prec, coll, geom):
     runs = []
     no3 = lbm.TUNE_NO_THREE_STEP
     for rows, flags in ((0, lbm.TUNE_NO_TWO_STEP), (16, no3), (5, no3), (1, no3),
-                        (5, lbm.TUNE_NO_GRAPH | no3)):
+                        (5, lbm.TUNE_GRAPH | no3)):
         o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
                              precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_FULL,
                              resid_every=7, trt_lambda=0.25, two_step_rows=rows,
@@ -842,7 +843,7 @@ def test_stop_rule_with_multi_step_launches(prec):
     crit = float(np.nextafter(np.float32(vals[len(vals) // 2, 0]), np.float32(np.inf)))
     runs = []
     for rows2, rows3, flags in ((0, 0, single), (5, 0, lbm.TUNE_NO_THREE_STEP), (5, 5, 0),
-                                (16, 16, lbm.TUNE_NO_GRAPH)):
+                                (16, 16, lbm.TUNE_GRAPH)):
         o = lbm.make_options(use_stop_rule=1, crit_v=crit, crit_rho=-1.0, max_iterations=100000,
                              two_step_rows=rows2, three_step_rows=rows3, tune_flags=flags, **base)
         s = lbm.Solver(case, o)
@@ -876,7 +877,7 @@ def test_three_step_launches_are_bit_identical(prec, coll, geom):
     case = make()
     runs = []
     for rows3, rows2, flags in ((0, 0, lbm.TUNE_NO_TWO_STEP), (16, 0, 0), (5, 0, 0), (1, 0, 0),
-                                (5, 5, 0), (5, 0, lbm.TUNE_NO_GRAPH), (1, 16, lbm.TUNE_NO_GRAPH)):
+                                (5, 5, 0), (5, 0, lbm.TUNE_GRAPH), (1, 16, lbm.TUNE_GRAPH)):
         o = lbm.make_options(collision=coll, uavg=0.05, rho_ini=1.0, viscosity=0.03,
                              precision=prec, init=lbm.INIT_REST, resid_kind=lbm.RESID_FULL,
                              resid_every=7, trt_lambda=0.25, two_step_rows=rows2,
